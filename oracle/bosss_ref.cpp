@@ -1,0 +1,965 @@
+// bosss_ref -- CPU restatement of the BoSSS DG spatial-operator evaluator.
+//
+// TEST INFRASTRUCTURE ONLY.  This library is the oracle the HIP library is
+// checked against.  Only tests/, __graft_entry__.smoke() and the cpu_baseline
+// leg of bench.py may load it; nothing in the product (bosss_amd/) links,
+// imports or calls it.
+//
+// It follows the reference algorithm (all paths relative to the BoSSS tree):
+//   * chunk loop of at most 20 cells/edges     Quadrature.cs:33-38, 354-531
+//   * volume quadrature                        NECQuadratureVolume.cs:328-798
+//       modal->nodal: u(j,k) = sum_n Scaling_j c_{j,n} phi_n(xi_k)   DGField_Evaluation.cs:38-129
+//       nonlinear fluxes scaled by -1          NECQuadratureVolume.cs:452
+//       flux * invJ (affine)                   NECQuadratureVolume.cs:584-627
+//       contraction with w*grad(phi), w*phi    NECQuadratureVolume.cs:689-716
+//       * detJ * Scaling                       NECQuadratureVolume.cs:661-667, 720-733
+//   * edge quadrature                          NECQuadratureEdge.cs:642-1427
+//       OUT flux = -IN flux                    NECQuadratureEdge.cs:1000
+//       * sqrtGram, grad-flux * invJ per side  NECQuadratureEdge.cs:1189-1245
+//       trace contraction, * Scaling per side  NECQuadratureEdge.cs:1253-1420
+//       scatter to IN and OUT cell             NECQuadratureEdge.cs:326-410
+//   * edge/volume forms probed with unit test values   NonlinFormVectorizer.cs:77-140, 370-430
+//   * forms: SIPLaplace (ipLaplace.cs:30-269), ScalarTransportFlux3D upwind
+//     (ScalarTransport/TransportFlux.cs:112-190, NonlinearFlux.cs:85-190),
+//     Euler components (EulerDensity/Momentum/EnergyComponent.cs), StateVector.cs,
+//     IdealGas.cs:81-134, HLLCFlux.cs:48-86 (+ HLLC{Density,Momentum,Energy}Flux.cs),
+//     EulerFlux.cs:216-251 (wave speeds), CNS/Convection/RusanovFlux.cs:55-65.
+//   * grid: Cartesian{2,3}DGrid numbering (Grid2D.cs:396ff, Grid3D.cs:85-235),
+//     CellLengthScale (GridData_CellData.cs:664-698).
+//   * reference element tables: Gauss rules (LineQuadRules_bin.txt, Cube.cs:155-196,
+//     Square.cs:110-127, RefElement.cs:844-872) and orthonormal polynomials
+//     (Cube_polynomials.cs, Square.cs, Polynomial.cs:389-460) read from the
+//     fixture files written by tests/golden/make_reference_tables.py.
+//   * block LU: unblocked partial-pivot LU as LAPACK dgetf2/dgetrs (IMatrixExtensions.cs:2128-2230).
+//
+// Derivatives (JVP, VJP, assembled Jacobian blocks) are obtained by running the
+// same evaluator on forward-mode dual numbers; the reference has no adjoint.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr int CHUNK = 20;  // Quadrature_Bulksize.CHUNK_LIMIT, Quadrature.cs:37
+
+// ---------------------------------------------------------------- dual numbers
+template <int M>
+struct Dual {
+  double v;
+  double d[M];
+  Dual() : v(0) { for (int i = 0; i < M; i++) d[i] = 0; }
+  Dual(double x) : v(x) { for (int i = 0; i < M; i++) d[i] = 0; }
+};
+template <int M> inline Dual<M> operator+(const Dual<M>& a, const Dual<M>& b) { Dual<M> r; r.v = a.v + b.v; for (int i = 0; i < M; i++) r.d[i] = a.d[i] + b.d[i]; return r; }
+template <int M> inline Dual<M> operator-(const Dual<M>& a, const Dual<M>& b) { Dual<M> r; r.v = a.v - b.v; for (int i = 0; i < M; i++) r.d[i] = a.d[i] - b.d[i]; return r; }
+template <int M> inline Dual<M> operator-(const Dual<M>& a) { Dual<M> r; r.v = -a.v; for (int i = 0; i < M; i++) r.d[i] = -a.d[i]; return r; }
+template <int M> inline Dual<M> operator*(const Dual<M>& a, const Dual<M>& b) { Dual<M> r; r.v = a.v * b.v; for (int i = 0; i < M; i++) r.d[i] = a.d[i] * b.v + a.v * b.d[i]; return r; }
+template <int M> inline Dual<M> operator/(const Dual<M>& a, const Dual<M>& b) { Dual<M> r; r.v = a.v / b.v; double ib = 1.0 / b.v; for (int i = 0; i < M; i++) r.d[i] = (a.d[i] - r.v * b.d[i]) * ib; return r; }
+template <int M> inline Dual<M> operator+(const Dual<M>& a, double b) { Dual<M> r = a; r.v += b; return r; }
+template <int M> inline Dual<M> operator+(double b, const Dual<M>& a) { Dual<M> r = a; r.v += b; return r; }
+template <int M> inline Dual<M> operator-(const Dual<M>& a, double b) { Dual<M> r = a; r.v -= b; return r; }
+template <int M> inline Dual<M> operator-(double b, const Dual<M>& a) { Dual<M> r = -a; r.v += b; return r; }
+template <int M> inline Dual<M> operator*(const Dual<M>& a, double b) { Dual<M> r; r.v = a.v * b; for (int i = 0; i < M; i++) r.d[i] = a.d[i] * b; return r; }
+template <int M> inline Dual<M> operator*(double b, const Dual<M>& a) { return a * b; }
+template <int M> inline Dual<M> operator/(const Dual<M>& a, double b) { return a * (1.0 / b); }
+template <int M> inline Dual<M> operator/(double a, const Dual<M>& b) { return Dual<M>(a) / b; }
+template <int M> inline Dual<M>& operator+=(Dual<M>& a, const Dual<M>& b) { a = a + b; return a; }
+template <int M> inline Dual<M>& operator-=(Dual<M>& a, const Dual<M>& b) { a = a - b; return a; }
+template <int M> inline Dual<M>& operator*=(Dual<M>& a, const Dual<M>& b) { a = a * b; return a; }
+template <int M> inline Dual<M>& operator*=(Dual<M>& a, double b) { a = a * b; return a; }
+template <int M> inline Dual<M> sqrt(const Dual<M>& a) { Dual<M> r; r.v = std::sqrt(a.v); double f = 0.5 / r.v; for (int i = 0; i < M; i++) r.d[i] = a.d[i] * f; return r; }
+template <int M> inline Dual<M> fabs(const Dual<M>& a) { return a.v < 0 ? -a : a; }
+template <int M> inline double val(const Dual<M>& a) { return a.v; }
+inline double val(double a) { return a; }
+using std::sqrt;
+using std::fabs;
+// Math.Max(a,b) returns b only when b > a (ties keep a): follow that.
+template <class T> inline T dmax(const T& a, const T& b) { return val(b) > val(a) ? b : a; }
+
+// ---------------------------------------------------------- reference tables
+struct Rule1D { int order; std::vector<double> x, w; };
+struct Term { double c; int e[3]; };
+using Poly = std::vector<Term>;
+
+struct Tables {
+  std::vector<Rule1D> lines;
+  std::vector<Poly> polys[4];  // by spatial dimension 2, 3
+  bool ok = false;
+};
+
+bool load_tables(const std::string& dir, Tables& T) {
+  {
+    std::ifstream f(dir + "/line_quad_rules.txt");
+    if (!f) return false;
+    int n; f >> n;
+    T.lines.resize(n);
+    for (int i = 0; i < n; i++) {
+      int K; f >> T.lines[i].order >> K;
+      T.lines[i].x.resize(K); T.lines[i].w.resize(K);
+      for (int k = 0; k < K; k++) f >> T.lines[i].x[k] >> T.lines[i].w[k];
+    }
+    if (!f) return false;
+  }
+  const char* names[4] = {nullptr, nullptr, "square", "cube"};
+  for (int D = 2; D <= 3; D++) {
+    std::ifstream f(dir + "/" + names[D] + "_polynomials.txt");
+    if (!f) return false;
+    int n, DD; f >> n >> DD;
+    if (DD != D) return false;
+    T.polys[D].resize(n);
+    for (int i = 0; i < n; i++) {
+      int nt; f >> nt;
+      T.polys[D][i].resize(nt);
+      for (int t = 0; t < nt; t++) {
+        Term& tm = T.polys[D][i][t];
+        tm.e[0] = tm.e[1] = tm.e[2] = 0;
+        f >> tm.c;
+        for (int d = 0; d < D; d++) f >> tm.e[d];
+      }
+    }
+    if (!f) return false;
+  }
+  T.ok = true;
+  return true;
+}
+
+// RefElement.GetQuadratureRule: fewest nodes among rules with order >= desired (RefElement.cs:844-872).
+const Rule1D* pick_rule(const Tables& T, int order) {
+  const Rule1D* best = nullptr;
+  for (auto& r : T.lines)
+    if (r.order >= order && (!best || r.x.size() < best->x.size())) best = &r;
+  return best;
+}
+
+struct QuadRule { int K = 0, D = 0; std::vector<double> nodes, w; };
+
+// Tensor rules: Cube.cs:167-178 (node (i*NN+j)*NN+k -> x=node k, y=node j, z=node i),
+// Square.cs:110-127 (node i*NN+j -> x=node j, y=node i).  D==1: the line rule itself.
+QuadRule tensor_rule(const Rule1D& r, int D) {
+  QuadRule q; int NN = (int)r.x.size();
+  q.D = D; q.K = 1; for (int d = 0; d < D; d++) q.K *= NN;
+  q.nodes.resize((size_t)q.K * D); q.w.resize(q.K);
+  if (D == 1) { for (int k = 0; k < NN; k++) { q.nodes[k] = r.x[k]; q.w[k] = r.w[k]; } }
+  else if (D == 2) {
+    for (int i = 0; i < NN; i++) for (int j = 0; j < NN; j++) {
+      int k = i * NN + j; q.nodes[k * 2 + 0] = r.x[j]; q.nodes[k * 2 + 1] = r.x[i]; q.w[k] = r.w[i] * r.w[j];
+    }
+  } else {
+    for (int i = 0; i < NN; i++) for (int j = 0; j < NN; j++) for (int kk = 0; kk < NN; kk++) {
+      int k = (i * NN + j) * NN + kk;
+      q.nodes[k * 3 + 0] = r.x[kk]; q.nodes[k * 3 + 1] = r.x[j]; q.nodes[k * 3 + 2] = r.x[i];
+      q.w[k] = r.w[i] * r.w[j] * r.w[kk];
+    }
+  }
+  return q;
+}
+
+// Polynomial.Evaluate (Polynomial.cs:389-460): sum of coeff * prod_d x_d^e_d.
+double poly_eval(const Poly& p, const double* x, int D) {
+  double acc = 0;
+  for (auto& t : p) {
+    double m = 1; for (int d = 0; d < D; d++) m *= std::pow(x[d], t.e[d]);
+    acc += m * t.c;
+  }
+  return acc;
+}
+double poly_deriv(const Poly& p, const double* x, int D, int dd) {
+  double acc = 0;
+  for (auto& t : p) {
+    if (t.e[dd] == 0) continue;
+    double m = t.c * t.e[dd];
+    for (int d = 0; d < D; d++) m *= std::pow(x[d], d == dd ? t.e[d] - 1 : t.e[d]);
+    acc += m;
+  }
+  return acc;
+}
+
+// Face -> volume coordinates: Square.TransformFaceCoordinates / Cube.TransformFaceCoordinates
+// faces 0:-x 1:+x 2:+y 3:-y 4:+z 5:-z  (Cube.Faces, Square.Faces)
+void face_to_vol(int D, int face, const double* s, double* xi) {
+  if (D == 2) {
+    switch (face) {
+      case 0: xi[0] = -1; xi[1] = s[0]; break;
+      case 1: xi[0] = 1; xi[1] = s[0]; break;
+      case 2: xi[0] = s[0]; xi[1] = 1; break;
+      default: xi[0] = s[0]; xi[1] = -1; break;
+    }
+  } else {
+    switch (face) {
+      case 0: xi[0] = -1; xi[1] = s[1]; xi[2] = s[0]; break;
+      case 1: xi[0] = 1; xi[1] = s[1]; xi[2] = s[0]; break;
+      case 2: xi[0] = s[0]; xi[1] = 1; xi[2] = s[1]; break;
+      case 3: xi[0] = s[0]; xi[1] = -1; xi[2] = s[1]; break;
+      case 4: xi[0] = s[0]; xi[1] = s[1]; xi[2] = 1; break;
+      default: xi[0] = s[0]; xi[1] = s[1]; xi[2] = -1; break;
+    }
+  }
+}
+inline int face_of(int d, int side) {  // side 0: negative, 1: positive
+  static const int f[3][2] = {{0, 1}, {3, 2}, {5, 4}};
+  return f[d][side];
+}
+
+// ------------------------------------------------------------------ the model
+enum OpKind { OP_SIP = 0, OP_ADVDIFF = 1, OP_EULER_LF = 2, OP_EULER_HLLC = 3 };
+
+struct Params {
+  double penalty_base = 1.3, nu = 1.0;
+  double vel[3] = {0, 0, 0};
+  double gamma = 1.4, mach = 1.0;
+};
+
+struct Model {
+  Tables T;
+  int D = 0, p = 0, N = 0, order = 0;
+  int n[3] = {1, 1, 1}; double lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1}; bool per[3] = {false, false, false};
+  int J = 0, E = 0;
+  // cell metrics (GridData.CellData)
+  std::vector<double> detJ, invJ, scal, lscale, center;
+  // edges (GridData.EdgeData): IN/OUT cell (OUT=-1 on boundary), local faces, normal, sqrt(Gram)
+  std::vector<int> cin, cout_, fin, fout;
+  std::vector<double> normal, sqrtG;
+  // volume rule and basis tables
+  QuadRule vq, fq;
+  int Kv = 0, Ke = 0;
+  std::vector<double> phi, dphi;    // [Kv][N], [Kv][N][D]
+  std::vector<double> tr, dtr;      // [2D][Ke][N], [2D][Ke][N][D]
+  int nfaces = 0;
+};
+
+int cell_index(const Model& m, const int* ijk) {
+  if (m.D == 2) return ijk[0] * m.n[1] + ijk[1];
+  return (ijk[0] * m.n[1] + ijk[1]) * m.n[2] + ijk[2];
+}
+
+bool build_model(Model& m) {
+  int D = m.D;
+  m.N = 1; { long num = 1, den = 1; for (int d = 1; d <= D; d++) { num *= (m.p + d); den *= d; } m.N = (int)(num / den); }
+  if ((int)m.T.polys[D].size() < m.N) return false;
+  const Rule1D* r = pick_rule(m.T, m.order);
+  if (!r) return false;
+  m.vq = tensor_rule(*r, D);
+  m.fq = tensor_rule(*r, D - 1);
+  m.Kv = m.vq.K; m.Ke = m.fq.K; m.nfaces = 2 * D;
+  int N = m.N, Kv = m.Kv, Ke = m.Ke;
+  m.phi.resize((size_t)Kv * N); m.dphi.resize((size_t)Kv * N * D);
+  for (int k = 0; k < Kv; k++)
+    for (int nn = 0; nn < N; nn++) {
+      const Poly& P = m.T.polys[D][nn];
+      m.phi[k * N + nn] = poly_eval(P, &m.vq.nodes[k * D], D);
+      for (int d = 0; d < D; d++) m.dphi[(k * N + nn) * D + d] = poly_deriv(P, &m.vq.nodes[k * D], D, d);
+    }
+  m.tr.resize((size_t)m.nfaces * Ke * N); m.dtr.resize((size_t)m.nfaces * Ke * N * D);
+  for (int f = 0; f < m.nfaces; f++)
+    for (int k = 0; k < Ke; k++) {
+      double xi[3];
+      face_to_vol(D, f, &m.fq.nodes[k * (D - 1)], xi);
+      for (int nn = 0; nn < N; nn++) {
+        const Poly& P = m.T.polys[D][nn];
+        m.tr[((size_t)f * Ke + k) * N + nn] = poly_eval(P, xi, D);
+        for (int d = 0; d < D; d++) m.dtr[(((size_t)f * Ke + k) * N + nn) * D + d] = poly_deriv(P, xi, D, d);
+      }
+    }
+  // grid
+  m.J = 1; for (int d = 0; d < D; d++) m.J *= m.n[d];
+  double h[3], vol = 1;
+  for (int d = 0; d < D; d++) { h[d] = (m.hi[d] - m.lo[d]) / m.n[d]; vol *= h[d]; }
+  double dj = 1; for (int d = 0; d < D; d++) dj *= h[d] / 2;
+  m.detJ.assign(m.J, dj);
+  m.scal.assign(m.J, 1.0 / std::sqrt(dj));
+  m.invJ.assign((size_t)m.J * D * D, 0.0);
+  m.center.assign((size_t)m.J * D, 0.0);
+  std::vector<double> bndArea(m.J, 0.0);
+  double surf = 0, farea[3];
+  for (int d = 0; d < D; d++) { farea[d] = 1; for (int e = 0; e < D; e++) if (e != d) farea[d] *= h[e]; surf += 2 * farea[d]; }
+  int ijk[3] = {0, 0, 0};
+  for (int j = 0; j < m.J; j++) {
+    int rem = j;
+    for (int d = D - 1; d >= 0; d--) { ijk[d] = rem % m.n[d]; rem /= m.n[d]; }
+    for (int d = 0; d < D; d++) { m.invJ[(size_t)j * D * D + d * D + d] = 2.0 / h[d]; m.center[j * D + d] = m.lo[d] + (ijk[d] + 0.5) * h[d]; }
+  }
+  // edges: per direction, interior (and periodic) faces first in cell order, then boundary faces
+  m.cin.clear(); m.cout_.clear(); m.fin.clear(); m.fout.clear(); m.normal.clear(); m.sqrtG.clear();
+  for (int d = 0; d < D; d++) {
+    double sg = farea[d] / std::pow(2.0, D - 1);
+    for (int j = 0; j < m.J; j++) {
+      int rem = j;
+      for (int dd = D - 1; dd >= 0; dd--) { ijk[dd] = rem % m.n[dd]; rem /= m.n[dd]; }
+      int nb[3] = {ijk[0], ijk[1], ijk[2]};
+      nb[d] = ijk[d] + 1;
+      if (nb[d] >= m.n[d]) { if (!m.per[d]) continue; nb[d] = 0; }
+      int jo = cell_index(m, nb);
+      m.cin.push_back(j); m.cout_.push_back(jo); m.fin.push_back(face_of(d, 1)); m.fout.push_back(face_of(d, 0));
+      for (int e = 0; e < D; e++) m.normal.push_back(e == d ? 1.0 : 0.0);
+      m.sqrtG.push_back(sg);
+    }
+  }
+  for (int d = 0; d < D; d++) {
+    if (m.per[d]) continue;
+    double sg = farea[d] / std::pow(2.0, D - 1);
+    for (int side = 0; side < 2; side++)
+      for (int j = 0; j < m.J; j++) {
+        int rem = j;
+        for (int dd = D - 1; dd >= 0; dd--) { ijk[dd] = rem % m.n[dd]; rem /= m.n[dd]; }
+        if (ijk[d] != (side ? m.n[d] - 1 : 0)) continue;
+        m.cin.push_back(j); m.cout_.push_back(-1); m.fin.push_back(face_of(d, side)); m.fout.push_back(-1);
+        for (int e = 0; e < D; e++) m.normal.push_back(e == d ? (side ? 1.0 : -1.0) : 0.0);
+        m.sqrtG.push_back(sg);
+        bndArea[j] += farea[d];
+      }
+  }
+  m.E = (int)m.cin.size();
+  // CellLengthScale = 1/cj, cj = (SurfaceArea + bndArea) * 0.5 / Volume  (GridData_CellData.cs:681-684)
+  m.lscale.resize(m.J);
+  for (int j = 0; j < m.J; j++) m.lscale[j] = 1.0 / ((surf + bndArea[j]) * 0.5 / vol);
+  return true;
+}
+
+int nvars(int kind, int D) { return (kind == OP_EULER_LF || kind == OP_EULER_HLLC) ? D + 2 : 1; }
+bool needs_grad(int kind) { return kind == OP_SIP || kind == OP_ADVDIFF; }
+
+// SIPLaplace.CoefficientUpdate / GetPenalty (ipLaplace.cs:100-144)
+double sip_penalty_deg(int p, int D) {
+  double _p = p, _D = D;
+  return std::max((_p + 1) * (_p + _D) / _D, (_p + 1.0) * (_p + 1.0));
+}
+double sip_penalty(const Model& m, const Params& P, int jin, int jout) {
+  double pd = sip_penalty_deg(m.p, m.D);
+  double mu = P.penalty_base * pd / m.lscale[jin];
+  if (jout >= 0) mu = std::max(mu, P.penalty_base * pd / m.lscale[jout]);
+  return mu;
+}
+
+// ------------------------------------------------------ pointwise components
+// SIPLaplace (ipLaplace.cs:183-249), constant Nu; Dirichlet g_D = 0 on all boundary edges.
+template <class T>
+T sip_volume(int D, double nu, const T* U, const T* GradU, double V, const double* GradV) {
+  T acc(0.0);
+  for (int d = 0; d < D; d++) acc = acc - GradU[d] * GradV[d] * nu;
+  (void)U; (void)V;
+  return acc;
+}
+template <class T>
+T sip_inner(int D, double nu, double pnlty, const double* n, const T* uA, const T* uB, const T* GuA, const T* GuB,
+            double vA, double vB, const double* GvA, const double* GvB) {
+  T Acc(0.0);
+  double nuA = nu, nuB = nu;
+  for (int d = 0; d < D; d++) {
+    Acc += 0.5 * (nuA * GuA[d] + nuB * GuB[d]) * (vA - vB) * n[d];
+    Acc += 0.5 * (nuA * GvA[d] + nuB * GvB[d]) * (uA[0] - uB[0]) * n[d];
+  }
+  double nuMax = (std::fabs(nuA) > std::fabs(nuB)) ? nuA : nuB;
+  Acc -= (uA[0] - uB[0]) * (vA - vB) * pnlty * nuMax;
+  return Acc;
+}
+template <class T>
+T sip_boundary(int D, double nu, double pnlty1, const double* n, const T* uA, const T* GuA, double vA, const double* GvA) {
+  T Acc(0.0);
+  double pnlty = 2 * pnlty1;
+  double g_D = 0.0;
+  for (int d = 0; d < D; d++) {
+    Acc += (nu * GuA[d]) * vA * n[d];
+    Acc += (nu * GvA[d]) * (uA[0] - g_D) * n[d];
+  }
+  Acc -= nu * (uA[0] - g_D) * (vA - 0) * pnlty;
+  return Acc;
+}
+
+// ScalarTransportFlux3D (TransportFlux.cs:112-190) with constant velocity parameters.
+template <class T>
+T upwind_inner(int D, const double* vel, const double* n, const T& uin, const T& uout) {
+  double un = 0; for (int d = 0; d < D; d++) un += vel[d] * n[d];
+  if (un > 0) return un * uin;
+  return un * uout;
+}
+
+// Euler (StateVector.cs, IdealGas.cs, Euler*Component.cs)
+template <class T>
+struct State {
+  T rho, m[3], E, u[3], p, a;
+};
+template <class T>
+State<T> make_state(int D, const T* U, const Params& P) {
+  State<T> s;
+  s.rho = U[0];
+  for (int d = 0; d < 3; d++) s.m[d] = T(0.0);
+  for (int d = 0; d < D; d++) s.m[d] = U[1 + d];
+  s.E = U[D + 1];
+  for (int d = 0; d < 3; d++) s.u[d] = T(0.0);
+  for (int d = 0; d < D; d++) s.u[d] = s.m[d] / s.rho;
+  T msq(0.0); for (int d = 0; d < D; d++) msq += s.m[d] * s.m[d];
+  T kin = 0.5 * msq / s.rho * (P.gamma * P.mach * P.mach);   // StateVector.KineticEnergy
+  s.p = (P.gamma - 1.0) * (s.E - kin);                        // IdealGas.GetPressure
+  s.a = sqrt(s.p / s.rho) / P.mach;                           // IdealGas.GetSpeedOfSound
+  return s;
+}
+// Flux vector of component c (0: density, 1..D: momentum, D+1: energy), direction d
+template <class T>
+T euler_flux(int D, const State<T>& s, int c, int d, const Params& P) {
+  if (c == 0) return s.m[d];
+  if (c <= D) {
+    T f = s.m[c - 1] * s.u[d];
+    if (d == c - 1) f += 1.0 / (P.gamma * P.mach * P.mach) * s.p;
+    return f;
+  }
+  return s.u[d] * (s.E + s.p);
+}
+template <class T>
+T euler_value(int D, const State<T>& s, int c) {
+  if (c == 0) return s.rho;
+  if (c <= D) return s.m[c - 1];
+  return s.E;
+}
+template <class T>
+T dotn(int D, const T* v, const double* n) { T r(0.0); for (int d = 0; d < D; d++) r += v[d] * n[d]; return r; }
+
+// RusanovFlux.InnerEdgeFlux (RusanovFlux.cs:55-65)
+template <class T>
+T rusanov(int D, const State<T>& si, const State<T>& so, int c, const double* n, const Params& P) {
+  T wsi = fabs(dotn(D, si.u, n)) + si.a;
+  T wso = fabs(dotn(D, so.u, n)) + so.a;
+  T pen = dmax(wsi, wso);
+  T fi(0.0), fo(0.0);
+  for (int d = 0; d < D; d++) { fi += euler_flux(D, si, c, d, P) * n[d]; fo += euler_flux(D, so, c, d, P) * n[d]; }
+  return 0.5 * (fi + fo - pen * (euler_value(D, so, c) - euler_value(D, si, c)));
+}
+
+// EulerFlux.EstimateWaveSpeeds (EulerFlux.cs:216-251)
+template <class T>
+void wave_speeds(int D, const State<T>& si, const State<T>& so, const double* n, const Params& P, T& wIn, T& wOut) {
+  T uni = dotn(D, si.u, n), uno = dotn(D, so.u, n);
+  T meanP = 0.5 * (si.p + so.p);
+  T meanR = 0.5 * (si.rho + so.rho);
+  T meanA = 0.5 * (si.a + so.a);
+  T jump = uno - uni;
+  double g = P.gamma;
+  double ms = g * P.mach * P.mach;
+  T pStar = meanP - 0.5 * ms * jump * meanA * meanR;
+  pStar = dmax(T(0.0), pStar);
+  T qIn(1.0), qOut(1.0);
+  if (val(pStar) > val(si.p)) qIn = sqrt(1.0 + 0.5 * (g + 1.0) * (pStar / si.p - 1.0) / g);
+  if (val(pStar) > val(so.p)) qOut = sqrt(1.0 + 0.5 * (g + 1.0) * (pStar / so.p - 1.0) / g);
+  wIn = uni - si.a * qIn;
+  wOut = uno + so.a * qOut;
+}
+// HLLC{Density,Momentum,Energy}Flux.GetModifiedVariableValue
+template <class T>
+T hllc_modified(int D, const State<T>& s, int c, const T& cws, const T& cun, const T& sm, const double* n, const Params& P) {
+  T base = s.rho * (cws - cun) / (cws - sm);
+  if (c == 0) return base;
+  if (c <= D) return base * (s.u[c - 1] + (sm - cun) * n[c - 1]);
+  double ms = P.gamma * P.mach * P.mach;
+  T factor = sm * ms + s.p / (s.rho * (cws - cun));
+  return base * (s.E / s.rho + factor * (sm - cun));
+}
+// HLLCFlux.InnerEdgeFlux (HLLCFlux.cs:48-86)
+template <class T>
+T hllc(int D, const State<T>& si, const State<T>& so, int c, const double* n, const Params& P) {
+  T wIn, wOut;
+  wave_speeds(D, si, so, n, P, wIn, wOut);
+  double ms = P.gamma * P.mach * P.mach;
+  T uni = dotn(D, si.u, n), uno = dotn(D, so.u, n);
+  T cIn = si.rho * (wIn - uni);
+  T cOut = so.rho * (wOut - uno);
+  T sm = (cOut * uno - cIn * uni + (si.p - so.p) / ms) / (cOut - cIn);
+  auto fn = [&](const State<T>& s) { T f(0.0); for (int d = 0; d < D; d++) f += euler_flux(D, s, c, d, P) * n[d]; return f; };
+  if (val(wIn) > 0.0) return fn(si);
+  if (val(wIn) <= 0.0 && 0.0 < val(sm)) return fn(si) + wIn * (hllc_modified(D, si, c, wIn, uni, sm, n, P) - euler_value(D, si, c));
+  if (val(sm) <= 0.0 && 0.0 <= val(wOut)) return fn(so) + wOut * (hllc_modified(D, so, c, wOut, uno, sm, n, P) - euler_value(D, so, c));
+  return fn(so);
+}
+
+// Pointwise evaluation of all components of an operator, vectorised as the
+// reference does (test-function probing for forms; -1 for nonlinear volume fluxes).
+// Volume: F[f*D+d] multiplies d(v)/dx_d, S[f] multiplies v.
+template <class T>
+void point_volume(const Model& m, int kind, const Params& P, const T* U, const T* GU, T* F, T* S) {
+  int D = m.D, V = nvars(kind, D);
+  for (int i = 0; i < V * D; i++) F[i] = T(0.0);
+  for (int i = 0; i < V; i++) S[i] = T(0.0);
+  if (kind == OP_SIP || kind == OP_ADVDIFF) {
+    if (kind == OP_ADVDIFF)  // NonlinearFlux: F = vel*u, scaled by -1 (NECQuadratureVolume.cs:452)
+      for (int d = 0; d < D; d++) F[d] = -(P.vel[d] * U[0]);
+    double GV[3];
+    for (int d = 0; d < D; d++) {  // form probed with GradV = e_d (NonlinFormVectorizer.cs:240-266)
+      for (int e = 0; e < D; e++) GV[e] = (e == d) ? 1.0 : 0.0;
+      F[d] += sip_volume(D, P.nu, U, GU, 0.0, GV);
+    }
+  } else {
+    State<T> s = make_state(D, U, P);
+    for (int c = 0; c < V; c++)
+      for (int d = 0; d < D; d++) F[c * D + d] = -euler_flux(D, s, c, d, P);
+  }
+}
+// Inner edge: fin/fot multiply v_in/v_out; gin/got multiply grad(v_in)/grad(v_out) (physical).
+template <class T>
+void point_inner(const Model& m, int kind, const Params& P, double mu, const double* n, const T* Ui, const T* Uo,
+                 const T* Gi, const T* Go, T* fin, T* fot, T* gin, T* got) {
+  int D = m.D, V = nvars(kind, D);
+  for (int i = 0; i < V; i++) { fin[i] = T(0.0); fot[i] = T(0.0); }
+  for (int i = 0; i < V * D; i++) { gin[i] = T(0.0); got[i] = T(0.0); }
+  if (kind == OP_SIP || kind == OP_ADVDIFF) {
+    if (kind == OP_ADVDIFF) {
+      T fl = upwind_inner(D, P.vel, n, Ui[0], Uo[0]);
+      fin[0] += fl; fot[0] -= fl;
+    }
+    double z[3] = {0, 0, 0}, ed[3];
+    fin[0] += sip_inner(D, P.nu, mu, n, Ui, Uo, Gi, Go, 1.0, 0.0, z, z);
+    fot[0] += sip_inner(D, P.nu, mu, n, Ui, Uo, Gi, Go, 0.0, 1.0, z, z);
+    for (int d = 0; d < D; d++) {
+      for (int e = 0; e < D; e++) ed[e] = (e == d) ? 1.0 : 0.0;
+      gin[d] += sip_inner(D, P.nu, mu, n, Ui, Uo, Gi, Go, 0.0, 0.0, ed, z);
+      got[d] += sip_inner(D, P.nu, mu, n, Ui, Uo, Gi, Go, 0.0, 0.0, z, ed);
+    }
+  } else {
+    State<T> si = make_state(D, Ui, P), so = make_state(D, Uo, P);
+    for (int c = 0; c < V; c++) {
+      T fl = (kind == OP_EULER_LF) ? rusanov(D, si, so, c, n, P) : hllc(D, si, so, c, n, P);
+      fin[c] = fl; fot[c] = -fl;
+    }
+  }
+}
+template <class T>
+void point_boundary(const Model& m, int kind, const Params& P, double mu, const double* n, const T* Ui, const T* Gi, T* fin, T* gin) {
+  int D = m.D, V = nvars(kind, D);
+  for (int i = 0; i < V; i++) fin[i] = T(0.0);
+  for (int i = 0; i < V * D; i++) gin[i] = T(0.0);
+  if (kind == OP_SIP || kind == OP_ADVDIFF) {
+    if (kind == OP_ADVDIFF) {  // BorderEdgeFlux, inflow value 0
+      double un = 0; for (int d = 0; d < D; d++) un += P.vel[d] * n[d];
+      fin[0] += (un >= 0) ? un * Ui[0] : T(un * 0.0);
+    }
+    double z[3] = {0, 0, 0}, ed[3];
+    fin[0] += sip_boundary(D, P.nu, mu, n, Ui, Gi, 1.0, z);
+    for (int d = 0; d < D; d++) {
+      for (int e = 0; e < D; e++) ed[e] = (e == d) ? 1.0 : 0.0;
+      gin[d] += sip_boundary(D, P.nu, mu, n, Ui, Gi, 0.0, ed);
+    }
+  }
+  // Euler: all configurations here are periodic (no boundary edges).
+}
+
+// ------------------------------------------------------------- evaluation
+// Evaluate field values (and gradients) at the given nodes of one cell:
+//   u[k*V+f] = sum_n Scaling_j c_{j,f,n} phi[k][n];  grad: sum_e invJ[e][d] * ref-gradient_e
+template <class T>
+void eval_cell(const Model& m, int V, bool grad, const T* c /*V*N*/, int j, const double* tab, const double* dtab, int K,
+               T* u /*K*V*/, T* gu /*K*V*D*/) {
+  int N = m.N, D = m.D;
+  double s = m.scal[j];
+  const double* iJ = &m.invJ[(size_t)j * D * D];
+  for (int k = 0; k < K; k++)
+    for (int f = 0; f < V; f++) {
+      T acc(0.0);
+      for (int nn = 0; nn < N; nn++) acc += (s * c[f * N + nn]) * tab[k * N + nn];
+      u[k * V + f] = acc;
+      if (grad) {
+        T gr[3];
+        for (int e = 0; e < D; e++) {
+          T a(0.0);
+          for (int nn = 0; nn < N; nn++) a += (s * c[f * N + nn]) * dtab[(k * N + nn) * D + e];
+          gr[e] = a;
+        }
+        for (int d = 0; d < D; d++) {
+          T a(0.0);
+          for (int e = 0; e < D; e++) a += gr[e] * iJ[e * D + d];
+          gu[(k * V + f) * D + d] = a;
+        }
+      }
+    }
+}
+
+// Test contraction for one cell and one side:
+//   res[f*N+n] += sqrtG_or_detJ * Scaling_j * sum_k w_k ( src[k,f] phi[k,n] + sum_e (sum_d flx[k,f,d] invJ[e,d]) dphi[k,n,e] )
+template <class T>
+void test_cell(const Model& m, int V, int j, const double* tab, const double* dtab, const double* w, int K, double metric,
+               const T* src /*K*V or null*/, const T* flx /*K*V*D or null*/, T* res) {
+  int N = m.N, D = m.D;
+  const double* iJ = &m.invJ[(size_t)j * D * D];
+  double sc = metric * m.scal[j];
+  std::vector<T> trf(flx ? (size_t)K * V * D : 0);
+  if (flx)
+    for (int k = 0; k < K; k++)
+      for (int f = 0; f < V; f++)
+        for (int e = 0; e < D; e++) {
+          T a(0.0);
+          for (int d = 0; d < D; d++) a += flx[(k * V + f) * D + d] * iJ[e * D + d];
+          trf[(k * V + f) * D + e] = a;
+        }
+  for (int f = 0; f < V; f++)
+    for (int nn = 0; nn < N; nn++) {
+      T acc(0.0);
+      for (int k = 0; k < K; k++) {
+        if (src) acc += src[k * V + f] * (w[k] * tab[k * N + nn]);
+        if (flx) for (int e = 0; e < D; e++) acc += trf[(k * V + f) * D + e] * (w[k] * dtab[(k * N + nn) * D + e]);
+      }
+      res[f * N + nn] += acc * sc;
+    }
+}
+
+template <class T>
+void volume_part(const Model& m, int kind, const Params& P, int j, const T* c, T* res) {
+  int V = nvars(kind, m.D), D = m.D, Kv = m.Kv;
+  bool grad = needs_grad(kind);
+  std::vector<T> u((size_t)Kv * V), gu(grad ? (size_t)Kv * V * D : 0), F((size_t)Kv * V * D), S((size_t)Kv * V);
+  eval_cell(m, V, grad, c, j, m.phi.data(), m.dphi.data(), Kv, u.data(), grad ? gu.data() : nullptr);
+  for (int k = 0; k < Kv; k++)
+    point_volume(m, kind, P, &u[k * V], grad ? &gu[(size_t)k * V * D] : nullptr, &F[(size_t)k * V * D], &S[k * V]);
+  test_cell(m, V, j, m.phi.data(), m.dphi.data(), m.vq.w.data(), Kv, m.detJ[j], (const T*)nullptr, F.data(), res);
+}
+
+// res_in / res_out: V*N each
+template <class T>
+void edge_part(const Model& m, int kind, const Params& P, int e, const T* cinp, const T* coutp, T* res_in, T* res_out) {
+  int V = nvars(kind, m.D), D = m.D, Ke = m.Ke, N = m.N;
+  bool grad = needs_grad(kind);
+  int ji = m.cin[e], jo = m.cout_[e];
+  const double* n = &m.normal[(size_t)e * D];
+  double mu = 0;
+  if (kind == OP_SIP || kind == OP_ADVDIFF) mu = sip_penalty(m, P, ji, jo);
+  size_t trs = (size_t)Ke * N;
+  std::vector<T> ui((size_t)Ke * V), uo((size_t)Ke * V), gi(grad ? (size_t)Ke * V * D : 1), go(grad ? (size_t)Ke * V * D : 1);
+  std::vector<T> fin((size_t)Ke * V), fot((size_t)Ke * V), gin((size_t)Ke * V * D), got((size_t)Ke * V * D);
+  eval_cell(m, V, grad, cinp, ji, &m.tr[m.fin[e] * trs], &m.dtr[m.fin[e] * trs * D], Ke, ui.data(), grad ? gi.data() : nullptr);
+  if (jo >= 0) {
+    eval_cell(m, V, grad, coutp, jo, &m.tr[m.fout[e] * trs], &m.dtr[m.fout[e] * trs * D], Ke, uo.data(), grad ? go.data() : nullptr);
+    for (int k = 0; k < Ke; k++)
+      point_inner(m, kind, P, mu, n, &ui[k * V], &uo[k * V], grad ? &gi[(size_t)k * V * D] : nullptr, grad ? &go[(size_t)k * V * D] : nullptr,
+                  &fin[k * V], &fot[k * V], &gin[(size_t)k * V * D], &got[(size_t)k * V * D]);
+  } else {
+    for (int k = 0; k < Ke; k++)
+      point_boundary(m, kind, P, mu, n, &ui[k * V], grad ? &gi[(size_t)k * V * D] : nullptr, &fin[k * V], &gin[(size_t)k * V * D]);
+  }
+  test_cell(m, V, ji, &m.tr[m.fin[e] * trs], &m.dtr[m.fin[e] * trs * D], m.fq.w.data(), Ke, m.sqrtG[e], fin.data(),
+            grad ? gin.data() : (const T*)nullptr, res_in);
+  if (jo >= 0)
+    test_cell(m, V, jo, &m.tr[m.fout[e] * trs], &m.dtr[m.fout[e] * trs * D], m.fq.w.data(), Ke, m.sqrtG[e], fot.data(),
+              grad ? got.data() : (const T*)nullptr, res_out);
+}
+
+// R <- beta R + alpha op(U), coefficients of type T (double or Dual<1> for JVP).
+template <class T>
+void evaluate(const Model& m, int kind, const Params& P, double alpha, double beta, const T* U, T* R) {
+  int V = nvars(kind, m.D), N = m.N;
+  size_t Nt = (size_t)V * N;
+  // DifferentialOperator.cs:1593: dscal(beta)
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)(Nt * m.J); i++) R[i] = (beta == 0.0) ? T(0.0) : R[i] * beta;
+  // volume: chunks of CHUNK cells
+  int nch = (m.J + CHUNK - 1) / CHUNK;
+#pragma omp parallel
+  {
+    std::vector<T> res(Nt);
+#pragma omp for schedule(dynamic, 4)
+    for (int ch = 0; ch < nch; ch++)
+      for (int j = ch * CHUNK; j < std::min(m.J, (ch + 1) * CHUNK); j++) {
+        for (size_t i = 0; i < Nt; i++) res[i] = T(0.0);
+        volume_part(m, kind, P, j, &U[j * Nt], res.data());
+        for (size_t i = 0; i < Nt; i++) R[j * Nt + i] += res[i] * alpha;
+      }
+  }
+  // edges: blocks of chunks computed in parallel, scattered in edge order
+  const int BLK = 8192;
+  std::vector<T> eres((size_t)BLK * 2 * Nt);
+  for (int e0 = 0; e0 < m.E; e0 += BLK) {
+    int e1 = std::min(m.E, e0 + BLK);
+#pragma omp parallel for schedule(dynamic, CHUNK)
+    for (int e = e0; e < e1; e++) {
+      T* ri = &eres[(size_t)(e - e0) * 2 * Nt];
+      T* ro = ri + Nt;
+      for (size_t i = 0; i < 2 * Nt; i++) ri[i] = T(0.0);
+      int jo = m.cout_[e];
+      edge_part(m, kind, P, e, &U[m.cin[e] * Nt], jo >= 0 ? &U[jo * Nt] : (const T*)nullptr, ri, ro);
+    }
+    for (int e = e0; e < e1; e++) {
+      const T* ri = &eres[(size_t)(e - e0) * 2 * Nt];
+      const T* ro = ri + Nt;
+      int ji = m.cin[e], jo = m.cout_[e];
+      for (size_t i = 0; i < Nt; i++) R[ji * Nt + i] += ri[i] * alpha;
+      if (jo >= 0) for (size_t i = 0; i < Nt; i++) R[jo * Nt + i] += ro[i] * alpha;
+    }
+  }
+}
+
+// Block of the exact Jacobian d(part residual)/d(coefficients), by forward-mode
+// dual numbers seeded one coefficient at a time (chunked by 8 directions).
+constexpr int NDIR = 8;
+using D8 = Dual<NDIR>;
+
+// Cell volume Jacobian: Jv[(f,n) row][(g,m) col], Nt x Nt
+void volume_jacobian(const Model& m, int kind, const Params& P, int j, const double* U, double* Jv) {
+  int V = nvars(kind, m.D), N = m.N; int Nt = V * N;
+  std::vector<D8> c(Nt), res(Nt);
+  for (int c0 = 0; c0 < Nt; c0 += NDIR) {
+    for (int i = 0; i < Nt; i++) { c[i] = D8(U[(size_t)j * Nt + i]); if (i >= c0 && i < c0 + NDIR) c[i].d[i - c0] = 1.0; }
+    for (int i = 0; i < Nt; i++) res[i] = D8(0.0);
+    volume_part(m, kind, P, j, c.data(), res.data());
+    for (int r = 0; r < Nt; r++) for (int q = c0; q < std::min(Nt, c0 + NDIR); q++) Jv[(size_t)r * Nt + q] = res[r].d[q - c0];
+  }
+}
+// Edge Jacobian: 4 blocks [in/out row side][in/out col side], each Nt x Nt
+void edge_jacobian(const Model& m, int kind, const Params& P, int e, const double* U, double* Je) {
+  int V = nvars(kind, m.D), N = m.N; int Nt = V * N;
+  int ji = m.cin[e], jo = m.cout_[e];
+  int ncol = (jo >= 0) ? 2 * Nt : Nt;
+  std::vector<D8> ci(Nt), co(Nt), ri(Nt), ro(Nt);
+  for (size_t i = 0; i < (size_t)4 * Nt * Nt; i++) Je[i] = 0;
+  for (int c0 = 0; c0 < ncol; c0 += NDIR) {
+    for (int i = 0; i < Nt; i++) {
+      ci[i] = D8(U[(size_t)ji * Nt + i]); if (i >= c0 && i < c0 + NDIR) ci[i].d[i - c0] = 1.0;
+      co[i] = D8(jo >= 0 ? U[(size_t)jo * Nt + i] : 0.0);
+      int q = i + Nt; if (jo >= 0 && q >= c0 && q < c0 + NDIR) co[i].d[q - c0] = 1.0;
+      ri[i] = D8(0.0); ro[i] = D8(0.0);
+    }
+    edge_part(m, kind, P, e, ci.data(), jo >= 0 ? co.data() : (const D8*)nullptr, ri.data(), ro.data());
+    for (int q = c0; q < std::min(ncol, c0 + NDIR); q++) {
+      int cs = q / Nt, cq = q % Nt;
+      for (int r = 0; r < Nt; r++) {
+        Je[((size_t)(0 * 2 + cs) * Nt + r) * Nt + cq] = ri[r].d[q - c0];
+        Je[((size_t)(1 * 2 + cs) * Nt + r) * Nt + cq] = ro[r].d[q - c0];
+      }
+    }
+  }
+}
+
+// VJP: out <- (dR/dU)^T W, assembled from the exact local Jacobians.
+void vjp(const Model& m, int kind, const Params& P, const double* U, const double* W, double* out) {
+  int V = nvars(kind, m.D), N = m.N; int Nt = V * N;
+  for (size_t i = 0; i < (size_t)m.J * Nt; i++) out[i] = 0;
+#pragma omp parallel
+  {
+    std::vector<double> Jv((size_t)Nt * Nt);
+#pragma omp for schedule(dynamic, 4)
+    for (int j = 0; j < m.J; j++) {
+      volume_jacobian(m, kind, P, j, U, Jv.data());
+      for (int q = 0; q < Nt; q++) {
+        double a = 0;
+        for (int r = 0; r < Nt; r++) a += Jv[(size_t)r * Nt + q] * W[(size_t)j * Nt + r];
+        out[(size_t)j * Nt + q] += a;
+      }
+    }
+  }
+  const int BLK = 4096;
+  std::vector<double> buf((size_t)BLK * 2 * Nt);
+  for (int e0 = 0; e0 < m.E; e0 += BLK) {
+    int e1 = std::min(m.E, e0 + BLK);
+#pragma omp parallel
+    {
+      std::vector<double> Je((size_t)4 * Nt * Nt);
+#pragma omp for schedule(dynamic, CHUNK)
+      for (int e = e0; e < e1; e++) {
+        edge_jacobian(m, kind, P, e, U, Je.data());
+        int ji = m.cin[e], jo = m.cout_[e];
+        double* bi = &buf[(size_t)(e - e0) * 2 * Nt];
+        double* bo = bi + Nt;
+        for (int cs = 0; cs < 2; cs++)
+          for (int q = 0; q < Nt; q++) {
+            double a = 0;
+            for (int r = 0; r < Nt; r++) {
+              a += Je[((size_t)(0 * 2 + cs) * Nt + r) * Nt + q] * W[(size_t)ji * Nt + r];
+              if (jo >= 0) a += Je[((size_t)(1 * 2 + cs) * Nt + r) * Nt + q] * W[(size_t)jo * Nt + r];
+            }
+            (cs == 0 ? bi : bo)[q] = a;
+          }
+      }
+    }
+    for (int e = e0; e < e1; e++) {
+      int ji = m.cin[e], jo = m.cout_[e];
+      const double* bi = &buf[(size_t)(e - e0) * 2 * Nt];
+      for (int q = 0; q < Nt; q++) out[(size_t)ji * Nt + q] += bi[q];
+      if (jo >= 0) for (int q = 0; q < Nt; q++) out[(size_t)jo * Nt + q] += bi[Nt + q];
+    }
+  }
+}
+
+// ------------------------------------------------------------- block LU
+// Unblocked right-looking LU with partial pivoting (LAPACK dgetf2 semantics):
+// pivot = first index of max |a| in the column; returns 0 or the 1-based index of
+// the first zero pivot (INFO > 0), as the LAPACK wrappers do (IMatrixExtensions.cs:2190-2230).
+int lu_factor(int n, double* A /*row-major n x n*/, int* piv) {
+  int info = 0;
+  for (int j = 0; j < n; j++) {
+    int p = j; double mx = std::fabs(A[(size_t)j * n + j]);
+    for (int i = j + 1; i < n; i++) { double v = std::fabs(A[(size_t)i * n + j]); if (v > mx) { mx = v; p = i; } }
+    piv[j] = p;
+    if (A[(size_t)p * n + j] == 0.0) { if (!info) info = j + 1; continue; }
+    if (p != j) for (int c = 0; c < n; c++) std::swap(A[(size_t)j * n + c], A[(size_t)p * n + c]);
+    double inv = 1.0 / A[(size_t)j * n + j];
+    for (int i = j + 1; i < n; i++) A[(size_t)i * n + j] *= inv;
+    for (int i = j + 1; i < n; i++) {
+      double l = A[(size_t)i * n + j];
+      for (int c = j + 1; c < n; c++) A[(size_t)i * n + c] -= l * A[(size_t)j * n + c];
+    }
+  }
+  return info;
+}
+void lu_solve(int n, const double* LU, const int* piv, double* b) {
+  for (int j = 0; j < n; j++) if (piv[j] != j) std::swap(b[j], b[piv[j]]);
+  for (int i = 0; i < n; i++) { double a = b[i]; for (int c = 0; c < i; c++) a -= LU[(size_t)i * n + c] * b[c]; b[i] = a; }
+  for (int i = n - 1; i >= 0; i--) { double a = b[i]; for (int c = i + 1; c < n; c++) a -= LU[(size_t)i * n + c] * b[c]; b[i] = a / LU[(size_t)i * n + i]; }
+}
+
+thread_local std::string g_err;
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+const char* bref_last_error() { return g_err.c_str(); }
+
+// Create a Cartesian DG model: D in {2,3}, ncells[D], lo/hi[D], periodic[D], degree p, quadrature order.
+void* bref_create(const char* tables_dir, int D, const int* ncells, const double* lo, const double* hi, const int* periodic,
+                  int p, int quad_order) {
+  Model* m = new Model();
+  if (!load_tables(tables_dir, m->T)) { g_err = "cannot read reference tables in " + std::string(tables_dir); delete m; return nullptr; }
+  if (D < 2 || D > 3) { g_err = "D must be 2 or 3"; delete m; return nullptr; }
+  m->D = D; m->p = p; m->order = quad_order;
+  for (int d = 0; d < D; d++) { m->n[d] = ncells[d]; m->lo[d] = lo[d]; m->hi[d] = hi[d]; m->per[d] = periodic[d] != 0; }
+  if (!build_model(*m)) { g_err = "unsupported degree/order"; delete m; return nullptr; }
+  return m;
+}
+void bref_destroy(void* h) { delete (Model*)h; }
+
+void bref_info(void* h, int* out /* J E N Kv Ke */) {
+  Model* m = (Model*)h;
+  out[0] = m->J; out[1] = m->E; out[2] = m->N; out[3] = m->Kv; out[4] = m->Ke;
+}
+
+// Geometry + tables export (for cross-checks of the product's host-side setup).
+void bref_export_grid(void* h, double* detJ, double* invJ, double* scal, double* lscale, int* cells /*E*2*/, int* faces /*E*2*/,
+                      double* normals /*E*D*/, double* sqrtg) {
+  Model* m = (Model*)h;
+  std::copy(m->detJ.begin(), m->detJ.end(), detJ);
+  std::copy(m->invJ.begin(), m->invJ.end(), invJ);
+  std::copy(m->scal.begin(), m->scal.end(), scal);
+  std::copy(m->lscale.begin(), m->lscale.end(), lscale);
+  for (int e = 0; e < m->E; e++) { cells[2 * e] = m->cin[e]; cells[2 * e + 1] = m->cout_[e]; faces[2 * e] = m->fin[e]; faces[2 * e + 1] = m->fout[e]; }
+  std::copy(m->normal.begin(), m->normal.end(), normals);
+  std::copy(m->sqrtG.begin(), m->sqrtG.end(), sqrtg);
+}
+void bref_export_tables(void* h, double* vnodes, double* vw, double* phi, double* dphi, double* fnodes, double* fw, double* tr, double* dtr) {
+  Model* m = (Model*)h;
+  std::copy(m->vq.nodes.begin(), m->vq.nodes.end(), vnodes);
+  std::copy(m->vq.w.begin(), m->vq.w.end(), vw);
+  std::copy(m->phi.begin(), m->phi.end(), phi);
+  std::copy(m->dphi.begin(), m->dphi.end(), dphi);
+  std::copy(m->fq.nodes.begin(), m->fq.nodes.end(), fnodes);
+  std::copy(m->fq.w.begin(), m->fq.w.end(), fw);
+  std::copy(m->tr.begin(), m->tr.end(), tr);
+  std::copy(m->dtr.begin(), m->dtr.end(), dtr);
+}
+// Evaluate a polynomial of the reference basis at points (for basis pins).
+void bref_basis_eval(void* h, int npts, const double* x, double* out /*npts*N*/, double* dout /*npts*N*D*/) {
+  Model* m = (Model*)h; int D = m->D, N = m->N;
+  for (int k = 0; k < npts; k++)
+    for (int n = 0; n < N; n++) {
+      out[k * N + n] = poly_eval(m->T.polys[D][n], &x[k * D], D);
+      if (dout) for (int d = 0; d < D; d++) dout[(k * N + n) * D + d] = poly_deriv(m->T.polys[D][n], &x[k * D], D, d);
+    }
+}
+
+static Params make_params(const double* prm) {
+  // prm: [penalty_base, nu, vel0, vel1, vel2, gamma, mach]
+  Params P;
+  P.penalty_base = prm[0]; P.nu = prm[1]; P.vel[0] = prm[2]; P.vel[1] = prm[3]; P.vel[2] = prm[4]; P.gamma = prm[5]; P.mach = prm[6];
+  return P;
+}
+
+static void set_threads(int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+}
+
+int bref_num_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+int bref_evaluate(void* h, int kind, const double* prm, double alpha, double beta, const double* U, double* R, int nthreads) {
+  set_threads(nthreads);
+  Model* m = (Model*)h;
+  evaluate<double>(*m, kind, make_params(prm), alpha, beta, U, R);
+  return 0;
+}
+
+// JVP: JdU = dR/dU * dU  (forward-mode duals through the same evaluator)
+int bref_jvp(void* h, int kind, const double* prm, const double* U, const double* dU, double* JdU, int nthreads) {
+  set_threads(nthreads);
+  Model* m = (Model*)h;
+  size_t n = (size_t)m->J * nvars(kind, m->D) * m->N;
+  std::vector<Dual<1>> u(n), r(n);
+  for (size_t i = 0; i < n; i++) { u[i] = Dual<1>(U[i]); u[i].d[0] = dU[i]; }
+  evaluate<Dual<1>>(*m, kind, make_params(prm), 1.0, 0.0, u.data(), r.data());
+  for (size_t i = 0; i < n; i++) JdU[i] = r[i].d[0];
+  return 0;
+}
+
+int bref_vjp(void* h, int kind, const double* prm, const double* U, const double* W, double* out, int nthreads) {
+  set_threads(nthreads);
+  vjp(*(Model*)h, kind, make_params(prm), U, W, out);
+  return 0;
+}
+
+// Assembled Jacobian blocks (exact; equal to the operator matrix for linear forms):
+// diag[J][Nt][Nt] (volume + all edge self-couplings), offd[E][2][Nt][Nt] (0: rows IN, cols OUT; 1: rows OUT, cols IN).
+int bref_assemble(void* h, int kind, const double* prm, const double* U, double* diag, double* offd, int nthreads) {
+  set_threads(nthreads);
+  Model* m = (Model*)h;
+  Params P = make_params(prm);
+  int Nt = nvars(kind, m->D) * m->N;
+  size_t bs = (size_t)Nt * Nt;
+#pragma omp parallel
+  {
+    std::vector<double> Jv(bs);
+#pragma omp for
+    for (int j = 0; j < m->J; j++) {
+      volume_jacobian(*m, kind, P, j, U, Jv.data());
+      std::copy(Jv.begin(), Jv.end(), diag + j * bs);
+    }
+  }
+  std::vector<double> Je(4 * bs);
+  for (int e = 0; e < m->E; e++) {
+    edge_jacobian(*m, kind, P, e, U, Je.data());
+    int ji = m->cin[e], jo = m->cout_[e];
+    for (size_t i = 0; i < bs; i++) diag[ji * bs + i] += Je[0 * bs + i];
+    if (jo >= 0) {
+      for (size_t i = 0; i < bs; i++) diag[jo * bs + i] += Je[3 * bs + i];
+      for (size_t i = 0; i < bs; i++) { offd[(size_t)e * 2 * bs + i] = Je[1 * bs + i]; offd[(size_t)e * 2 * bs + bs + i] = Je[2 * bs + i]; }
+    } else {
+      for (size_t i = 0; i < 2 * bs; i++) offd[(size_t)e * 2 * bs + i] = 0;
+    }
+  }
+  return 0;
+}
+
+// Batched uneven block LU (row-major blocks at offsets off[b] with sizes n[b]).
+int bref_block_lu(int nblk, const int* n, const int64_t* off, double* A, int* piv, const int64_t* pivoff, int* info) {
+  int bad = 0;
+  for (int b = 0; b < nblk; b++) {
+    info[b] = lu_factor(n[b], A + off[b], piv + pivoff[b]);
+    if (info[b] && !bad) bad = b + 1;
+  }
+  return bad;
+}
+int bref_block_solve(int nblk, const int* n, const int64_t* off, const double* LU, const int* piv, const int64_t* pivoff,
+                     const int64_t* xoff, double* x) {
+  for (int b = 0; b < nblk; b++) lu_solve(n[b], LU + off[b], piv + pivoff[b], x + xoff[b]);
+  return 0;
+}
+
+}  // extern "C"
