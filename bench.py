@@ -1,0 +1,149 @@
+#!/usr/bin/env python3
+"""Headline benchmark: GDOF/s of the DG residual + VJP on one MI355X.
+
+Workload (default): 3D compressible Euler, HLLC flux, 64^3 periodic hex mesh of
+the unit cube, p = 3 (20 modes x 5 variables = 26 214 400 DOF), FP64.  One step
+is one fused pass computing the residual R(U) and the vector-Jacobian product
+J(U)^T W over the whole mesh, with U and W resident in HBM.
+
+Prints ONE JSON line (see the keys below).  The roofline entry prices the fused
+kernel's dense-contraction flops (the reference formulation, 2x the forward
+count) against the MI355X FP64 peak; per-launch durations come from HIP events
+on the stream the kernel runs on.  The CPU baseline is the C++ restatement of
+the reference evaluator (oracle/) timed on a bounded sample on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6
+METRIC = "GDOF/s DG residual apply + VJP (FP64 vs mixed) at fixed accuracy, % roofline"
+
+
+def cpu_baseline(form, p, budget_s=15.0, n_max=64):
+    """Time the oracle's residual + VJP on a periodic n^3 sample sized to ~budget_s."""
+    import numpy as np
+    from bosss_amd.grid import CartesianGrid
+    from bosss_amd.operator import default_quad_order
+    from bosss_amd.refelement import num_modes
+    from bosss_amd.workloads import euler_state, uniform_coefficients
+    from oracle.bosss_ref import RefModel, params_vector, OP_EULER_HLLC, OP_EULER_LF, OP_ADVDIFF, OP_SIP, _load
+
+    kind = {"euler_hllc": OP_EULER_HLLC, "euler_lf": OP_EULER_LF, "advdiff": OP_ADVDIFF, "sip": OP_SIP}[form]
+    order = default_quad_order(form, p)
+    N = num_modes(3, p)
+    prm = params_vector(1.3, -0.01, (1.0, 0.5, 0.25), 1.4)
+
+    def run(n):
+        g = CartesianGrid((n,) * 3, periodic=[True] * 3)
+        m = RefModel(3, (n,) * 3, p, order, periodic=[1, 1, 1])
+        U = euler_state(g, N, seed=1) if form.startswith("euler") else uniform_coefficients(g.J * N, 1)
+        W = uniform_coefficients(U.size, 2)
+        t0 = time.perf_counter()
+        m.evaluate(kind, prm, U)
+        t1 = time.perf_counter()
+        m.vjp(kind, prm, U, W)
+        t2 = time.perf_counter()
+        return U.size, t1 - t0, t2 - t1
+
+    dof0, tr0, tv0 = run(8)                                  # calibration on 512 cells
+    per_cell = (tr0 + tv0) / 512.0
+    n = 8
+    while (n + 2) ** 3 * per_cell < budget_s and n < n_max:
+        n += 2
+    dof, tr, tv = run(n)
+    threads = _load().bref_num_threads()
+    return {
+        "value": dof / (tr + tv) / 1e9, "unit": "GDOF/s", "cores": int(threads), "kind": "port",
+        "sample": "%s p=%d, %d^3 periodic cells (%d DOF): residual %.3f s + VJP %.3f s (residual alone %.4f GDOF/s)"
+                  % (form, p, n, dof, tr, tv, dof / tr / 1e9),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=64, help="cells per direction")
+    ap.add_argument("--p", type=int, default=3)
+    ap.add_argument("--form", default="euler_hllc", choices=["euler_hllc", "euler_lf", "advdiff", "sip"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the multi-GPU sharding is a design, not a code path); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from bosss_amd.grid import CartesianGrid
+    from bosss_amd.operator import DifferentialOperator
+    from bosss_amd.workloads import euler_state, uniform_coefficients, contraction_flops
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    n, p, form = args.n, args.p, args.form
+    grid = CartesianGrid((n,) * 3, periodic=[True] * 3)
+    op = DifferentialOperator(grid, form, p, penalty=1.3, nu=-0.01, velocity=(1.0, 0.5, 0.25), gamma=1.4)
+    U = euler_state(grid, op.N, seed=0) if form.startswith("euler") else uniform_coefficients(op.ndof, 0)
+    W = uniform_coefficients(op.ndof, 1)
+    Ut = torch.from_numpy(U).to(dev)
+    Wt = torch.from_numpy(W).to(dev)
+    Rt = torch.zeros_like(Ut)
+    Gt = torch.zeros_like(Ut)
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        op.evaluate_vjp_dev(Ut, Rt, Wt, Gt)
+    torch.cuda.synchronize()
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        op.evaluate_vjp_dev(Ut, Rt, Wt, Gt)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kernel_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / args.steps
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = op.ndof / (ms_per_step * 1e-3) / 1e9
+
+    flops = 2 * contraction_flops(form, 3, p, grid.J, grid.E)  # residual + VJP
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "GDOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "3D %s, %d^3 periodic hex mesh of the unit cube, p=%d (%d modes x %d vars, %d DOF), "
+                               "fused residual + VJP per step" % (form, n, n, p, op.N, op.nvars, op.ndof),
+                   "mesh": [n, n, n], "degree": p, "form": form, "dof": op.ndof},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "k_adjoint<FormEuler<3,HLLC>,3,20,5,double,fused>" if form == "euler_hllc" else form,
+                     "kernel_ms": round(kernel_ms, 4), "alg_flops_per_launch": flops},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(form, p)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

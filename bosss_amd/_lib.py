@@ -1,0 +1,92 @@
+"""ctypes binding of ``libbosss_hip.so`` (the C ABI declared in include/bosss_hip.h).
+
+The product path has no CPU fallback: if the HIP library is missing this module
+raises on import of the operator API.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libbosss_hip.so")
+
+BOSSS_FORM_SIP, BOSSS_FORM_ADVDIFF, BOSSS_FORM_EULER_LF, BOSSS_FORM_EULER_HLLC = 0, 1, 2, 3
+BOSSS_FP64, BOSSS_FP32 = 0, 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("D", ctypes.c_int32), ("J", ctypes.c_int32), ("E", ctypes.c_int32),
+                ("detJ", _dp), ("invJ", _dp), ("scaling", _dp), ("lenscale", _dp),
+                ("cells", _ip), ("trafo", _ip), ("normals", _dp), ("sqrtgram", _dp)]
+
+
+class TablesDesc(ctypes.Structure):
+    _fields_ = [("D", ctypes.c_int32), ("degree", ctypes.c_int32), ("N", ctypes.c_int32), ("Kv", ctypes.c_int32),
+                ("Ke", ctypes.c_int32), ("ntrafo", ctypes.c_int32),
+                ("vol_weights", _dp), ("phi", _dp), ("dphi", _dp), ("face_weights", _dp), ("trace", _dp), ("dtrace", _dp)]
+
+
+class FormParams(ctypes.Structure):
+    _fields_ = [("penalty_base", ctypes.c_double), ("nu", ctypes.c_double), ("velocity", ctypes.c_double * 3),
+                ("gamma", ctypes.c_double), ("mach", ctypes.c_double)]
+
+
+# symbol -> (restype, argtypes); must match include/bosss_hip.h
+SIGNATURES = {
+    "bosss_hip_last_error": (ctypes.c_char_p, []),
+    "bosss_hip_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "bosss_hip_ctx_destroy": (ctypes.c_int, [_vp]),
+    "bosss_hip_sync": (ctypes.c_int, [_vp]),
+    "bosss_hip_ctx_stream": (_vp, [_vp]),
+    "bosss_hip_grid_upload": (ctypes.c_int, [_vp, ctypes.POINTER(GridDesc), ctypes.POINTER(_vp)]),
+    "bosss_hip_grid_destroy": (ctypes.c_int, [_vp]),
+    "bosss_hip_op_create": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(TablesDesc), ctypes.c_int, ctypes.POINTER(FormParams),
+                                            ctypes.c_int, ctypes.POINTER(_vp)]),
+    "bosss_hip_op_destroy": (ctypes.c_int, [_vp]),
+    "bosss_hip_op_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32),
+                                         ctypes.POINTER(ctypes.c_int32)]),
+    "bosss_hip_evaluate": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]),
+    "bosss_hip_jvp": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "bosss_hip_vjp": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "bosss_hip_evaluate_dev": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp, _vp, _vp]),
+    "bosss_hip_jvp_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "bosss_hip_vjp_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "bosss_hip_evaluate_vjp_dev": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp, _vp, _vp, _vp, _vp]),
+    "bosss_hip_block_lu_dev": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32, _vp]),
+    "bosss_hip_block_solve_dev": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "bosss_hip_block_gemv_dev": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp]),
+    "bosss_hip_block_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+class BosssError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("bosss_hip error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise ImportError("libbosss_hip.so not found at %s; run __graft_entry__.build() first" % LIBPATH)
+        L = ctypes.CDLL(LIBPATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().bosss_hip_last_error().decode()
+        raise BosssError(rc, msg)
+    return rc

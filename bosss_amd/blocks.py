@@ -1,0 +1,63 @@
+"""Batched uneven cell-block factorisation and block-Jacobi (host mirror of BlockJacobi).
+
+BlockJacobi.InitImpl (BlockJacobi.cs:63-120) reads each cell's diagonal block
+(BlockMsrMatrix.ReadBlock) and inverts it in place (IMatrixExtensions.InvertInPlace ->
+LAPACK DGETRF + DGETRI); BlockJacobi.Solve (BlockJacobi.cs:148-197) sweeps
+    x <- omega * invDiag (b - (A - D) x) + (1 - omega) x.
+Here the factorisation of all blocks is one kernel launch on device-resident
+blocks, whatever mix of block sizes the batch holds (e.g. 10- and 20-DOF XDG cells).
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import lib
+
+
+class BlockBatch:
+    """Device-resident batch of dense square blocks of uneven sizes (torch tensors)."""
+
+    def __init__(self, sizes, device="cuda:0"):
+        import torch
+        self.sizes = np.asarray(sizes, dtype=np.int32)
+        if self.sizes.min() < 1 or self.sizes.max() > 128:
+            raise ValueError("block sizes must be in 1..128")
+        n64 = self.sizes.astype(np.int64)
+        off = np.zeros(len(n64), np.int64); off[1:] = np.cumsum(n64 * n64)[:-1]
+        xoff = np.zeros(len(n64), np.int64); xoff[1:] = np.cumsum(n64)[:-1]
+        self.nblk = len(self.sizes)
+        self.nmax = int(self.sizes.max())
+        self.total = int((n64 * n64).sum())
+        self.ndof = int(n64.sum())
+        self.dev = torch.device(device)
+        self.n_t = torch.from_numpy(self.sizes).to(self.dev)
+        self.off_t = torch.from_numpy(off).to(self.dev)
+        self.xoff_t = torch.from_numpy(xoff).to(self.dev)
+        self.off, self.xoff = off, xoff
+
+    @staticmethod
+    def _p(t):
+        return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+    def _stream(self):
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def factor(self, A, ipiv, info, inverse=False):
+        """In-place LU (or inverse) of all blocks of A (float64 tensor, packed row-major)."""
+        rc = lib().bosss_hip_block_lu_dev(None, self.nblk, self.nmax, self._p(self.n_t), self._p(self.off_t), self._p(A),
+                                           self._p(ipiv), self._p(self.xoff_t), self._p(info), 1 if inverse else 0, self._stream())
+        if rc != 0:
+            raise RuntimeError("block LU failed (%d): %s" % (rc, lib().bosss_hip_block_last_error().decode()))
+
+    def solve(self, LU, ipiv, x):
+        rc = lib().bosss_hip_block_solve_dev(None, self.nblk, self._p(self.n_t), self._p(self.off_t), self._p(LU), self._p(ipiv),
+                                              self._p(self.xoff_t), self._p(self.xoff_t), self._p(x), self._stream())
+        if rc != 0:
+            raise RuntimeError("block solve failed (%d): %s" % (rc, lib().bosss_hip_block_last_error().decode()))
+
+    def gemv(self, Ainv, x, y, alpha=1.0, beta=0.0):
+        rc = lib().bosss_hip_block_gemv_dev(None, self.nblk, self._p(self.n_t), self._p(self.off_t), self._p(Ainv), self._p(self.xoff_t),
+                                             alpha, self._p(x), beta, self._p(y), self._stream())
+        if rc != 0:
+            raise RuntimeError("block gemv failed (%d): %s" % (rc, lib().bosss_hip_block_last_error().decode()))

@@ -1,0 +1,209 @@
+// Batched uneven cell-block LU, inverse and block-Jacobi sweep (BlockJacobi.cs:63-197,
+// IMatrixExtensions.InvertInPlace:2128-2230 -> LAPACK DGETRF/DGETRI semantics).
+//
+// One wavefront owns one block.  The block is staged in LDS with an odd row
+// stride (no bank conflicts on column access); lane i owns rows i and i+64.
+// Partial pivoting picks the first row with the largest |a| in the column,
+// as LAPACK's idamax does; a zero pivot sets INFO = j+1 (1-based) and the
+// factorisation continues, like DGETF2.  Pivots are stored 1-based (ipiv).
+//
+// All blocks of a batch are processed by one launch (blocks of different size
+// share the launch; each wave reads its own size), so a whole block-Jacobi set
+// of uneven XDG blocks (10x10 and 20x20) is one kernel.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/bosss_hip.h"
+
+namespace bosss {
+
+constexpr int LU_MAXN = 128;
+
+template <typename Real>
+__device__ __forceinline__ void wave_argmax(Real& v, int& idx) {
+  // reduce (|a|, index) over the 64 lanes; ties keep the smaller index
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    Real ov = __shfl_xor(v, o, 64);
+    int oi = __shfl_xor(idx, o, 64);
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+  }
+}
+
+// In-place LU of the block in LDS (row stride ld).  Returns INFO (0 or first zero pivot, 1-based).
+template <typename Real>
+__device__ int lds_getrf(Real* A, int n, int ld, int* piv) {
+  const int lane = threadIdx.x & 63;
+  int info = 0;
+  for (int j = 0; j < n; j++) {
+    Real best = Real(-1);
+    int bi = n;
+    for (int i = lane; i < n; i += 64)
+      if (i >= j) {
+        Real v = fabs(A[i * ld + j]);
+        if (v > best || (v == best && i < bi)) { best = v; bi = i; }
+      }
+    wave_argmax(best, bi);
+    if (bi >= n) bi = j;   // no comparable candidate (NaN column): treated as a zero pivot below
+    if (lane == 0) piv[j] = bi;
+    __syncthreads();
+    if (!(A[bi * ld + j] != Real(0))) {
+      if (!info) info = j + 1;
+      __syncthreads();
+      continue;
+    }
+    if (bi != j)
+      for (int c = lane; c < n; c += 64) { Real t = A[j * ld + c]; A[j * ld + c] = A[bi * ld + c]; A[bi * ld + c] = t; }
+    __syncthreads();
+    const Real inv = Real(1) / A[j * ld + j];
+    for (int i = lane; i < n; i += 64)
+      if (i > j) {
+        Real l = A[i * ld + j] * inv;
+        A[i * ld + j] = l;
+        for (int c = j + 1; c < n; c++) A[i * ld + c] -= l * A[j * ld + c];
+      }
+    __syncthreads();
+  }
+  return info;
+}
+
+template <typename Real>
+__global__ void __launch_bounds__(64) k_block_lu(int nblk, int nmax, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
+                                                 Real* __restrict__ A, int32_t* __restrict__ ipiv, const int64_t* __restrict__ pivoff,
+                                                 int32_t* __restrict__ info, int want_inverse) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int b = blockIdx.x;
+  if (b >= nblk) return;
+  const int n = nsz[b];
+  if (n <= 0 || n > nmax) {          // size outside what the launch was sized for: report, touch nothing
+    if (threadIdx.x == 0 && info) info[b] = -1;
+    return;
+  }
+  const int ld = n | 1;
+  const int cap = nmax * (nmax | 1);
+  Real* L = reinterpret_cast<Real*>(smem_raw);
+  Real* X = L + cap;                                        // inverse workspace (only if want_inverse)
+  int* piv = reinterpret_cast<int*>(L + (want_inverse ? 2 : 1) * cap);
+  Real* g = A + off[b];
+  const int lane = threadIdx.x;
+  for (int i = lane; i < n * n; i += 64) L[(i / n) * ld + (i % n)] = g[i];
+  __syncthreads();
+  int inf = lds_getrf(L, n, ld, piv);
+  if (want_inverse && inf == 0) {
+    // DGETRI result: column c of inv(A) solves A x = e_c; lane c owns column c of X.
+    for (int i = lane; i < n * n; i += 64) X[(i / n) * ld + (i % n)] = Real((i / n) == (i % n) ? 1 : 0);
+    __syncthreads();
+    for (int c = lane; c < n; c += 64) {
+      for (int j = 0; j < n; j++) { int p = piv[j]; if (p != j) { Real t = X[j * ld + c]; X[j * ld + c] = X[p * ld + c]; X[p * ld + c] = t; } }
+      for (int i = 0; i < n; i++) { Real a = X[i * ld + c]; for (int k = 0; k < i; k++) a -= L[i * ld + k] * X[k * ld + c]; X[i * ld + c] = a; }
+      for (int i = n - 1; i >= 0; i--) {
+        Real a = X[i * ld + c];
+        for (int k = i + 1; k < n; k++) a -= L[i * ld + k] * X[k * ld + c];
+        X[i * ld + c] = a / L[i * ld + i];
+      }
+    }
+    __syncthreads();
+    for (int i = lane; i < n * n; i += 64) g[i] = X[(i / n) * ld + (i % n)];
+  } else {
+    for (int i = lane; i < n * n; i += 64) g[i] = L[(i / n) * ld + (i % n)];
+  }
+  if (ipiv)
+    for (int j = lane; j < n; j += 64) ipiv[pivoff[b] + j] = piv[j] + 1;
+  if (lane == 0 && info) info[b] = inf;
+}
+
+size_t lu_lds_bytes(int nmax, int want_inverse) {
+  return (size_t)(want_inverse ? 2 : 1) * nmax * (nmax | 1) * sizeof(double) + (size_t)nmax * sizeof(int);
+}
+
+// x_b <- LU_b \ x_b   (getrs with the 1-based pivots)
+template <typename Real>
+__global__ void __launch_bounds__(64) k_block_solve(int nblk, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
+                                                    const Real* __restrict__ LU, const int32_t* __restrict__ ipiv,
+                                                    const int64_t* __restrict__ pivoff, const int64_t* __restrict__ xoff,
+                                                    Real* __restrict__ x) {
+  const int b = blockIdx.x * 64 + threadIdx.x;   // one lane per block (small blocks, many of them)
+  if (b >= nblk) return;
+  const int n = nsz[b];
+  const Real* L = LU + off[b];
+  const int32_t* pv = ipiv + pivoff[b];
+  Real* xb = x + xoff[b];
+  for (int j = 0; j < n; j++) { int p = pv[j] - 1; if (p != j) { Real t = xb[j]; xb[j] = xb[p]; xb[p] = t; } }
+  for (int i = 0; i < n; i++) { Real a = xb[i]; for (int k = 0; k < i; k++) a -= L[i * n + k] * xb[k]; xb[i] = a; }
+  for (int i = n - 1; i >= 0; i--) { Real a = xb[i]; for (int k = i + 1; k < n; k++) a -= L[i * n + k] * xb[k]; xb[i] = a / L[i * n + i]; }
+}
+
+// y_b <- alpha * Ainv_b x_b + beta * y_b   (block-diagonal apply with explicit inverses, one wave per block)
+template <typename Real>
+__global__ void __launch_bounds__(64) k_block_gemv(int nblk, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
+                                                   const Real* __restrict__ Ainv, const int64_t* __restrict__ xoff, Real alpha,
+                                                   const Real* __restrict__ xin, Real beta, Real* __restrict__ y) {
+  const int b = blockIdx.x;
+  if (b >= nblk) return;
+  const int n = nsz[b];
+  const Real* M = Ainv + off[b];
+  const Real* xb = xin + xoff[b];
+  Real* yb = y + xoff[b];
+  for (int i = threadIdx.x; i < n; i += 64) {
+    Real a = 0;
+    for (int k = 0; k < n; k++) a += M[i * n + k] * xb[k];
+    yb[i] = alpha * a + (beta != Real(0) ? beta * yb[i] : Real(0));
+  }
+}
+
+}  // namespace bosss
+
+using namespace bosss;
+
+namespace {
+thread_local std::string g_lu_err;
+}
+
+extern "C" {
+
+// Host-side error text of the block routines (the context-level message lives in bosss_hip.hip).
+const char* bosss_hip_block_last_error(void) { return g_lu_err.c_str(); }
+
+int bosss_hip_block_lu_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t* n, const int64_t* off, double* A,
+                           int32_t* ipiv, const int64_t* pivoff, int32_t* info, int32_t want_inverse, void* stream) {
+  (void)ctx;
+  if (nblk <= 0) return BOSSS_OK;
+  if (nmax <= 0 || nmax > LU_MAXN) { g_lu_err = "block size out of range (1..128)"; return BOSSS_ERR_ARG; }
+  size_t lds = lu_lds_bytes(nmax, want_inverse);
+  if (lds > 160 * 1024) { g_lu_err = "block too large for an in-LDS inverse (n <= 100)"; return BOSSS_ERR_UNSUPPORTED; }
+  if (hipFuncSetAttribute((const void*)k_block_lu<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
+    g_lu_err = "hipFuncSetAttribute failed"; return BOSSS_ERR_HIP;
+  }
+  hipLaunchKernelGGL(k_block_lu<double>, dim3(nblk), dim3(64), lds, (hipStream_t)stream, nblk, nmax, n, off, A, ipiv, pivoff, info,
+                     want_inverse);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
+  return BOSSS_OK;
+}
+
+int bosss_hip_block_solve_dev(bosss_ctx* ctx, int nblk, const int32_t* n, const int64_t* off, const double* LU, const int32_t* ipiv,
+                              const int64_t* pivoff, const int64_t* xoff, double* x, void* stream) {
+  (void)ctx;
+  if (nblk <= 0) return BOSSS_OK;
+  hipLaunchKernelGGL(k_block_solve<double>, dim3((nblk + 63) / 64), dim3(64), 0, (hipStream_t)stream, nblk, n, off, LU, ipiv,
+                     pivoff, xoff, x);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
+  return BOSSS_OK;
+}
+
+int bosss_hip_block_gemv_dev(bosss_ctx* ctx, int nblk, const int32_t* n, const int64_t* off, const double* Ainv, const int64_t* xoff,
+                             double alpha, const double* x, double beta, double* y, void* stream) {
+  (void)ctx;
+  if (nblk <= 0) return BOSSS_OK;
+  hipLaunchKernelGGL(k_block_gemv<double>, dim3(nblk), dim3(64), 0, (hipStream_t)stream, nblk, n, off, Ainv, xoff, alpha, x, beta, y);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
+  return BOSSS_OK;
+}
+
+}  // extern "C"

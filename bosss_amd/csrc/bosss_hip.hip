@@ -1,0 +1,399 @@
+// C ABI of the gfx950 DG operator library (declarations: include/bosss_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/bosss_hip.h"
+#include "dg_kernels.hpp"
+#include "registry.hpp"
+
+using namespace bosss;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                                                  \
+  do {                                                                                             \
+    hipError_t e_ = (x);                                                                           \
+    if (e_ != hipSuccess) return fail(BOSSS_ERR_HIP, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+template <class T>
+int upload(T** dst, const T* src, size_t n) {
+  HIPCHK(hipMalloc((void**)dst, n * sizeof(T) + 16));
+  if (src) HIPCHK(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return BOSSS_OK;
+}
+
+}  // namespace
+
+struct bosss_ctx {
+  int device;
+  hipStream_t stream;
+  int num_cu;
+};
+
+struct bosss_grid {
+  bosss_ctx* ctx;
+  int D, J, E, NF;
+  bool has_boundary;
+  double *detJ, *invJ, *scaling, *lenscale;
+  int* nb;          // [J][NF]
+  double* sqrtg;    // [J][NF]
+  double* normal;   // [J][NF][D]
+  int pair[6];
+};
+
+struct bosss_op {
+  bosss_ctx* ctx;
+  const bosss_grid* grid;
+  int kind, D, N, K1, V, precision;
+  const KernelSet* ks;
+  void* tables;      // packed tables (Real) on the device
+  FormParams fp;
+  size_t ndof;
+  // staging buffers for the host-pointer entry points
+  double *hU, *hR, *hX, *hY;
+};
+
+extern "C" {
+
+const char* bosss_hip_last_error(void) { return g_err.c_str(); }
+
+int bosss_hip_ctx_create(int device, bosss_ctx** ctx) {
+  if (!ctx) return fail(BOSSS_ERR_ARG, "ctx is NULL");
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(BOSSS_ERR_ARG, "device %d out of range (%d devices)", device, n);
+  HIPCHK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(BOSSS_ERR_UNSUPPORTED, "device %d is %s; this library is built for gfx950 only", device, prop.gcnArchName);
+  bosss_ctx* c = new bosss_ctx();
+  c->device = device;
+  c->num_cu = prop.multiProcessorCount;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete c; return fail(BOSSS_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
+  *ctx = c;
+  return BOSSS_OK;
+}
+
+int bosss_hip_ctx_destroy(bosss_ctx* ctx) {
+  if (!ctx) return BOSSS_OK;
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return BOSSS_OK;
+}
+
+int bosss_hip_sync(bosss_ctx* ctx) {
+  if (!ctx) return fail(BOSSS_ERR_ARG, "ctx is NULL");
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BOSSS_OK;
+}
+
+void* bosss_hip_ctx_stream(bosss_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int bosss_hip_grid_upload(bosss_ctx* ctx, const bosss_grid_desc* g, bosss_grid** out) {
+  if (!ctx || !g || !out) return fail(BOSSS_ERR_ARG, "NULL argument");
+  const int D = g->D, J = g->J, E = g->E;
+  if (D < 2 || D > 3 || J <= 0 || E < 0) return fail(BOSSS_ERR_ARG, "bad sizes D=%d J=%d E=%d", D, J, E);
+  if (!g->detJ || !g->invJ || !g->scaling || !g->lenscale || !g->cells || !g->trafo || !g->normals || !g->sqrtgram)
+    return fail(BOSSS_ERR_ARG, "grid descriptor has NULL arrays");
+  const int NF = 2 * D;
+  // Cell-centric face table: local face slot = trafo index of the cell's side.
+  std::vector<int> nb((size_t)J * NF, -2);
+  std::vector<double> sg((size_t)J * NF, 0.0), nrm((size_t)J * NF * D, 0.0);
+  int pair[6] = {-1, -1, -1, -1, -1, -1};
+  bool has_boundary = false;
+  for (int e = 0; e < E; e++) {
+    int ci = g->cells[2 * e], co = g->cells[2 * e + 1];
+    int ti = g->trafo[2 * e], to = g->trafo[2 * e + 1];
+    if (ci < 0 || ci >= J || co >= J || ti < 0 || ti >= NF) return fail(BOSSS_ERR_ARG, "edge %d: bad cell/trafo index", e);
+    if (nb[(size_t)ci * NF + ti] != -2) return fail(BOSSS_ERR_UNSUPPORTED, "cell %d has two edges on face %d (non-conforming grid)", ci, ti);
+    nb[(size_t)ci * NF + ti] = co;
+    sg[(size_t)ci * NF + ti] = g->sqrtgram[e];
+    for (int d = 0; d < D; d++) nrm[((size_t)ci * NF + ti) * D + d] = g->normals[(size_t)e * D + d];
+    if (co >= 0) {
+      if (to < 0 || to >= NF) return fail(BOSSS_ERR_ARG, "edge %d: bad OUT trafo", e);
+      if (nb[(size_t)co * NF + to] != -2) return fail(BOSSS_ERR_UNSUPPORTED, "cell %d has two edges on face %d (non-conforming grid)", co, to);
+      nb[(size_t)co * NF + to] = ci;
+      sg[(size_t)co * NF + to] = g->sqrtgram[e];
+      for (int d = 0; d < D; d++) nrm[((size_t)co * NF + to) * D + d] = -g->normals[(size_t)e * D + d];
+      if (pair[ti] < 0) pair[ti] = to;
+      if (pair[to] < 0) pair[to] = ti;
+      if (pair[ti] != to || pair[to] != ti)
+        return fail(BOSSS_ERR_UNSUPPORTED, "edge %d: face pairing %d<->%d differs from the rest of the grid", e, ti, to);
+    } else {
+      has_boundary = true;
+    }
+  }
+  for (size_t i = 0; i < nb.size(); i++)
+    if (nb[i] == -2) return fail(BOSSS_ERR_UNSUPPORTED, "cell %zu has no edge on local face %zu", i / NF, i % NF);
+  for (int f = 0; f < NF; f++)
+    if (pair[f] < 0) pair[f] = (f % 2 == 0) ? f + 1 : f - 1;  // only boundary edges on this face: pairing unused
+
+  HIPCHK(hipSetDevice(ctx->device));
+  bosss_grid* G = new bosss_grid();
+  G->ctx = ctx; G->D = D; G->J = J; G->E = E; G->NF = NF; G->has_boundary = has_boundary;
+  for (int f = 0; f < 6; f++) G->pair[f] = f < NF ? pair[f] : 0;
+  int rc;
+  if ((rc = upload(&G->detJ, g->detJ, J)) || (rc = upload(&G->invJ, g->invJ, (size_t)J * D * D)) ||
+      (rc = upload(&G->scaling, g->scaling, J)) || (rc = upload(&G->lenscale, g->lenscale, J)) ||
+      (rc = upload(&G->nb, nb.data(), nb.size())) || (rc = upload(&G->sqrtg, sg.data(), sg.size())) ||
+      (rc = upload(&G->normal, nrm.data(), nrm.size()))) {
+    bosss_hip_grid_destroy(G);
+    return rc;
+  }
+  *out = G;
+  return BOSSS_OK;
+}
+
+int bosss_hip_grid_destroy(bosss_grid* g) {
+  if (!g) return BOSSS_OK;
+  (void)hipFree(g->detJ); (void)hipFree(g->invJ); (void)hipFree(g->scaling); (void)hipFree(g->lenscale);
+  (void)hipFree(g->nb); (void)hipFree(g->sqrtg); (void)hipFree(g->normal);
+  delete g;
+  return BOSSS_OK;
+}
+
+}  // extern "C"
+
+static int form_vars(int kind, int D) { return (kind == BOSSS_FORM_EULER_LF || kind == BOSSS_FORM_EULER_HLLC) ? D + 2 : 1; }
+
+template <typename Real>
+static void pack_tables(const bosss_tables_desc* t, const KernelSet* ks, std::vector<Real>& out) {
+  const int D = t->D, N = t->N, Kv = t->Kv, Ke = t->Ke, NF = 2 * D;
+  const int KVP = ks->kvp, KEP = ks->kep;
+  out.assign(ks->table_size, Real(0));
+  for (int k = 0; k < Kv; k++)
+    for (int n = 0; n < N; n++) {
+      out[ks->o_vphi + k * N + n] = Real(t->phi[k * N + n]);
+      for (int d = 0; d < D; d++) out[ks->o_vdphi + (d * KVP + k) * N + n] = Real(t->dphi[(k * N + n) * D + d]);
+    }
+  for (int k = 0; k < Kv; k++) out[ks->o_vw + k] = Real(t->vol_weights[k]);
+  for (int f = 0; f < NF; f++)
+    for (int k = 0; k < Ke; k++)
+      for (int n = 0; n < N; n++) {
+        out[ks->o_tr + (f * KEP + k) * N + n] = Real(t->trace[((size_t)f * Ke + k) * N + n]);
+        if (ks->facegrad)
+          for (int d = 0; d < D; d++)
+            out[ks->o_dtr + ((f * D + d) * KEP + k) * N + n] = Real(t->dtrace[(((size_t)f * Ke + k) * N + n) * D + d]);
+      }
+  for (int k = 0; k < Ke; k++) out[ks->o_fw + k] = Real(t->face_weights[k]);
+}
+
+extern "C" {
+
+int bosss_hip_op_create(bosss_ctx* ctx, const bosss_grid* grid, const bosss_tables_desc* t, int kind,
+                        const bosss_form_params* prm, int precision, bosss_op** out) {
+  if (!ctx || !grid || !t || !prm || !out) return fail(BOSSS_ERR_ARG, "NULL argument");
+  if (t->D != grid->D) return fail(BOSSS_ERR_ARG, "tables are %dD, grid is %dD", t->D, grid->D);
+  if (kind < 0 || kind > 3) return fail(BOSSS_ERR_UNSUPPORTED, "form kind %d has no GPU kernel", kind);
+  if (precision != BOSSS_FP64 && precision != BOSSS_FP32) return fail(BOSSS_ERR_ARG, "precision %d", precision);
+  if (t->ntrafo != 2 * t->D) return fail(BOSSS_ERR_UNSUPPORTED, "expected %d face trafos, got %d", 2 * t->D, t->ntrafo);
+  int K1 = (int)std::lround(std::pow((double)t->Kv, 1.0 / t->D));
+  int Ke1 = 1; for (int d = 0; d < t->D - 1; d++) Ke1 *= K1;
+  int Kv1 = Ke1 * K1;
+  if (Kv1 != t->Kv || Ke1 != t->Ke) return fail(BOSSS_ERR_UNSUPPORTED, "volume/face rules must be tensor Gauss rules (Kv=%d, Ke=%d)", t->Kv, t->Ke);
+  if ((kind == BOSSS_FORM_EULER_LF || kind == BOSSS_FORM_EULER_HLLC) && grid->has_boundary)
+    return fail(BOSSS_ERR_UNSUPPORTED, "Euler kernels cover periodic grids only (boundary edges present)");
+  const KernelSet* ks = find_kernels(kind, t->D, t->N, K1, precision);
+  if (!ks) return fail(BOSSS_ERR_UNSUPPORTED, "no kernel for kind=%d D=%d N=%d K1=%d precision=%d", kind, t->D, t->N, K1, precision);
+  if (ks->lds64 > 160 * 1024 || (precision == BOSSS_FP32 && ks->lds32 > 160 * 1024))
+    return fail(BOSSS_ERR_UNSUPPORTED, "tables need %zu bytes of LDS (> 160 KiB)", precision == BOSSS_FP64 ? ks->lds64 : ks->lds32);
+  if (!t->phi || !t->dphi || !t->vol_weights || !t->trace || !t->dtrace || !t->face_weights)
+    return fail(BOSSS_ERR_ARG, "tables descriptor has NULL arrays");
+  HIPCHK(hipSetDevice(ctx->device));
+  bosss_op* op = new bosss_op();
+  op->ctx = ctx; op->grid = grid; op->kind = kind; op->D = t->D; op->N = t->N; op->K1 = K1;
+  op->V = form_vars(kind, t->D); op->precision = precision; op->ks = ks;
+  op->ndof = (size_t)grid->J * op->V * op->N;
+  op->fp.penalty_base = prm->penalty_base;
+  op->fp.nu = prm->nu;
+  for (int d = 0; d < 3; d++) op->fp.vel[d] = prm->velocity[d];
+  op->fp.gamma = prm->gamma;
+  op->fp.mach = prm->mach;
+  {
+    double p = t->degree, D = t->D;
+    op->fp.penalty_deg = std::max((p + 1) * (p + D) / D, (p + 1.0) * (p + 1.0));
+  }
+  int rc;
+  if (precision == BOSSS_FP64) {
+    std::vector<double> packed;
+    pack_tables<double>(t, ks, packed);
+    rc = upload((double**)&op->tables, packed.data(), packed.size());
+  } else {
+    std::vector<float> packed;
+    pack_tables<float>(t, ks, packed);
+    rc = upload((float**)&op->tables, packed.data(), packed.size());
+  }
+  if (rc) { delete op; return rc; }
+  if ((rc = ks->prepare()) != 0) { (void)hipFree(op->tables); delete op; return fail(BOSSS_ERR_HIP, "kernel attribute setup failed (%d)", rc); }
+  *out = op;
+  return BOSSS_OK;
+}
+
+int bosss_hip_op_destroy(bosss_op* op) {
+  if (!op) return BOSSS_OK;
+  (void)hipFree(op->tables);
+  if (op->hU) (void)hipFree(op->hU);
+  if (op->hR) (void)hipFree(op->hR);
+  if (op->hX) (void)hipFree(op->hX);
+  if (op->hY) (void)hipFree(op->hY);
+  delete op;
+  return BOSSS_OK;
+}
+
+int bosss_hip_op_info(const bosss_op* op, int64_t* ndof, int32_t* nvars, int32_t* nmodes) {
+  if (!op) return fail(BOSSS_ERR_ARG, "op is NULL");
+  if (ndof) *ndof = (int64_t)op->ndof;
+  if (nvars) *nvars = op->V;
+  if (nmodes) *nmodes = op->N;
+  return BOSSS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- launching
+template <typename Real>
+static KArgs<Real> make_args(const bosss_op* op, double alpha, double beta, const void* U, const void* X, void* R, void* out) {
+  KArgs<Real> a;
+  const bosss_grid* g = op->grid;
+  a.J = g->J;
+  a.detJ = g->detJ; a.invJ = g->invJ; a.scaling = g->scaling; a.lenscale = g->lenscale;
+  a.cf.nb = g->nb; a.cf.sqrtg = g->sqrtg; a.cf.normal = g->normal;
+  for (int f = 0; f < 6; f++) a.cf.pair[f] = g->pair[f];
+  a.tables = (const Real*)op->tables;
+  a.fp = op->fp;
+  a.alpha = alpha; a.beta = beta;
+  a.U = (const Real*)U; a.dU = (const Real*)X; a.R = (Real*)R; a.out = (Real*)out;
+  return a;
+}
+
+static int launch(bosss_op* op, int which, double alpha, double beta, const void* U, const void* X, void* R, void* out, void* stream) {
+  hipStream_t s = stream ? (hipStream_t)stream : op->ctx->stream;
+  const int cells_per_block = 16 * (KernelSet::BLOCK / 64);
+  dim3 grid((op->grid->J + cells_per_block - 1) / cells_per_block), block(KernelSet::BLOCK);
+  HIPCHK(hipSetDevice(op->ctx->device));
+  if (op->precision == BOSSS_FP64) {
+    KArgs<double> a = make_args<double>(op, alpha, beta, U, X, R, out);
+    void (*k)(KArgs<double>) = op->ks->k64[which];
+    if (!k) return fail(BOSSS_ERR_UNSUPPORTED, "kernel variant %d not built", which);
+    hipLaunchKernelGGL(k, grid, block, op->ks->lds64, s, a);
+  } else {
+    KArgs<float> a = make_args<float>(op, alpha, beta, U, X, R, out);
+    void (*k)(KArgs<float>) = op->ks->k32[which];
+    if (!k) return fail(BOSSS_ERR_UNSUPPORTED, "kernel variant %d not built", which);
+    hipLaunchKernelGGL(k, grid, block, op->ks->lds32, s, a);
+  }
+  HIPCHK(hipGetLastError());
+  return BOSSS_OK;
+}
+
+extern "C" {
+
+int bosss_hip_evaluate_dev(bosss_op* op, double alpha, double beta, const double* U, double* R, void* stream) {
+  if (!op || !U || !R) return fail(BOSSS_ERR_ARG, "NULL argument");
+  return launch(op, KernelSet::RES, alpha, beta, U, nullptr, R, nullptr, stream);
+}
+int bosss_hip_jvp_dev(bosss_op* op, const double* U, const double* dU, double* JdU, void* stream) {
+  if (!op || !U || !dU || !JdU) return fail(BOSSS_ERR_ARG, "NULL argument");
+  return launch(op, KernelSet::JVP, 1.0, 0.0, U, dU, nullptr, JdU, stream);
+}
+int bosss_hip_vjp_dev(bosss_op* op, const double* U, const double* W, double* JtW, void* stream) {
+  if (!op || !U || !W || !JtW) return fail(BOSSS_ERR_ARG, "NULL argument");
+  return launch(op, KernelSet::VJP, 1.0, 0.0, U, W, nullptr, JtW, stream);
+}
+int bosss_hip_evaluate_vjp_dev(bosss_op* op, double alpha, double beta, const double* U, double* R, const double* W,
+                               double* JtW, void* stream) {
+  if (!op || !U || !R || !W || !JtW) return fail(BOSSS_ERR_ARG, "NULL argument");
+  return launch(op, KernelSet::RES_VJP, alpha, beta, U, W, R, JtW, stream);
+}
+
+static int ensure_staging(bosss_op* op) {
+  if (op->hU) return BOSSS_OK;
+  size_t bytes = op->ndof * (op->precision == BOSSS_FP64 ? 8 : 4) + 16;
+  HIPCHK(hipMalloc((void**)&op->hU, bytes));
+  HIPCHK(hipMalloc((void**)&op->hR, bytes));
+  HIPCHK(hipMalloc((void**)&op->hX, bytes));
+  HIPCHK(hipMalloc((void**)&op->hY, bytes));
+  return BOSSS_OK;
+}
+
+// Host-pointer entry points: FP64 arrays in and out; an FP32 operator converts at the boundary.
+static int to_device(bosss_op* op, void* dst, const double* src) {
+  hipStream_t s = op->ctx->stream;
+  if (op->precision == BOSSS_FP64) {
+    HIPCHK(hipMemcpyAsync(dst, src, op->ndof * 8, hipMemcpyHostToDevice, s));
+  } else {
+    std::vector<float> tmp(op->ndof);
+    for (size_t i = 0; i < op->ndof; i++) tmp[i] = (float)src[i];
+    HIPCHK(hipMemcpyAsync(dst, tmp.data(), op->ndof * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  return BOSSS_OK;
+}
+static int to_host(bosss_op* op, double* dst, const void* src) {
+  hipStream_t s = op->ctx->stream;
+  if (op->precision == BOSSS_FP64) {
+    HIPCHK(hipMemcpyAsync(dst, src, op->ndof * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  } else {
+    std::vector<float> tmp(op->ndof);
+    HIPCHK(hipMemcpyAsync(tmp.data(), src, op->ndof * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (size_t i = 0; i < op->ndof; i++) dst[i] = tmp[i];
+  }
+  return BOSSS_OK;
+}
+
+int bosss_hip_evaluate(bosss_op* op, double time, double alpha, double beta, const double* U, double* R) {
+  (void)time;  // the supported forms do not depend on time
+  if (!op || !U || !R) return fail(BOSSS_ERR_ARG, "NULL argument");
+  int rc;
+  HIPCHK(hipSetDevice(op->ctx->device));
+  if ((rc = ensure_staging(op)) || (rc = to_device(op, op->hU, U))) return rc;
+  if (beta != 0.0 && (rc = to_device(op, op->hR, R))) return rc;
+  if ((rc = launch(op, KernelSet::RES, alpha, beta, op->hU, nullptr, op->hR, nullptr, nullptr))) return rc;
+  return to_host(op, R, op->hR);
+}
+
+int bosss_hip_jvp(bosss_op* op, const double* U, const double* dU, double* JdU) {
+  if (!op || !U || !dU || !JdU) return fail(BOSSS_ERR_ARG, "NULL argument");
+  int rc;
+  HIPCHK(hipSetDevice(op->ctx->device));
+  if ((rc = ensure_staging(op)) || (rc = to_device(op, op->hU, U)) || (rc = to_device(op, op->hX, dU))) return rc;
+  if ((rc = launch(op, KernelSet::JVP, 1.0, 0.0, op->hU, op->hX, nullptr, op->hY, nullptr))) return rc;
+  return to_host(op, JdU, op->hY);
+}
+
+int bosss_hip_vjp(bosss_op* op, const double* U, const double* W, double* JtW) {
+  if (!op || !U || !W || !JtW) return fail(BOSSS_ERR_ARG, "NULL argument");
+  int rc;
+  HIPCHK(hipSetDevice(op->ctx->device));
+  if ((rc = ensure_staging(op)) || (rc = to_device(op, op->hU, U)) || (rc = to_device(op, op->hX, W))) return rc;
+  if ((rc = launch(op, KernelSet::VJP, 1.0, 0.0, op->hU, op->hX, nullptr, op->hY, nullptr))) return rc;
+  return to_host(op, JtW, op->hY);
+}
+
+}  // extern "C"

@@ -1,0 +1,15 @@
+// Euler (Rusanov and HLLC) instantiations: 3D p=2 (4 pts), p=3 (5 pts); 2D p=2 (4 pts).
+#include <vector>
+
+#include "registry.hpp"
+
+namespace bosss {
+void register_euler(std::vector<KernelSet>& v) {
+  v.push_back(make_kernel_set<FormEuler<2, false>, 2, 6, 4, false>(2));
+  v.push_back(make_kernel_set<FormEuler<2, true>, 2, 6, 4, false>(3));
+  v.push_back(make_kernel_set<FormEuler<3, false>, 3, 10, 4, true>(2));
+  v.push_back(make_kernel_set<FormEuler<3, true>, 3, 10, 4, true>(3));
+  v.push_back(make_kernel_set<FormEuler<3, false>, 3, 20, 5, true>(2));
+  v.push_back(make_kernel_set<FormEuler<3, true>, 3, 20, 5, true>(3));
+}
+}  // namespace bosss

@@ -1,0 +1,53 @@
+// Table of compiled kernel instantiations, keyed by (form, D, N, 1D Gauss points).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "dg_kernels.hpp"
+
+namespace bosss {
+
+struct KernelSet {
+  enum { RES = 0, JVP = 1, VJP = 2, RES_VJP = 3, NVARIANT = 4 };
+  static constexpr int BLOCK = 256;
+  int kind, D, N, K1;
+  int kvp, kep, table_size;
+  bool facegrad;
+  int o_vphi, o_vdphi, o_vw, o_tr, o_dtr, o_fw;
+  size_t lds64, lds32;
+  void (*k64[NVARIANT])(KArgs<double>);
+  void (*k32[NVARIANT])(KArgs<float>);
+  int prepare() const;
+};
+
+const KernelSet* find_kernels(int kind, int D, int N, int K1, int precision);
+
+template <class Form, int D, int N, int K1, bool WITH32>
+KernelSet make_kernel_set(int kind) {
+  typedef Shape<D, N, K1, Form::FACE_GRADU || Form::FACE_TESTG> S;
+  KernelSet k;
+  k.kind = kind; k.D = D; k.N = N; k.K1 = K1;
+  k.facegrad = Form::FACE_GRADU || Form::FACE_TESTG;
+  k.kvp = S::KVP; k.kep = S::KEP; k.table_size = S::SIZE;
+  k.o_vphi = S::O_VPHI; k.o_vdphi = S::O_VDPHI; k.o_vw = S::O_VW; k.o_tr = S::O_TR; k.o_dtr = S::O_DTR; k.o_fw = S::O_FW;
+  k.lds64 = (size_t)S::SIZE * sizeof(double);
+  k.lds32 = (size_t)S::SIZE * sizeof(float);
+  k.k64[KernelSet::RES] = k_forward<Form, D, N, K1, double, MODE_RES>;
+  k.k64[KernelSet::JVP] = k_forward<Form, D, N, K1, double, MODE_JVP>;
+  k.k64[KernelSet::VJP] = k_adjoint<Form, D, N, K1, double, false>;
+  k.k64[KernelSet::RES_VJP] = k_adjoint<Form, D, N, K1, double, true>;
+  for (int i = 0; i < KernelSet::NVARIANT; i++) k.k32[i] = nullptr;
+  if constexpr (WITH32) {
+    k.k32[KernelSet::RES] = k_forward<Form, D, N, K1, float, MODE_RES>;
+    k.k32[KernelSet::JVP] = k_forward<Form, D, N, K1, float, MODE_JVP>;
+    k.k32[KernelSet::VJP] = k_adjoint<Form, D, N, K1, float, false>;
+    k.k32[KernelSet::RES_VJP] = k_adjoint<Form, D, N, K1, float, true>;
+  }
+  return k;
+}
+
+// each translation unit kernels_<family>.hip provides one of these
+void register_sip(std::vector<KernelSet>& out);
+void register_advdiff(std::vector<KernelSet>& out);
+void register_euler(std::vector<KernelSet>& out);
+
+}  // namespace bosss

@@ -1,0 +1,187 @@
+"""DifferentialOperator / evaluator mirror over the HIP library.
+
+Mirrors the reference interface of the hot path:
+  DifferentialOperator(DomainVar, CodomainVar, QuadOrderFunc) + EquationComponents
+      (DifferentialOperator.cs:81, 391-452)
+  .GetEvaluatorEx(...)  -> IEvaluatorNonLin    (DifferentialOperator.cs:1013-1033)
+  .Evaluate(alpha, beta, output)                (IEvaluator.cs:81-97):  output <- beta*output + alpha*R(U)
+plus the linearisation the Newton solver needs (JVP replacing Newton.dirder,
+Newton.cs:1450-1500) and the adjoint (VJP), which the reference lacks.
+
+Only the forms with hand-written kernels are accepted; anything else raises
+NotSupportedException-style errors (the C# side keeps the reference CPU evaluator
+as the fallback for those).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .grid import CartesianGrid
+from .refelement import Tables
+
+FORMS = {"sip": _lib.BOSSS_FORM_SIP, "advdiff": _lib.BOSSS_FORM_ADVDIFF,
+         "euler_lf": _lib.BOSSS_FORM_EULER_LF, "euler_hllc": _lib.BOSSS_FORM_EULER_HLLC}
+
+
+def default_quad_order(form, p):
+    """QuadOrderFunc used by the reference applications (QuadOrderFunc.cs)."""
+    if form in ("sip", "advdiff"):
+        o = 2 * p                      # SumOfMaxDegrees(RoundUp): p + p, even
+        return o + (o % 2)
+    return 3 * p                       # NonLinearWithoutParameters(2): 2p + p (CNS Operator.cs:143-173)
+
+
+class Context:
+    _cache = {}
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib().bosss_hip_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+        self.device = device
+
+    @classmethod
+    def get(cls, device=0):
+        if device not in cls._cache:
+            cls._cache[device] = cls(device)
+        return cls._cache[device]
+
+    def sync(self):
+        check(lib().bosss_hip_sync(self.h))
+
+    @property
+    def stream(self):
+        return lib().bosss_hip_ctx_stream(self.h)
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+class DifferentialOperator:
+    """A DG operator made of one of the supported equation components on a Cartesian grid."""
+
+    def __init__(self, grid, form, degree, penalty=1.3, nu=1.0, velocity=(0.0, 0.0, 0.0), gamma=1.4, mach=None,
+                 quad_order=None, precision="fp64", device=0):
+        if form not in FORMS:
+            raise NotImplementedError("no GPU kernel for form %r" % (form,))
+        self.grid, self.form, self.degree = grid, form, degree
+        self.quad_order = default_quad_order(form, degree) if quad_order is None else quad_order
+        self.ctx = Context.get(device)
+        self.tables = Tables(grid.D, degree, self.quad_order)
+        self.precision = precision
+        mach = 1.0 / np.sqrt(gamma) if mach is None else mach
+        self.params = dict(penalty=penalty, nu=nu, velocity=tuple(velocity), gamma=gamma, mach=mach)
+        L = lib()
+        # grid upload
+        g = grid
+        self._keep = [np.ascontiguousarray(x) for x in (g.detJ, g.invJ, g.scaling, g.lenscale, g.cells, g.trafo, g.normals, g.sqrtgram)]
+        gd = _lib.GridDesc(g.D, g.J, g.E, _dp(self._keep[0]), _dp(self._keep[1]), _dp(self._keep[2]), _dp(self._keep[3]),
+                           _ip(self._keep[4]), _ip(self._keep[5]), _dp(self._keep[6]), _dp(self._keep[7]))
+        gh = ctypes.c_void_p()
+        check(L.bosss_hip_grid_upload(self.ctx.h, ctypes.byref(gd), ctypes.byref(gh)))
+        self._grid_h = gh
+        t = self.tables
+        tk = [np.ascontiguousarray(x) for x in (t.vw, t.phi, t.dphi, t.fw, t.trace, t.dtrace)]
+        td = _lib.TablesDesc(g.D, degree, t.N, t.Kv, t.Ke, t.ntrafo, *[_dp(x) for x in tk])
+        v = list(velocity) + [0.0] * (3 - len(velocity))
+        fp = _lib.FormParams(penalty, nu, (ctypes.c_double * 3)(*v), gamma, mach)
+        oh = ctypes.c_void_p()
+        prec = _lib.BOSSS_FP64 if precision == "fp64" else _lib.BOSSS_FP32
+        rc = L.bosss_hip_op_create(self.ctx.h, gh, ctypes.byref(td), FORMS[form], ctypes.byref(fp), prec, ctypes.byref(oh))
+        if rc != 0:
+            L.bosss_hip_grid_destroy(gh)
+            check(rc)
+        self.h = oh
+        ndof = ctypes.c_int64(); nv = ctypes.c_int32(); nm = ctypes.c_int32()
+        check(L.bosss_hip_op_info(oh, ctypes.byref(ndof), ctypes.byref(nv), ctypes.byref(nm)))
+        self.ndof, self.nvars, self.N = ndof.value, nv.value, nm.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bosss_hip_op_destroy(self.h)
+            lib().bosss_hip_grid_destroy(self._grid_h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- host API
+    def GetEvaluatorEx(self, U):
+        return EvaluatorNonLin(self, U)
+
+    def evaluate(self, U, alpha=1.0, beta=0.0, R=None, time=0.0):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        R = np.zeros(self.ndof) if R is None else R
+        assert U.size == self.ndof and R.size == self.ndof and R.dtype == np.float64 and R.flags.c_contiguous
+        check(lib().bosss_hip_evaluate(self.h, time, alpha, beta, _dp(U), _dp(R)))
+        return R
+
+    def jvp(self, U, dU):
+        U = np.ascontiguousarray(U, dtype=np.float64); dU = np.ascontiguousarray(dU, dtype=np.float64)
+        out = np.zeros(self.ndof)
+        check(lib().bosss_hip_jvp(self.h, _dp(U), _dp(dU), _dp(out)))
+        return out
+
+    def vjp(self, U, W):
+        U = np.ascontiguousarray(U, dtype=np.float64); W = np.ascontiguousarray(W, dtype=np.float64)
+        out = np.zeros(self.ndof)
+        check(lib().bosss_hip_vjp(self.h, _dp(U), _dp(W), _dp(out)))
+        return out
+
+    # ------------------------------------------------------- device-resident API
+    @staticmethod
+    def _ptr(t):
+        return ctypes.c_void_p(t.data_ptr())
+
+    def _check_dev(self, *ts):
+        import torch
+        want = torch.float64 if self.precision == "fp64" else torch.float32
+        for t in ts:
+            if not (t.is_cuda and t.dtype == want and t.is_contiguous() and t.numel() == self.ndof):
+                raise ValueError("expected a contiguous %s CUDA tensor with %d entries" % (want, self.ndof))
+
+    def _stream(self, stream):
+        if stream is None:
+            import torch
+            return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        return ctypes.c_void_p(stream)
+
+    def evaluate_dev(self, U, R, alpha=1.0, beta=0.0, stream=None):
+        self._check_dev(U, R)
+        check(lib().bosss_hip_evaluate_dev(self.h, alpha, beta, self._ptr(U), self._ptr(R), self._stream(stream)))
+
+    def jvp_dev(self, U, dU, out, stream=None):
+        self._check_dev(U, dU, out)
+        check(lib().bosss_hip_jvp_dev(self.h, self._ptr(U), self._ptr(dU), self._ptr(out), self._stream(stream)))
+
+    def vjp_dev(self, U, W, out, stream=None):
+        self._check_dev(U, W, out)
+        check(lib().bosss_hip_vjp_dev(self.h, self._ptr(U), self._ptr(W), self._ptr(out), self._stream(stream)))
+
+    def evaluate_vjp_dev(self, U, R, W, out, alpha=1.0, beta=0.0, stream=None):
+        self._check_dev(U, R, W, out)
+        check(lib().bosss_hip_evaluate_vjp_dev(self.h, alpha, beta, self._ptr(U), self._ptr(R), self._ptr(W), self._ptr(out),
+                                               self._stream(stream)))
+
+
+class EvaluatorNonLin:
+    """IEvaluatorNonLin: Evaluate(alpha, beta, output) with the domain state fixed at creation."""
+
+    def __init__(self, op, U):
+        self.op = op
+        self.U = np.ascontiguousarray(U, dtype=np.float64)
+        self.time = 0.0
+
+    def Evaluate(self, alpha, beta, output):
+        self.op.evaluate(self.U, alpha, beta, output, self.time)
+        return output

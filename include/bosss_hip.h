@@ -1,0 +1,142 @@
+/* bosss_hip.h -- C ABI of the MI355X (gfx950) DG spatial-operator library.
+ *
+ * Drop-in boundary for the BoSSS DG operator hot path.  A C# host binds these
+ * symbols by name through a DynLibLoader subclass (MPI.Wrappers/Utils/DynLibLoader.cs:30-450,
+ * the same mechanism as BLAS.cs:1065-1090 / LAPACK.cs:28-66).  Conventions follow
+ * that precedent:
+ *   - every function returns int: 0 = ok, < 0 = error (message via
+ *     bosss_hip_last_error()), > 0 = LAPACK-style INFO (1-based pivot index of
+ *     the first singular block);
+ *   - host pointers are only read/written during the call (managed arrays are
+ *     pinned with `fixed` for one call only, DGField_Evaluation.cs:78-110);
+ *   - a context is bound to one host thread and one GPU; calls on one context
+ *     must not overlap (evaluators are not re-entrant, NECQuadratureVolume.cs:191-283).
+ *
+ * Coefficient vectors use the reference layout of UnsetteledCoordinateMapping
+ * (UnsetteledCoordinateMapping.cs:696-706): entry (cell j, variable f, mode n)
+ * is at j*V*N + f*N + n (all variables share one basis of N modes).
+ */
+#ifndef BOSSS_HIP_H
+#define BOSSS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BOSSS_OK 0
+#define BOSSS_ERR_ARG (-1)         /* invalid argument / size mismatch          */
+#define BOSSS_ERR_HIP (-2)         /* HIP runtime error                          */
+#define BOSSS_ERR_UNSUPPORTED (-3) /* form/degree/grid not covered by a kernel  */
+#define BOSSS_ERR_NONFINITE (-4)   /* NaN/Inf detected (CheckForNanOrInf)        */
+
+/* Equation components with hand-written kernels.  Anything else stays on the
+ * reference CPU evaluator (the C# side keeps base.GetEvaluatorEx as fallback). */
+enum bosss_form_kind {
+  BOSSS_FORM_SIP = 0,        /* SIPLaplace, constant Nu, Dirichlet g=0      (ipLaplace.cs:30-269)          */
+  BOSSS_FORM_ADVDIFF = 1,    /* upwind ScalarTransportFlux3D + SIPLaplace   (TransportFlux.cs:112-190)     */
+  BOSSS_FORM_EULER_LF = 2,   /* Euler components + RusanovFlux              (RusanovFlux.cs:55-65)         */
+  BOSSS_FORM_EULER_HLLC = 3  /* Euler components + HLLC{Density,Momentum,Energy}Flux (HLLCFlux.cs:48-86)   */
+};
+
+enum bosss_precision { BOSSS_FP64 = 0, BOSSS_FP32 = 1 };
+
+typedef struct bosss_ctx bosss_ctx;
+typedef struct bosss_grid bosss_grid;
+typedef struct bosss_op bosss_op;
+
+/* Affine cell and edge metrics as held by GridData (GridData_CellData.cs:293-387, 656-698;
+ * GridData_EdgeData.cs:213-222, 2321, 2524-2580). */
+typedef struct {
+  int32_t D, J, E;
+  const double* detJ;      /* [J]       JacobiDet                                         */
+  const double* invJ;      /* [J][D][D] InverseTransformation: invJ[j][e][d] = dxi_e/dx_d  */
+  const double* scaling;   /* [J]       ChefBasis.Scaling = 1/sqrt(detJ)                  */
+  const double* lenscale;  /* [J]       Cells.CellLengthScale                             */
+  const int32_t* cells;    /* [E][2]    CellIndices, OUT = -1 on boundary edges           */
+  const int32_t* trafo;    /* [E][2]    Edge2CellTrafoIndex (face of IN / OUT cell)       */
+  const double* normals;   /* [E][D]    NormalsForAffine (outward from IN)                */
+  const double* sqrtgram;  /* [E]       SqrtGramian                                       */
+} bosss_grid_desc;
+
+/* Reference-element tables (RefElement quadrature + Basis evaluation, BasisData.cs:44-69). */
+typedef struct {
+  int32_t D, degree, N, Kv, Ke, ntrafo;
+  const double* vol_weights;  /* [Kv]                                    */
+  const double* phi;          /* [Kv][N]     phi_n(xi_k)                 */
+  const double* dphi;         /* [Kv][N][D]  d phi_n / d xi_d            */
+  const double* face_weights; /* [Ke]                                    */
+  const double* trace;        /* [ntrafo][Ke][N]     EdgeEval            */
+  const double* dtrace;       /* [ntrafo][Ke][N][D]  EdgeGradientEval    */
+} bosss_tables_desc;
+
+/* Form coefficients (SipControl.penalty_poisson, Nu, transport velocity, IdealGas). */
+typedef struct {
+  double penalty_base; /* SIPLaplace m_penalty_base                    */
+  double nu;           /* SIPLaplace.Nu (signed)                       */
+  double velocity[3];  /* constant transport velocity                  */
+  double gamma;        /* HeatCapacityRatio                            */
+  double mach;         /* MachNumber                                   */
+} bosss_form_params;
+
+const char* bosss_hip_last_error(void);
+
+/* ---- context / grid / operator (replaces DifferentialOperator.GetEvaluatorEx setup,
+ *      DifferentialOperator.cs:1013-1033, and the grid metrics it reads) ---- */
+int bosss_hip_ctx_create(int device, bosss_ctx** ctx);
+int bosss_hip_ctx_destroy(bosss_ctx* ctx);
+int bosss_hip_sync(bosss_ctx* ctx);
+/* The HIP stream the context launches on (hipStream_t). */
+void* bosss_hip_ctx_stream(bosss_ctx* ctx);
+
+int bosss_hip_grid_upload(bosss_ctx* ctx, const bosss_grid_desc* desc, bosss_grid** grid);
+int bosss_hip_grid_destroy(bosss_grid* grid);
+
+int bosss_hip_op_create(bosss_ctx* ctx, const bosss_grid* grid, const bosss_tables_desc* tables, int kind,
+                        const bosss_form_params* params, int precision, bosss_op** op);
+int bosss_hip_op_destroy(bosss_op* op);
+/* ndof = J*V*N; nvars = V; nmodes = N. */
+int bosss_hip_op_info(const bosss_op* op, int64_t* ndof, int32_t* nvars, int32_t* nmodes);
+
+/* ---- residual (replaces EvaluatorNonLin.Evaluate, DifferentialOperator.cs:1584-1661;
+ *      IEvaluator.cs:81-97):  R <- beta*R + alpha*Op(U).  Host pointers, synchronous. ---- */
+int bosss_hip_evaluate(bosss_op* op, double time, double alpha, double beta, const double* U, double* R);
+
+/* ---- linearisation (replaces Newton.dirder FD-JVP, Newton.cs:1450-1500, and the
+ *      transpose of FDJacobianBuilder, DifferentialOperator.cs:1905-2880) ---- */
+int bosss_hip_jvp(bosss_op* op, const double* U, const double* dU, double* JdU);
+int bosss_hip_vjp(bosss_op* op, const double* U, const double* W, double* JtW);
+
+/* ---- device-resident variants: pointers are device memory, work is queued on
+ *      `stream` (NULL = the context stream) and returns without synchronising. ---- */
+int bosss_hip_evaluate_dev(bosss_op* op, double alpha, double beta, const double* U, double* R, void* stream);
+int bosss_hip_jvp_dev(bosss_op* op, const double* U, const double* dU, double* JdU, void* stream);
+int bosss_hip_vjp_dev(bosss_op* op, const double* U, const double* W, double* JtW, void* stream);
+/* fused: R <- beta*R + alpha*Op(U) and JtW <- J(U)^T W in one pass over U */
+int bosss_hip_evaluate_vjp_dev(bosss_op* op, double alpha, double beta, const double* U, double* R, const double* W,
+                               double* JtW, void* stream);
+
+/* ---- batched uneven cell-block factorisation (replaces BlockJacobi.InitImpl's
+ *      ReadBlock -> InvertInPlace -> LAPACK DGETRF/DGETRI per block, BlockJacobi.cs:63-120,
+ *      IMatrixExtensions.cs:2128-2230, and Schwarz's LAPACK block solver, Schwarz.cs:827-915).
+ *      Blocks are dense row-major at element offsets off[b], sizes n[b] <= nmax <= 128.
+ *      ipiv (1-based, LAPACK convention) at pivoff[b]; info[b] as DGETRF's INFO
+ *      (0, or j+1 for the first zero pivot; -1 if n[b] exceeds nmax).
+ *      want_inverse != 0 overwrites each block with its inverse (DGETRI), n <= 100.
+ *      All arrays are device memory; one launch covers the whole batch. ---- */
+int bosss_hip_block_lu_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t* n, const int64_t* off, double* A,
+                           int32_t* ipiv, const int64_t* pivoff, int32_t* info, int32_t want_inverse, void* stream);
+/* x_b <- (LU_b)^-1 x_b  (DGETRS), x_b at element offset xoff[b] */
+int bosss_hip_block_solve_dev(bosss_ctx* ctx, int nblk, const int32_t* n, const int64_t* off, const double* LU,
+                              const int32_t* ipiv, const int64_t* pivoff, const int64_t* xoff, double* x, void* stream);
+/* y_b <- alpha Ainv_b x_b + beta y_b  (block-diagonal apply of BlockJacobi.Solve, BlockJacobi.cs:148-197) */
+int bosss_hip_block_gemv_dev(bosss_ctx* ctx, int nblk, const int32_t* n, const int64_t* off, const double* Ainv,
+                             const int64_t* xoff, double alpha, const double* x, double beta, double* y, void* stream);
+const char* bosss_hip_block_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BOSSS_HIP_H */

@@ -780,6 +780,172 @@ void vjp(const Model& m, int kind, const Params& P, const double* U, const doubl
   }
 }
 
+// Structured VJP (the "port" of the adjoint): per quadrature node, the pointwise
+// flux Jacobian w.r.t. the node inputs (values, and gradients where the form uses
+// them) comes from forward-mode duals; the modal<->nodal contractions are applied
+// transposed.  Cost is a small multiple of one residual evaluation.
+template <int MV, int ME>
+void vjp_structured(const Model& m, int kind, const Params& P, const double* U, const double* W, double* out) {
+  const int D = m.D, V = nvars(kind, D), N = m.N, Kv = m.Kv, Ke = m.Ke;
+  const bool grad = needs_grad(kind);
+  const int ni = grad ? V * (1 + D) : V;        // inputs per side and node
+  const size_t Nt = (size_t)V * N;
+  for (size_t i = 0; i < (size_t)m.J * Nt; i++) out[i] = 0;
+  typedef Dual<MV> TV;
+  typedef Dual<ME> TE;
+  // ---------------- volume
+#pragma omp parallel
+  {
+    std::vector<double> u((size_t)Kv * V), gu((size_t)Kv * V * D), wg((size_t)Kv * V * D);
+    std::vector<TV> Ud(V), Gd((size_t)V * D), F((size_t)V * D), S(V);
+#pragma omp for schedule(dynamic, 4)
+    for (int j = 0; j < m.J; j++) {
+      const double* c = &U[(size_t)j * Nt];
+      const double* w = &W[(size_t)j * Nt];
+      const double* iJ = &m.invJ[(size_t)j * D * D];
+      double sc = m.scal[j], met = m.detJ[j] * sc;
+      eval_cell(m, V, grad, c, j, m.phi.data(), m.dphi.data(), Kv, u.data(), grad ? gu.data() : nullptr);
+      // reference gradients of W at the nodes (W is a cotangent: no scaling)
+      for (int k = 0; k < Kv; k++)
+        for (int f = 0; f < V; f++)
+          for (int e = 0; e < D; e++) {
+            double a = 0;
+            for (int n = 0; n < N; n++) a += w[f * N + n] * m.dphi[((size_t)k * N + n) * D + e];
+            wg[((size_t)k * V + f) * D + e] = a;
+          }
+      double* o = &out[(size_t)j * Nt];
+      for (int k = 0; k < Kv; k++) {
+        for (int f = 0; f < V; f++) {
+          Ud[f] = TV(u[k * V + f]); Ud[f].d[f] = 1.0;
+          for (int d = 0; d < D; d++) {
+            Gd[f * D + d] = TV(grad ? gu[((size_t)k * V + f) * D + d] : 0.0);
+            if (grad) Gd[f * D + d].d[V + f * D + d] = 1.0;
+          }
+        }
+        point_volume(m, kind, P, Ud.data(), Gd.data(), F.data(), S.data());
+        double ub[MV];
+        for (int q = 0; q < MV; q++) ub[q] = 0;
+        for (int f = 0; f < V; f++)
+          for (int d = 0; d < D; d++) {
+            double fb = 0;
+            for (int e = 0; e < D; e++) fb += wg[((size_t)k * V + f) * D + e] * iJ[e * D + d];
+            fb *= met * m.vq.w[k];
+            for (int q = 0; q < ni; q++) ub[q] += fb * F[f * D + d].d[q];
+          }
+        // pull back: dU(k)/dc_n = sc phi_n ; dGU_d(k)/dc_n = sc sum_e invJ[e][d] dphi_n,e
+        for (int f = 0; f < V; f++)
+          for (int n = 0; n < N; n++) {
+            double a = ub[f] * m.phi[(size_t)k * N + n];
+            if (grad)
+              for (int d = 0; d < D; d++) {
+                double gd = 0;
+                for (int e = 0; e < D; e++) gd += iJ[e * D + d] * m.dphi[((size_t)k * N + n) * D + e];
+                a += ub[V + f * D + d] * gd;
+              }
+            o[f * N + n] += sc * a;
+          }
+      }
+    }
+  }
+  // ---------------- edges
+  const int BLK = 8192;
+  std::vector<double> buf((size_t)BLK * 2 * Nt);
+  for (int e0 = 0; e0 < m.E; e0 += BLK) {
+    int e1 = std::min(m.E, e0 + BLK);
+#pragma omp parallel
+    {
+      std::vector<double> ui((size_t)Ke * V), uo((size_t)Ke * V), gi((size_t)Ke * V * D), go((size_t)Ke * V * D);
+      std::vector<TE> Ui(V), Uo(V), Gi((size_t)V * D), Go((size_t)V * D), fi(V), fo(V), gin((size_t)V * D), got((size_t)V * D);
+#pragma omp for schedule(dynamic, CHUNK)
+      for (int e = e0; e < e1; e++) {
+        double* bi = &buf[(size_t)(e - e0) * 2 * Nt];
+        double* bo = bi + Nt;
+        for (size_t i = 0; i < 2 * Nt; i++) bi[i] = 0;
+        const int ji = m.cin[e], jo = m.cout_[e];
+        const double* nrm = &m.normal[(size_t)e * D];
+        double mu = (kind == OP_SIP || kind == OP_ADVDIFF) ? sip_penalty(m, P, ji, jo) : 0.0;
+        size_t trs = (size_t)Ke * N;
+        const double* tri = &m.tr[m.fin[e] * trs];
+        const double* dtri = &m.dtr[m.fin[e] * trs * D];
+        const double* tro = jo >= 0 ? &m.tr[m.fout[e] * trs] : nullptr;
+        const double* dtro = jo >= 0 ? &m.dtr[m.fout[e] * trs * D] : nullptr;
+        const double* iJi = &m.invJ[(size_t)ji * D * D];
+        const double* iJo = jo >= 0 ? &m.invJ[(size_t)jo * D * D] : nullptr;
+        eval_cell(m, V, grad, &U[(size_t)ji * Nt], ji, tri, dtri, Ke, ui.data(), grad ? gi.data() : nullptr);
+        if (jo >= 0) eval_cell(m, V, grad, &U[(size_t)jo * Nt], jo, tro, dtro, Ke, uo.data(), grad ? go.data() : nullptr);
+        const double* Wi = &W[(size_t)ji * Nt];
+        const double* Wo = jo >= 0 ? &W[(size_t)jo * Nt] : nullptr;
+        double si = m.sqrtG[e] * m.scal[ji], so = jo >= 0 ? m.sqrtG[e] * m.scal[jo] : 0.0;
+        for (int k = 0; k < Ke; k++) {
+          const double wk = m.fq.w[k];
+          for (int f = 0; f < V; f++) {
+            Ui[f] = TE(ui[k * V + f]); Ui[f].d[f] = 1.0;
+            Uo[f] = TE(jo >= 0 ? uo[k * V + f] : 0.0); if (jo >= 0) Uo[f].d[ni + f] = 1.0;
+            for (int d = 0; d < D; d++) {
+              Gi[f * D + d] = TE(grad ? gi[((size_t)k * V + f) * D + d] : 0.0);
+              Go[f * D + d] = TE(grad && jo >= 0 ? go[((size_t)k * V + f) * D + d] : 0.0);
+              if (grad) { Gi[f * D + d].d[V + f * D + d] = 1.0; if (jo >= 0) Go[f * D + d].d[ni + V + f * D + d] = 1.0; }
+            }
+          }
+          if (jo >= 0) point_inner(m, kind, P, mu, nrm, Ui.data(), Uo.data(), Gi.data(), Go.data(), fi.data(), fo.data(), gin.data(), got.data());
+          else point_boundary(m, kind, P, mu, nrm, Ui.data(), Gi.data(), fi.data(), gin.data());
+          // cotangents of the node coefficients from W traced on each side
+          double ub[ME];
+          for (int q = 0; q < ME; q++) ub[q] = 0;
+          for (int f = 0; f < V; f++) {
+            double wv = 0, wvo = 0;
+            for (int n = 0; n < N; n++) { wv += Wi[f * N + n] * tri[(size_t)k * N + n]; if (jo >= 0) wvo += Wo[f * N + n] * tro[(size_t)k * N + n]; }
+            wv *= si * wk; wvo *= so * wk;
+            for (int q = 0; q < 2 * ni; q++) ub[q] += wv * fi[f].d[q] + (jo >= 0 ? wvo * fo[f].d[q] : 0.0);
+            if (grad)
+              for (int d = 0; d < D; d++) {
+                double wg_ = 0, wgo = 0;
+                for (int n = 0; n < N; n++)
+                  for (int ee = 0; ee < D; ee++) {
+                    wg_ += Wi[f * N + n] * dtri[((size_t)k * N + n) * D + ee] * iJi[ee * D + d];
+                    if (jo >= 0) wgo += Wo[f * N + n] * dtro[((size_t)k * N + n) * D + ee] * iJo[ee * D + d];
+                  }
+                wg_ *= si * wk; wgo *= so * wk;
+                for (int q = 0; q < 2 * ni; q++) ub[q] += wg_ * gin[f * D + d].d[q] + (jo >= 0 ? wgo * got[f * D + d].d[q] : 0.0);
+              }
+          }
+          // pull back to the IN and OUT coefficients
+          for (int side = 0; side < (jo >= 0 ? 2 : 1); side++) {
+            const double* tr_ = side ? tro : tri;
+            const double* dtr_ = side ? dtro : dtri;
+            const double* iJ = side ? iJo : iJi;
+            double sc = side ? m.scal[jo] : m.scal[ji];
+            double* bb = side ? bo : bi;
+            for (int f = 0; f < V; f++)
+              for (int n = 0; n < N; n++) {
+                double a = ub[side * ni + f] * tr_[(size_t)k * N + n];
+                if (grad)
+                  for (int d = 0; d < D; d++) {
+                    double gd = 0;
+                    for (int ee = 0; ee < D; ee++) gd += iJ[ee * D + d] * dtr_[((size_t)k * N + n) * D + ee];
+                    a += ub[side * ni + V + f * D + d] * gd;
+                  }
+                bb[f * N + n] += sc * a;
+              }
+          }
+        }
+      }
+    }
+    for (int e = e0; e < e1; e++) {
+      const double* bi = &buf[(size_t)(e - e0) * 2 * Nt];
+      int ji = m.cin[e], jo = m.cout_[e];
+      for (size_t q = 0; q < Nt; q++) out[(size_t)ji * Nt + q] += bi[q];
+      if (jo >= 0) for (size_t q = 0; q < Nt; q++) out[(size_t)jo * Nt + q] += bi[Nt + q];
+    }
+  }
+}
+
+void vjp_fast(const Model& m, int kind, const Params& P, const double* U, const double* W, double* out) {
+  bool euler = (kind == OP_EULER_LF || kind == OP_EULER_HLLC);
+  if (m.D == 3) { if (euler) vjp_structured<5, 10>(m, kind, P, U, W, out); else vjp_structured<4, 8>(m, kind, P, U, W, out); }
+  else { if (euler) vjp_structured<4, 8>(m, kind, P, U, W, out); else vjp_structured<3, 6>(m, kind, P, U, W, out); }
+}
+
 // ------------------------------------------------------------- block LU
 // Unblocked right-looking LU with partial pivoting (LAPACK dgetf2 semantics):
 // pivot = first index of max |a| in the column; returns 0 or the 1-based index of
@@ -910,6 +1076,12 @@ int bref_jvp(void* h, int kind, const double* prm, const double* U, const double
 }
 
 int bref_vjp(void* h, int kind, const double* prm, const double* U, const double* W, double* out, int nthreads) {
+  set_threads(nthreads);
+  vjp_fast(*(Model*)h, kind, make_params(prm), U, W, out);
+  return 0;
+}
+// VJP through the exact local Jacobian blocks (slow; cross-check of bref_vjp)
+int bref_vjp_dense(void* h, int kind, const double* prm, const double* U, const double* W, double* out, int nthreads) {
   set_threads(nthreads);
   vjp(*(Model*)h, kind, make_params(prm), U, W, out);
   return 0;

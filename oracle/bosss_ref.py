@@ -43,6 +43,7 @@ def _load():
         lib.bref_evaluate.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.c_int]
         lib.bref_jvp.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int]
         lib.bref_vjp.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int]
+        lib.bref_vjp_dense.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int]
         lib.bref_assemble.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int]
         lib.bref_block_lu.argtypes = [ctypes.c_int, _ip, _lp, _dp, _ip, _lp, _ip]
         lib.bref_block_solve.argtypes = [ctypes.c_int, _ip, _lp, _dp, _ip, _lp, _lp, _dp]
@@ -128,6 +129,13 @@ class RefModel:
         W = np.ascontiguousarray(W, dtype=np.float64)
         out = np.zeros_like(U)
         self.lib.bref_vjp(self.h, kind, _p(prm), _p(U), _p(W), _p(out), nthreads)
+        return out
+
+    def vjp_dense(self, kind, prm, U, W, nthreads=0):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        out = np.zeros_like(U)
+        self.lib.bref_vjp_dense(self.h, kind, _p(prm), _p(U), _p(W), _p(out), nthreads)
         return out
 
     def assemble(self, kind, prm, U=None, nthreads=0):

@@ -1,0 +1,55 @@
+"""CPU-side checks of the C ABI and of the host-side setup (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from bosss_amd import _lib
+from bosss_amd.grid import CartesianGrid
+from bosss_amd.refelement import Tables
+from oracle.bosss_ref import RefModel
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "bosss_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(bosss_hip_\w+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(_lib.LIBPATH)
+    syms = declared_symbols()
+    assert len(syms) >= 17
+    for s in syms:
+        assert hasattr(L, s), s
+    # and the Python binding covers exactly the declared set
+    assert sorted(_lib.SIGNATURES) == syms
+
+
+def test_error_path_without_gpu_is_reported():
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    rc = L.bosss_hip_ctx_create(-1, ctypes.byref(h))
+    assert rc < 0
+    assert L.bosss_hip_last_error().decode()
+
+
+@pytest.mark.parametrize("D,n,p,order,per", [(2, (4, 4), 2, 4, False), (3, (3, 3, 3), 3, 9, True), (3, (3, 4, 2), 2, 4, False),
+                                             (3, (2, 2, 2), 3, 6, True), (2, (3, 5), 2, 6, True)])
+def test_host_setup_matches_oracle(D, n, p, order, per):
+    g = CartesianGrid(n, periodic=[per] * D)
+    t = Tables(D, p, order)
+    m = RefModel(D, n, p, order, periodic=[int(per)] * D)
+    rg, rt = m.grid(), m.tables()
+    assert (g.cells == rg["cells"]).all() and (g.trafo == rg["faces"]).all()
+    for a, b in ((g.lenscale, rg["lenscale"]), (g.normals, rg["normals"]), (g.sqrtgram, rg["sqrtgram"]), (g.detJ, rg["detJ"]),
+                 (g.invJ, rg["invJ"]), (g.scaling, rg["scaling"])):
+        assert np.abs(a - b).max() <= 1e-15 * max(1.0, np.abs(b).max())
+    # closed-form basis/Gauss tables vs the reference's printed coefficient tables
+    for a, b in ((t.phi, rt["phi"]), (t.dphi, rt["dphi"]), (t.trace, rt["trace"]), (t.dtrace, rt["dtrace"]),
+                 (t.vw, rt["vw"]), (t.fw, rt["fw"]), (t.vnodes, rt["vnodes"])):
+        assert np.abs(a - b).max() < 5e-14 * max(1.0, np.abs(b).max())
