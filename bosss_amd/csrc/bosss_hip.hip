@@ -290,8 +290,8 @@ static KArgs<Real> make_args(const bosss_op* op, double alpha, double beta, cons
   return a;
 }
 
-static int launch(bosss_op* op, int which, double alpha, double beta, const void* U, const void* X, void* R, void* out, void* stream) {
-  hipStream_t s = stream ? (hipStream_t)stream : op->ctx->stream;
+// `stream` is a hipStream_t; NULL is HIP's null (default) stream, as everywhere in HIP.
+static int launch(bosss_op* op, int which, double alpha, double beta, const void* U, const void* X, void* R, void* out, hipStream_t s) {
   const int cells_per_block = 16 * (KernelSet::BLOCK / 64);
   dim3 grid((op->grid->J + cells_per_block - 1) / cells_per_block), block(KernelSet::BLOCK);
   HIPCHK(hipSetDevice(op->ctx->device));
@@ -314,20 +314,20 @@ extern "C" {
 
 int bosss_hip_evaluate_dev(bosss_op* op, double alpha, double beta, const double* U, double* R, void* stream) {
   if (!op || !U || !R) return fail(BOSSS_ERR_ARG, "NULL argument");
-  return launch(op, KernelSet::RES, alpha, beta, U, nullptr, R, nullptr, stream);
+  return launch(op, KernelSet::RES, alpha, beta, U, nullptr, R, nullptr, (hipStream_t)stream);
 }
 int bosss_hip_jvp_dev(bosss_op* op, const double* U, const double* dU, double* JdU, void* stream) {
   if (!op || !U || !dU || !JdU) return fail(BOSSS_ERR_ARG, "NULL argument");
-  return launch(op, KernelSet::JVP, 1.0, 0.0, U, dU, nullptr, JdU, stream);
+  return launch(op, KernelSet::JVP, 1.0, 0.0, U, dU, nullptr, JdU, (hipStream_t)stream);
 }
 int bosss_hip_vjp_dev(bosss_op* op, const double* U, const double* W, double* JtW, void* stream) {
   if (!op || !U || !W || !JtW) return fail(BOSSS_ERR_ARG, "NULL argument");
-  return launch(op, KernelSet::VJP, 1.0, 0.0, U, W, nullptr, JtW, stream);
+  return launch(op, KernelSet::VJP, 1.0, 0.0, U, W, nullptr, JtW, (hipStream_t)stream);
 }
 int bosss_hip_evaluate_vjp_dev(bosss_op* op, double alpha, double beta, const double* U, double* R, const double* W,
                                double* JtW, void* stream) {
   if (!op || !U || !R || !W || !JtW) return fail(BOSSS_ERR_ARG, "NULL argument");
-  return launch(op, KernelSet::RES_VJP, alpha, beta, U, W, R, JtW, stream);
+  return launch(op, KernelSet::RES_VJP, alpha, beta, U, W, R, JtW, (hipStream_t)stream);
 }
 
 static int ensure_staging(bosss_op* op) {
@@ -374,7 +374,7 @@ int bosss_hip_evaluate(bosss_op* op, double time, double alpha, double beta, con
   HIPCHK(hipSetDevice(op->ctx->device));
   if ((rc = ensure_staging(op)) || (rc = to_device(op, op->hU, U))) return rc;
   if (beta != 0.0 && (rc = to_device(op, op->hR, R))) return rc;
-  if ((rc = launch(op, KernelSet::RES, alpha, beta, op->hU, nullptr, op->hR, nullptr, nullptr))) return rc;
+  if ((rc = launch(op, KernelSet::RES, alpha, beta, op->hU, nullptr, op->hR, nullptr, op->ctx->stream))) return rc;
   return to_host(op, R, op->hR);
 }
 
@@ -383,7 +383,7 @@ int bosss_hip_jvp(bosss_op* op, const double* U, const double* dU, double* JdU) 
   int rc;
   HIPCHK(hipSetDevice(op->ctx->device));
   if ((rc = ensure_staging(op)) || (rc = to_device(op, op->hU, U)) || (rc = to_device(op, op->hX, dU))) return rc;
-  if ((rc = launch(op, KernelSet::JVP, 1.0, 0.0, op->hU, op->hX, nullptr, op->hY, nullptr))) return rc;
+  if ((rc = launch(op, KernelSet::JVP, 1.0, 0.0, op->hU, op->hX, nullptr, op->hY, op->ctx->stream))) return rc;
   return to_host(op, JdU, op->hY);
 }
 
@@ -392,7 +392,7 @@ int bosss_hip_vjp(bosss_op* op, const double* U, const double* W, double* JtW) {
   int rc;
   HIPCHK(hipSetDevice(op->ctx->device));
   if ((rc = ensure_staging(op)) || (rc = to_device(op, op->hU, U)) || (rc = to_device(op, op->hX, W))) return rc;
-  if ((rc = launch(op, KernelSet::VJP, 1.0, 0.0, op->hU, op->hX, nullptr, op->hY, nullptr))) return rc;
+  if ((rc = launch(op, KernelSet::VJP, 1.0, 0.0, op->hU, op->hX, nullptr, op->hY, op->ctx->stream))) return rc;
   return to_host(op, JtW, op->hY);
 }
 

@@ -151,6 +151,7 @@ class DifferentialOperator:
                 raise ValueError("expected a contiguous %s CUDA tensor with %d entries" % (want, self.ndof))
 
     def _stream(self, stream):
+        """HIP stream handle: torch's current stream by default (0 is HIP's null stream)."""
         if stream is None:
             import torch
             return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)

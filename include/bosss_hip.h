@@ -109,7 +109,8 @@ int bosss_hip_jvp(bosss_op* op, const double* U, const double* dU, double* JdU);
 int bosss_hip_vjp(bosss_op* op, const double* U, const double* W, double* JtW);
 
 /* ---- device-resident variants: pointers are device memory, work is queued on
- *      `stream` (NULL = the context stream) and returns without synchronising. ---- */
+ *      `stream` (a hipStream_t; NULL is HIP's null stream, bosss_hip_ctx_stream()
+ *      gives the context's own stream) and returns without synchronising. ---- */
 int bosss_hip_evaluate_dev(bosss_op* op, double alpha, double beta, const double* U, double* R, void* stream);
 int bosss_hip_jvp_dev(bosss_op* op, const double* U, const double* dU, double* JdU, void* stream);
 int bosss_hip_vjp_dev(bosss_op* op, const double* U, const double* W, double* JtW, void* stream);

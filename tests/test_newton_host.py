@@ -1,0 +1,53 @@
+"""Host logic of the Newton-Krylov driver and GMRES, on the CPU.
+
+The operator is a stand-in that evaluates the residual and its JVP/VJP with the
+CPU oracle on CPU tensors, so the driver (GMRES, defect correction, adjoint solve)
+is checked without a GPU.  The GPU runs of the same driver are in test_gpu_newton.py.
+"""
+import numpy as np
+import torch
+
+from bosss_amd.grid import CartesianGrid
+from bosss_amd.newton import ImplicitStep, gmres
+from bosss_amd.workloads import euler_state
+from oracle.bosss_ref import RefModel, params_vector, OP_EULER_HLLC
+
+
+class OracleOp:
+    def __init__(self, n, p):
+        self.ref = RefModel(3, (n,) * 3, p, 3 * p, periodic=[1, 1, 1])
+        self.prm = params_vector(gamma=1.4)
+
+    def _run(self, fn, *args, out):
+        res = fn(OP_EULER_HLLC, self.prm, *[a.double().numpy() for a in args])
+        out.copy_(torch.from_numpy(res).to(out.dtype))
+
+    def evaluate_dev(self, U, R):
+        self._run(self.ref.evaluate, U, out=R)
+
+    def jvp_dev(self, U, dU, out):
+        self._run(self.ref.jvp, U, dU, out=out)
+
+    def vjp_dev(self, U, W, out):
+        self._run(self.ref.vjp, U, W, out=out)
+
+
+def test_gmres_solves_small_nonsymmetric_system():
+    rng = np.random.default_rng(0)
+    A = torch.from_numpy(np.eye(40) * 4 + rng.uniform(-1, 1, (40, 40)))
+    b = torch.from_numpy(rng.uniform(-1, 1, 40))
+    x, its, rel = gmres(lambda v, out: out.copy_(A @ v), b, 1e-12, restart=15, maxit=200)
+    assert rel <= 1e-12 and torch.linalg.vector_norm(A @ x - b) <= 1e-11 * torch.linalg.vector_norm(b)
+
+
+def test_newton_fp64_and_defect_correction_converge():
+    n, p = 4, 2
+    g = CartesianGrid((n,) * 3, periodic=[True] * 3)
+    Un = torch.from_numpy(euler_state(g, 10, seed=7))
+    dt = 0.5 / n / (2 * p + 1)
+    st = ImplicitStep(OracleOp(n, p), Un, dt, op32=OracleOp(n, p))
+    U64 = st.solve_fp64(Un, tol=1e-10)
+    assert st.stats["history"][-1] <= 1e-10
+    Um = st.solve_mixed(Un, tol=1e-10)
+    assert st.stats["history"][-1] <= 1e-10
+    assert torch.linalg.vector_norm(U64 - Um) <= 1e-9
