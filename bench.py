@@ -65,6 +65,29 @@ def cpu_baseline(form, p, budget_s=15.0, n_max=64):
     }
 
 
+def newton_comparison(grid, op64, Un, form, p):
+    """One implicit Euler step solved to ||F|| <= 1e-10: FP64 Newton-GMRES vs FP64 defect
+    correction around FP32 JVPs (same driver, bosss_amd.newton)."""
+    import torch
+    from bosss_amd.newton import ImplicitStep
+    from bosss_amd.operator import DifferentialOperator
+    op32 = DifferentialOperator(grid, form, p, gamma=1.4, precision="fp32")
+    dt = 0.5 / grid.ncells[0] / (2 * p + 1)
+    st = ImplicitStep(op64, Un, dt, op32=op32)
+    st.solve_fp64(Un, tol=1e-10)        # warm-up (allocations, first launches)
+    res = {}
+    for name, fn in (("fp64", st.solve_fp64), ("mixed", st.solve_mixed)):
+        torch.cuda.synchronize()
+        U = fn(Un, tol=1e-10)
+        s = dict(st.stats)
+        res[name] = {"seconds": round(s["seconds"], 4), "newton_steps": s["newton"], "jvps": s["jvps"],
+                     "final_residual": s["history"][-1]}
+        del U
+    res["dt"] = dt
+    res["speedup_mixed_vs_fp64"] = round(res["fp64"]["seconds"] / res["mixed"]["seconds"], 3)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +97,8 @@ def main():
     ap.add_argument("--p", type=int, default=3)
     ap.add_argument("--form", default="euler_hllc", choices=["euler_hllc", "euler_lf", "advdiff", "sip"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--split", action="store_true", help="time separate residual and VJP launches instead of the fused one")
+    ap.add_argument("--no-newton", action="store_true", help="skip the implicit-step Newton comparison (FP64 vs mixed)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU sharding is a design, not a code path); "
@@ -99,8 +124,15 @@ def main():
     Gt = torch.zeros_like(Ut)
     stream = torch.cuda.current_stream()
 
+    def step():
+        if args.split:
+            op.evaluate_dev(Ut, Rt)
+            op.vjp_dev(Ut, Wt, Gt)
+        else:
+            op.evaluate_vjp_dev(Ut, Rt, Wt, Gt)
+
     for _ in range(args.warmup):
-        op.evaluate_vjp_dev(Ut, Rt, Wt, Gt)
+        step()
     torch.cuda.synchronize()
 
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
@@ -109,7 +141,7 @@ def main():
     t0 = time.perf_counter()
     for i in range(args.steps):
         starts[i].record(stream)
-        op.evaluate_vjp_dev(Ut, Rt, Wt, Gt)
+        step()
         ends[i].record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -133,13 +165,18 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "3D %s, %d^3 periodic hex mesh of the unit cube, p=%d (%d modes x %d vars, %d DOF), "
-                               "fused residual + VJP per step" % (form, n, n, p, op.N, op.nvars, op.ndof),
+                               "fused residual + VJP per step" % (form, n, p, op.N, op.nvars, op.ndof),
                    "mesh": [n, n, n], "degree": p, "form": form, "dof": op.ndof},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                     "kernel": "k_adjoint<FormEuler<3,HLLC>,3,20,5,double,fused>" if form == "euler_hllc" else form,
+                     "kernel": "kt_adjoint<FormEuler<3,HLLC>,D=3,P=3,K1=5,double,fused>" if form == "euler_hllc" else form,
                      "kernel_ms": round(kernel_ms, 4), "alg_flops_per_launch": flops},
     }
+    print("bench: %s kernels, %s, %.3f ms/step, %.3f ms kernel" % (
+        "generic-table" if os.environ.get("BOSSS_HIP_GENERIC_TABLES") == "1" else "tensor-table",
+        "split" if args.split else "fused", ms_per_step, kernel_ms), file=sys.stderr, flush=True)
+    if form.startswith("euler") and not args.no_newton:
+        out["newton_implicit_step"] = newton_comparison(grid, op, Ut, form, p)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(form, p)
     print(json.dumps(out), flush=True)

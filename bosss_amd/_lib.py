@@ -55,6 +55,10 @@ SIGNATURES = {
     "bosss_hip_jvp_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "bosss_hip_vjp_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "bosss_hip_evaluate_vjp_dev": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp, _vp, _vp, _vp, _vp]),
+    "bosss_hip_op_matrix_layout": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
+                                                  ctypes.POINTER(ctypes.c_int32), _vp]),
+    "bosss_hip_op_assemble_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.c_double, _vp, _vp]),
+    "bosss_hip_bell_spmv_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp]),
     "bosss_hip_block_lu_dev": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32, _vp]),
     "bosss_hip_block_solve_dev": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "bosss_hip_block_gemv_dev": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp]),

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -54,6 +55,7 @@ struct bosss_grid {
   bosss_ctx* ctx;
   int D, J, E, NF;
   bool has_boundary;
+  std::vector<int> nb_host;  // [J][NF], for host-side colouring
   double *detJ, *invJ, *scaling, *lenscale;
   int* nb;          // [J][NF]
   double* sqrtg;    // [J][NF]
@@ -67,11 +69,25 @@ struct bosss_op {
   int kind, D, N, K1, V, precision;
   const KernelSet* ks;
   void* tables;      // packed tables (Real) on the device
+  void* ttab;        // 1D tensor tables (Real) on the device, if the uploaded tables are a tensor product
+  bool tensor;
   FormParams fp;
   size_t ndof;
   // staging buffers for the host-pointer entry points
   double *hU, *hR, *hX, *hY;
+  // matrix assembly: distance-2 colouring and block-ELL columns (built on first use)
+  int ncolors;
+  int32_t *d_color, *d_cols;
+  std::vector<int32_t> cols_host;
 };
+
+namespace bosss {
+void launch_probe(int J, int Nt, const int32_t* color, int k, int q, double* X, hipStream_t s);
+void launch_extract(int J, int Nt, int S, const int32_t* cols, const int32_t* color, int k, int q, const double* Y, double alpha,
+                    double beta, double* blocks, hipStream_t s);
+void launch_bell_spmv(int J, int Nt, int S, const int32_t* cols, const double* blocks, double alpha, const double* x, double beta,
+                      double* y, hipStream_t s);
+}
 
 extern "C" {
 
@@ -155,6 +171,7 @@ int bosss_hip_grid_upload(bosss_ctx* ctx, const bosss_grid_desc* g, bosss_grid**
   HIPCHK(hipSetDevice(ctx->device));
   bosss_grid* G = new bosss_grid();
   G->ctx = ctx; G->D = D; G->J = J; G->E = E; G->NF = NF; G->has_boundary = has_boundary;
+  G->nb_host = nb;
   for (int f = 0; f < 6; f++) G->pair[f] = f < NF ? pair[f] : 0;
   int rc;
   if ((rc = upload(&G->detJ, g->detJ, J)) || (rc = upload(&G->invJ, g->invJ, (size_t)J * D * D)) ||
@@ -200,6 +217,109 @@ static void pack_tables(const bosss_tables_desc* t, const KernelSet* ks, std::ve
             out[ks->o_dtr + ((f * D + d) * KEP + k) * N + n] = Real(t->dtrace[(((size_t)f * Ke + k) * N + n) * D + d]);
       }
   for (int k = 0; k < Ke; k++) out[ks->o_fw + k] = Real(t->face_weights[k]);
+}
+
+// ---- 1D factors of the hypercube tables (Gauss-Legendre rule, normalised Legendre polynomials)
+static void gauss_legendre(int K, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(K, 0.0); w.assign(K, 0.0);
+  for (int i = 0; i < K; i++) {
+    double z = std::cos(M_PI * (i + 0.75) / (K + 0.5)), pp = 0;
+    for (int it = 0; it < 100; it++) {
+      double p0 = 1, p1 = 0;
+      for (int n = 1; n <= K; n++) { double p2 = p1; p1 = p0; p0 = ((2 * n - 1) * z * p1 - (n - 1) * p2) / n; }
+      pp = K * (z * p0 - p1) / (z * z - 1);
+      double dz = p0 / pp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    x[K - 1 - i] = z;
+    w[K - 1 - i] = 2.0 / ((1 - z * z) * pp * pp);
+  }
+}
+static void legendre_normalised(int P, double x, double* L, double* DL) {
+  double p0 = 1, p1 = x, d0 = 0, d1 = 1;
+  for (int n = 0; n <= P; n++) {
+    double pn, dn;
+    if (n == 0) { pn = 1; dn = 0; }
+    else if (n == 1) { pn = x; dn = 1; }
+    else {
+      pn = ((2 * n - 1) * x * p1 - (n - 1) * p0) / n;
+      dn = d0 + (2 * n - 1) * p1;
+      p0 = p1; p1 = pn; d0 = d1; d1 = dn;
+    }
+    double c = std::sqrt((2 * n + 1) / 2.0);
+    L[n] = c * pn;
+    DL[n] = c * dn;
+  }
+}
+
+// Build the 1D table block of TShape and check that the uploaded tables are its tensor product.
+static bool build_tensor_tables(const bosss_tables_desc* t, const KernelSet* ks, int K1, std::vector<double>& tab, double* maxdiff) {
+  const int D = t->D, P = ks->P, K2 = ks->t_k2, N = t->N;
+  tab.assign(ks->ttab_size, 0.0);
+  std::vector<double> x, w;
+  gauss_legendre(K1, x, w);
+  std::vector<double> L(P + 1), DL(P + 1);
+  for (int q = 0; q < K2; q++) {
+    double xx = q < K1 ? x[q] : (q == K1 ? -1.0 : 1.0);
+    legendre_normalised(P, xx, L.data(), DL.data());
+    for (int a = 0; a <= P; a++) { tab[ks->t_o_l + a * K2 + q] = L[a]; tab[ks->t_o_dl + a * K2 + q] = DL[a]; }
+  }
+  for (int q = 0; q < K1; q++) tab[ks->t_o_w + q] = w[q];
+  std::vector<int> ex;   // modes sorted by degree, then lexicographic in (e_x, e_y, e_z)
+  for (int deg = 0; deg <= P; deg++)
+    for (int a = 0; a <= deg; a++)
+      for (int b = 0; b <= deg - a; b++) {
+        int c = deg - a - b;
+        if (D == 2 && c != 0) continue;
+        if (D == 2) { ex.push_back(a | (b << 4)); }
+        else ex.push_back(a | (b << 4) | (c << 8));
+      }
+  // for D == 2 the loop above emits (a, deg-a) once per a: same lexicographic order
+  if ((int)ex.size() != N) return false;
+  for (int n = 0; n < N; n++) tab[ks->t_o_code + n] = ex[n];
+  auto val = [&](const int* q, int code, int deriv) {
+    int e[3] = {code & 15, (code >> 4) & 15, (code >> 8) & 15};
+    double r = 1;
+    for (int d = 0; d < D; d++) {
+      const double* T = &tab[(deriv == d ? ks->t_o_dl : ks->t_o_l)];
+      r *= T[e[d] * K2 + q[d]];
+    }
+    return r;
+  };
+  double md = 0, scale = 1;
+  auto cmp = [&](double a, double b) { md = std::max(md, std::fabs(a - b)); scale = std::max(scale, std::fabs(b)); };
+  for (int k = 0; k < t->Kv; k++) {
+    int q[3];
+    if (D == 3) { q[0] = k % K1; q[1] = (k / K1) % K1; q[2] = k / (K1 * K1); } else { q[0] = k % K1; q[1] = k / K1; q[2] = 0; }
+    double wk = D == 3 ? (w[q[2]] * w[q[1]]) * w[q[0]] : w[q[1]] * w[q[0]];
+    cmp(wk, t->vol_weights[k]);
+    for (int n = 0; n < N; n++) {
+      cmp(val(q, ex[n], -1), t->phi[k * N + n]);
+      for (int d = 0; d < D; d++) cmp(val(q, ex[n], d), t->dphi[(k * N + n) * D + d]);
+    }
+  }
+  for (int f = 0; f < 2 * D; f++)
+    for (int k = 0; k < t->Ke; k++) {
+      int q[3];
+      const int lo = K1, hi = K1 + 1;
+      if (D == 3) {
+        int s0 = k % K1, s1 = k / K1;
+        int m[6][3] = {{lo, s1, s0}, {hi, s1, s0}, {s0, hi, s1}, {s0, lo, s1}, {s0, s1, hi}, {s0, s1, lo}};
+        for (int d = 0; d < 3; d++) q[d] = m[f][d];
+        if (f == 0) cmp(w[k / K1] * w[k % K1], t->face_weights[k]);
+      } else {
+        int m[4][2] = {{lo, k}, {hi, k}, {k, hi}, {k, lo}};
+        q[0] = m[f][0]; q[1] = m[f][1]; q[2] = 0;
+        if (f == 0) cmp(w[k], t->face_weights[k]);
+      }
+      for (int n = 0; n < N; n++) {
+        cmp(val(q, ex[n], -1), t->trace[((size_t)f * t->Ke + k) * N + n]);
+        for (int d = 0; d < D; d++) cmp(val(q, ex[n], d), t->dtrace[(((size_t)f * t->Ke + k) * N + n) * D + d]);
+      }
+    }
+  *maxdiff = md / scale;
+  return md <= 1e-12 * scale;
 }
 
 extern "C" {
@@ -248,6 +368,22 @@ int bosss_hip_op_create(bosss_ctx* ctx, const bosss_grid* grid, const bosss_tabl
     rc = upload((float**)&op->tables, packed.data(), packed.size());
   }
   if (rc) { delete op; return rc; }
+  {
+    std::vector<double> tab;
+    double md = 0;
+    const char* force_generic = std::getenv("BOSSS_HIP_GENERIC_TABLES");
+    op->tensor = ks->kt64[KernelSet::RES] && !(force_generic && force_generic[0] == '1') && build_tensor_tables(t, ks, K1, tab, &md);
+    op->ttab = nullptr;
+    if (op->tensor) {
+      if (precision == BOSSS_FP64) {
+        rc = upload((double**)&op->ttab, tab.data(), tab.size());
+      } else {
+        std::vector<float> tf(tab.begin(), tab.end());
+        rc = upload((float**)&op->ttab, tf.data(), tf.size());
+      }
+      if (rc) { (void)hipFree(op->tables); delete op; return rc; }
+    }
+  }
   if ((rc = ks->prepare()) != 0) { (void)hipFree(op->tables); delete op; return fail(BOSSS_ERR_HIP, "kernel attribute setup failed (%d)", rc); }
   *out = op;
   return BOSSS_OK;
@@ -256,10 +392,13 @@ int bosss_hip_op_create(bosss_ctx* ctx, const bosss_grid* grid, const bosss_tabl
 int bosss_hip_op_destroy(bosss_op* op) {
   if (!op) return BOSSS_OK;
   (void)hipFree(op->tables);
+  if (op->ttab) (void)hipFree(op->ttab);
   if (op->hU) (void)hipFree(op->hU);
   if (op->hR) (void)hipFree(op->hR);
   if (op->hX) (void)hipFree(op->hX);
   if (op->hY) (void)hipFree(op->hY);
+  if (op->d_color) (void)hipFree(op->d_color);
+  if (op->d_cols) (void)hipFree(op->d_cols);
   delete op;
   return BOSSS_OK;
 }
@@ -291,10 +430,35 @@ static KArgs<Real> make_args(const bosss_op* op, double alpha, double beta, cons
 }
 
 // `stream` is a hipStream_t; NULL is HIP's null (default) stream, as everywhere in HIP.
+template <typename Real>
+static TArgs<Real> make_targs(const bosss_op* op, double alpha, double beta, const void* U, const void* X, void* R, void* out) {
+  KArgs<Real> k = make_args<Real>(op, alpha, beta, U, X, R, out);
+  TArgs<Real> a;
+  a.J = k.J; a.detJ = k.detJ; a.invJ = k.invJ; a.scaling = k.scaling; a.lenscale = k.lenscale; a.cf = k.cf;
+  a.tab = (const Real*)op->ttab;
+  a.fp = k.fp; a.alpha = alpha; a.beta = beta;
+  a.U = (const Real*)U; a.X = (const Real*)X; a.R = (Real*)R; a.out = (Real*)out;
+  return a;
+}
+
 static int launch(bosss_op* op, int which, double alpha, double beta, const void* U, const void* X, void* R, void* out, hipStream_t s) {
+  HIPCHK(hipSetDevice(op->ctx->device));
+  if (op->tensor) {
+    dim3 grid((op->grid->J + 16 * TWAVES - 1) / (16 * TWAVES)), block(64 * TWAVES);
+    if (op->precision == BOSSS_FP64) {
+      void (*k)(TArgs<double>) = op->ks->kt64[which];
+      if (!k) return fail(BOSSS_ERR_UNSUPPORTED, "tensor kernel variant %d not built", which);
+      hipLaunchKernelGGL(k, grid, block, op->ks->ktlds64[which], s, make_targs<double>(op, alpha, beta, U, X, R, out));
+    } else {
+      void (*k)(TArgs<float>) = op->ks->kt32[which];
+      if (!k) return fail(BOSSS_ERR_UNSUPPORTED, "tensor kernel variant %d not built", which);
+      hipLaunchKernelGGL(k, grid, block, op->ks->ktlds32[which], s, make_targs<float>(op, alpha, beta, U, X, R, out));
+    }
+    HIPCHK(hipGetLastError());
+    return BOSSS_OK;
+  }
   const int cells_per_block = 16 * (KernelSet::BLOCK / 64);
   dim3 grid((op->grid->J + cells_per_block - 1) / cells_per_block), block(KernelSet::BLOCK);
-  HIPCHK(hipSetDevice(op->ctx->device));
   if (op->precision == BOSSS_FP64) {
     KArgs<double> a = make_args<double>(op, alpha, beta, U, X, R, out);
     void (*k)(KArgs<double>) = op->ks->k64[which];
@@ -394,6 +558,92 @@ int bosss_hip_vjp(bosss_op* op, const double* U, const double* W, double* JtW) {
   if ((rc = ensure_staging(op)) || (rc = to_device(op, op->hU, U)) || (rc = to_device(op, op->hX, W))) return rc;
   if ((rc = launch(op, KernelSet::VJP, 1.0, 0.0, op->hU, op->hX, nullptr, op->hY, op->ctx->stream))) return rc;
   return to_host(op, JtW, op->hY);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ matrix assembly
+static int ensure_matrix_setup(bosss_op* op) {
+  if (op->d_cols) return BOSSS_OK;
+  const bosss_grid* g = op->grid;
+  const int J = g->J, NF = g->NF, S = 1 + NF;
+  // block-ELL columns; a neighbour met twice in one row (periodic direction of 2 cells) keeps one slot
+  op->cols_host.assign((size_t)J * S, -1);
+  for (int j = 0; j < J; j++) {
+    op->cols_host[(size_t)j * S] = j;
+    for (int f = 0; f < NF; f++) {
+      int c = g->nb_host[(size_t)j * NF + f];
+      bool dup = (c == j);
+      for (int s2 = 1; s2 <= f && !dup; s2++) dup = (op->cols_host[(size_t)j * S + s2] == c);
+      op->cols_host[(size_t)j * S + 1 + f] = dup ? -1 : c;
+    }
+  }
+  // greedy distance-2 colouring of the cell graph
+  std::vector<int32_t> color(J, -1);
+  std::vector<int> seen;
+  int ncol = 0;
+  for (int j = 0; j < J; j++) {
+    seen.assign(ncol + 1, 0);
+    auto mark = [&](int c) { if (c >= 0 && color[c] >= 0) seen[color[c]] = 1; };
+    for (int f = 0; f < NF; f++) {
+      int c = g->nb_host[(size_t)j * NF + f];
+      mark(c);
+      if (c >= 0)
+        for (int f2 = 0; f2 < NF; f2++) mark(g->nb_host[(size_t)c * NF + f2]);
+    }
+    int k = 0;
+    while (k < ncol && seen[k]) k++;
+    color[j] = k;
+    if (k == ncol) ncol++;
+  }
+  op->ncolors = ncol;
+  int rc;
+  if ((rc = upload(&op->d_color, color.data(), color.size())) || (rc = upload(&op->d_cols, op->cols_host.data(), op->cols_host.size())))
+    return rc;
+  return BOSSS_OK;
+}
+
+extern "C" {
+
+int bosss_hip_op_matrix_layout(bosss_op* op, int32_t* blocks_per_row, int64_t* entries, int32_t* ncolors, int32_t* cols) {
+  if (!op) return fail(BOSSS_ERR_ARG, "op is NULL");
+  int rc;
+  HIPCHK(hipSetDevice(op->ctx->device));
+  if ((rc = ensure_matrix_setup(op))) return rc;
+  const int S = 1 + op->grid->NF, Nt = op->V * op->N;
+  if (blocks_per_row) *blocks_per_row = S;
+  if (entries) *entries = (int64_t)op->grid->J * S * Nt * Nt;
+  if (ncolors) *ncolors = op->ncolors;
+  if (cols) std::copy(op->cols_host.begin(), op->cols_host.end(), cols);
+  return BOSSS_OK;
+}
+
+int bosss_hip_op_assemble_dev(bosss_op* op, const double* U, double alpha, double beta, double* blocks, void* stream) {
+  if (!op || !U || !blocks) return fail(BOSSS_ERR_ARG, "NULL argument");
+  if (op->precision != BOSSS_FP64) return fail(BOSSS_ERR_UNSUPPORTED, "assembly is FP64 only");
+  int rc;
+  HIPCHK(hipSetDevice(op->ctx->device));
+  if ((rc = ensure_matrix_setup(op)) || (rc = ensure_staging(op))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int J = op->grid->J, Nt = op->V * op->N, S = 1 + op->grid->NF;
+  for (int k = 0; k < op->ncolors; k++)
+    for (int q = 0; q < Nt; q++) {
+      launch_probe(J, Nt, op->d_color, k, q, op->hX, s);
+      if ((rc = launch(op, KernelSet::JVP, 1.0, 0.0, U, op->hX, nullptr, op->hY, s))) return rc;
+      launch_extract(J, Nt, S, op->d_cols, op->d_color, k, q, op->hY, alpha, beta, blocks, s);
+    }
+  HIPCHK(hipGetLastError());
+  return BOSSS_OK;
+}
+
+int bosss_hip_bell_spmv_dev(bosss_op* op, const double* blocks, double alpha, const double* x, double beta, double* y, void* stream) {
+  if (!op || !blocks || !x || !y) return fail(BOSSS_ERR_ARG, "NULL argument");
+  int rc;
+  HIPCHK(hipSetDevice(op->ctx->device));
+  if ((rc = ensure_matrix_setup(op))) return rc;
+  launch_bell_spmv(op->grid->J, op->V * op->N, 1 + op->grid->NF, op->d_cols, blocks, alpha, x, beta, y, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  return BOSSS_OK;
 }
 
 }  // extern "C"

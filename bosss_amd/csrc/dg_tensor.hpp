@@ -1,0 +1,890 @@
+// Cell-centric DG operator kernels, tensor-table variant (hypercube cells, tensor Gauss rules).
+//
+// Same algorithm and data flow as dg_kernels.hpp (one wavefront = 16 cells = the
+// 16 MFMA columns; volume + all 2D faces per cell; transposed MFMA contractions),
+// with the register and LDS budget rearranged so the FP64 Euler p=3 kernels do
+// not spill:
+//
+//  * the reference tables phi, dphi, traces are never stored: every A operand is
+//    the product of D one-dimensional factors L_a(x_q) (or L'_a, or the endpoint
+//    values at -1/+1) read from a ~0.5 KiB table in LDS.  The host verifies at
+//    operator creation that the uploaded tables are exactly this tensor product
+//    (to 1e-13), which they are for the reference's hypercube elements;
+//  * the LDS that the tables no longer need holds each wavefront's own 16 cells'
+//    coefficients (U, and W for the adjoint), staged with one coalesced copy and
+//    then read as B operands;
+//  * the neighbour's B operands are loaded just in time from global memory;
+//  * for Euler the face cotangent is merged before the pointwise adjoint
+//    (fn = -fo, so only sc_own*W_own.trace - sc_nb*W_nb.trace is needed) and the
+//    pointwise adjoint is the hand-written reverse sweep of euler_adjoint.hpp.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "dg_kernels.hpp"
+#include "euler_adjoint.hpp"
+#include "forms.hpp"
+
+namespace bosss {
+
+constexpr int tdeg_count(int D, int p) { return D == 2 ? (p + 1) * (p + 2) / 2 : (p + 1) * (p + 2) * (p + 3) / 6; }
+
+// LDS layout of the 1D tables: L[P+1][K2], DL[P+1][K2] (K2 = K1 + 2: Gauss nodes, then x=-1, x=+1),
+// w1[K1], then the packed mode exponents code[N] = a | b << 4 | c << 8 (as Real-sized slots).
+template <int D_, int P_, int K1_>
+struct TShape {
+  static constexpr int D = D_, P = P_, K1 = K1_, K2 = K1 + 2;
+  static constexpr int N = tdeg_count(D, P);
+  static constexpr int KV = ipow(K1, D), KE = ipow(K1, D - 1), NF = 2 * D;
+  static constexpr int TV = (KV + 15) / 16, TE = (KE + 15) / 16;
+  static constexpr int NK = (N + 3) / 4, MT = (N + 15) / 16;
+  static constexpr int O_L = 0, O_DL = (P + 1) * K2, O_W = 2 * (P + 1) * K2, O_CODE = O_W + K1;
+  static constexpr int TAB = O_CODE + N;     // elements (Real) of the 1D table block
+};
+
+// Per-axis 1D indices of a node: volume node kv = (i*K1 + j)*K1 + k has x = k, y = j, z = i
+// (Cube.cs:167-178; 2D: kv = i*K1 + j, x = j, y = i).  Face f node kf (Square rule
+// kf = i*K1 + j: s0 = node j, s1 = node i; Line rule in 2D: s0 = node kf) is mapped by
+// the face coordinate maps of Cube/Square.TransformFaceCoordinates; endpoint -1 -> K1, +1 -> K1+1.
+template <int D, int K1>
+__device__ __forceinline__ void vol_axes(int kv, int (&q)[3]) {
+  if (D == 3) { q[0] = kv % K1; q[1] = (kv / K1) % K1; q[2] = kv / (K1 * K1); }
+  else { q[0] = kv % K1; q[1] = kv / K1; q[2] = 0; }
+}
+template <int D, int K1>
+__device__ __forceinline__ void face_axes(int f, int kf, int (&q)[3]) {
+  const int lo = K1, hi = K1 + 1;
+  if (D == 3) {
+    int s0 = kf % K1, s1 = kf / K1;
+    switch (f) {
+      case 0: q[0] = lo; q[1] = s1; q[2] = s0; break;
+      case 1: q[0] = hi; q[1] = s1; q[2] = s0; break;
+      case 2: q[0] = s0; q[1] = hi; q[2] = s1; break;
+      case 3: q[0] = s0; q[1] = lo; q[2] = s1; break;
+      case 4: q[0] = s0; q[1] = s1; q[2] = hi; break;
+      default: q[0] = s0; q[1] = s1; q[2] = lo; break;
+    }
+  } else {
+    int s0 = kf;
+    switch (f) {
+      case 0: q[0] = lo; q[1] = s0; break;
+      case 1: q[0] = hi; q[1] = s0; break;
+      case 2: q[0] = s0; q[1] = hi; break;
+      default: q[0] = s0; q[1] = lo; break;
+    }
+    q[2] = 0;
+  }
+}
+
+// table value: deriv = -1 -> phi_n, deriv = e -> d phi_n / d xi_e
+template <class TS, typename Real>
+__device__ __forceinline__ Real tval(const Real* tab, const int (&q)[3], int code, int deriv) {
+  const Real* L = tab + TS::O_L;
+  const Real* DL = tab + TS::O_DL;
+  int a = code & 15, b = (code >> 4) & 15;
+  Real x = (deriv == 0 ? DL : L)[a * TS::K2 + q[0]];
+  Real y = (deriv == 1 ? DL : L)[b * TS::K2 + q[1]];
+  if (TS::D == 3) {
+    int c = (code >> 8) & 15;
+    Real z = (deriv == 2 ? DL : L)[c * TS::K2 + q[2]];
+    return x * y * z;
+  }
+  return x * y;
+}
+
+template <class TS, typename Real>
+__device__ __forceinline__ int tcode(const Real* tab, int n) {
+  return (n < TS::N) ? (int)tab[TS::O_CODE + n] : -1;
+}
+
+// eval: acc[v] (nodes x cells) = table(node(col), mode) . B,   B operand from `Bsrc(v, kk)`
+template <class TS, typename Real, int V, class BF>
+__device__ __forceinline__ void t_eval(const Real* tab, const int (&q)[3], int deriv, int grp, BF Bsrc,
+                                       typename MM<Real>::acc (&acc)[V]) {
+#pragma unroll
+  for (int v = 0; v < V; v++) acc[v] = acc_zero<Real>();
+#pragma unroll
+  for (int kk = 0; kk < TS::NK; kk++) {
+    int code = tcode<TS>(tab, 4 * kk + grp);
+    Real a = (code >= 0) ? tval<TS>(tab, q, code, deriv) : Real(0);
+#pragma unroll
+    for (int v = 0; v < V; v++) acc[v] = MM<Real>::mma(a, Bsrc(v, kk), acc[v]);
+  }
+}
+
+// test: racc[v][mt] += table(node(grp,r), mode(mt*16 + col))^T g[v]
+template <class TS, typename Real, int V>
+__device__ __forceinline__ void t_test(const Real* tab, const int (&q)[3], int deriv, int col, const Real (&g)[V],
+                                       typename MM<Real>::acc (&racc)[V][TS::MT]) {
+#pragma unroll
+  for (int mt = 0; mt < TS::MT; mt++) {
+    int code = tcode<TS>(tab, mt * 16 + col);
+    Real a = (code >= 0) ? tval<TS>(tab, q, code, deriv) : Real(0);
+#pragma unroll
+    for (int v = 0; v < V; v++) racc[v][mt] = MM<Real>::mma(a, g[v], racc[v][mt]);
+  }
+}
+
+template <class TS, typename Real>
+__device__ __forceinline__ Real vol_weight(const Real* tab, const int (&q)[3]) {
+  const Real* w = tab + TS::O_W;
+  if (TS::D == 3) return (w[q[2]] * w[q[1]]) * w[q[0]];
+  return w[q[1]] * w[q[0]];
+}
+template <class TS, typename Real>
+__device__ __forceinline__ Real face_weight(const Real* tab, int kf) {
+  const Real* w = tab + TS::O_W;
+  if (TS::D == 3) return w[kf / TS::K1] * w[kf % TS::K1];
+  return w[kf];
+}
+
+// Stage the wave's 16 cells of one array into LDS: dst[c * stride + i], i < V*N (coalesced copy).
+template <typename Real, int NT>
+__device__ __forceinline__ void stage16(Real* dst, int stride, const Real* __restrict__ src, int cell0, int J, int lane) {
+  for (int i = lane; i < 16 * NT; i += 64) {
+    int c = i / NT, e = i - c * NT;
+    dst[c * stride + e] = (cell0 + c < J) ? src[(size_t)cell0 * NT + i] : Real(0);
+  }
+}
+
+template <typename Real>
+struct TArgs {
+  int J;
+  const double* detJ;
+  const double* invJ;
+  const double* scaling;
+  const double* lenscale;
+  CellFaces cf;
+  const Real* tab;      // 1D tables (TShape layout) in global memory
+  FormParams fp;
+  double alpha, beta;
+  const Real* U;
+  const Real* X;        // tangent (JVP) or cotangent W (VJP)
+  Real* R;
+  Real* out;
+};
+
+template <class Form> struct HandAdjoint : std::false_type {};
+template <int D, bool H> struct HandAdjoint<FormEuler<D, H>> : std::true_type {};
+template <class Form> struct IsHLLC : std::false_type {};
+template <int D> struct IsHLLC<FormEuler<D, true>> : std::true_type {};
+
+constexpr int TWAVES = 4;
+
+// ----------------------------------------------------------------------------
+// Forward: residual (MODE_RES) or JVP (MODE_JVP).
+// ----------------------------------------------------------------------------
+template <class Form, int D, int P, int K1, typename Real, int MODE>
+__global__ void __launch_bounds__(64 * TWAVES) kt_forward(TArgs<Real> a) {
+  typedef TShape<D, P, K1> TS;
+  typedef typename MM<Real>::acc acc_t;
+  constexpr int V = Form::V, N = TS::N, NT = V * N, NF = TS::NF, MT = TS::MT;
+  constexpr bool TAN = (MODE == MODE_JVP);
+  constexpr int NARR = TAN ? 2 : 1;
+  constexpr int STRIDE = NT | 1;
+  typedef typename std::conditional<TAN, Dual<1, Real>, Real>::type T;
+
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Real* tab = reinterpret_cast<Real*>(smem_raw);
+  for (int i = threadIdx.x; i < TS::TAB; i += blockDim.x) tab[i] = a.tab[i];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, grp = lane >> 4;
+  const int cell0 = (blockIdx.x * TWAVES + wave) * 16;
+  Real* stU = tab + ((TS::TAB + 1) & ~1) + wave * (NARR * 16 * STRIDE);
+  Real* stX = stU + 16 * STRIDE;
+  stage16<Real, NT>(stU, STRIDE, a.U, cell0, a.J, lane);
+  if (TAN) stage16<Real, NT>(stX, STRIDE, a.X, cell0, a.J, lane);
+  __syncthreads();
+
+  const int cell = cell0 + col;
+  const bool valid = cell < a.J;
+  const int cj = valid ? cell : 0;
+  const Real sc = Real(a.scaling[cj]);
+  const Real detJ = Real(a.detJ[cj]);
+  Real iJ[D][D];
+#pragma unroll
+  for (int e = 0; e < D; e++)
+#pragma unroll
+    for (int d = 0; d < D; d++) iJ[e][d] = Real(a.invJ[(size_t)cj * D * D + e * D + d]);
+
+  acc_t racc[V][MT];
+#pragma unroll
+  for (int v = 0; v < V; v++)
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) racc[v][mt] = acc_zero<Real>();
+
+  const Real* ownU = stU + col * STRIDE;
+  const Real* ownX = stX + col * STRIDE;
+  auto Bu = [&](int v, int kk) { int m = 4 * kk + grp; return (m < N) ? ownU[v * N + m] * sc : Real(0); };
+  auto Bx = [&](int v, int kk) { int m = 4 * kk + grp; return (m < N) ? ownX[v * N + m] * sc : Real(0); };
+
+  // ------------------------------------------------------------------ volume
+  for (int t = 0; t < TS::TV; t++) {
+    int qe[3];
+    const int nodeE = t * 16 + col;
+    vol_axes<D, K1>(nodeE < TS::KV ? nodeE : 0, qe);
+    acc_t u[V], du[V], gu[D][V], dgu[D][V];
+    t_eval<TS, Real, V>(tab, qe, -1, grp, Bu, u);
+    if (TAN) t_eval<TS, Real, V>(tab, qe, -1, grp, Bx, du);
+    if (Form::VOL_GRADU) {
+#pragma unroll
+      for (int e = 0; e < D; e++) {
+        t_eval<TS, Real, V>(tab, qe, e, grp, Bu, gu[e]);
+        if (TAN) t_eval<TS, Real, V>(tab, qe, e, grp, Bx, dgu[e]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int node = t * 16 + MM<Real>::row(grp, r);
+      const bool pad = node >= TS::KV;
+      int q[3];
+      vol_axes<D, K1>(pad ? 0 : node, q);
+      const Real wq = pad ? Real(0) : vol_weight<TS>(tab, q) * detJ * sc;
+      T Uq[V], GU[V][D], F[V][D];
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        if constexpr (TAN) { Uq[v].v = u[v][r]; Uq[v].d[0] = du[v][r]; } else { Uq[v] = u[v][r]; }
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+          if (Form::VOL_GRADU) {
+            Real gv = 0, gt = 0;
+#pragma unroll
+            for (int e = 0; e < D; e++) { gv += gu[e][v][r] * iJ[e][d]; if (TAN) gt += dgu[e][v][r] * iJ[e][d]; }
+            if constexpr (TAN) { GU[v][d].v = gv; GU[v][d].d[0] = gt; } else { GU[v][d] = gv; }
+          } else {
+            GU[v][d] = Uq[v];
+          }
+        }
+      }
+      Form::template volume<D, T, Real>(a.fp, Uq, GU, F);
+#pragma unroll
+      for (int e = 0; e < D; e++) {
+        Real g[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          Real s = 0;
+#pragma unroll
+          for (int d = 0; d < D; d++) {
+            if constexpr (TAN) s += F[v][d].d[0] * iJ[e][d]; else s += F[v][d] * iJ[e][d];
+          }
+          g[v] = pad ? Real(0) : s * wq;
+        }
+        t_test<TS, Real, V>(tab, q, e, col, g, racc);
+      }
+    }
+  }
+
+  // ------------------------------------------------------------------- faces
+  const Real pbd = Real(a.fp.penalty_base * a.fp.penalty_deg);
+  const Real cj_own = Real(1.0 / a.lenscale[cj]);
+  for (int f = 0; f < NF; f++) {
+    const int nb = valid ? a.cf.nb[(size_t)cj * NF + f] : -1;
+    const bool has_nb = nb >= 0;
+    const int cn = has_nb ? nb : cj;
+    const int fo = a.cf.pair[f];
+    const Real sg = Real(valid ? a.cf.sqrtg[(size_t)cj * NF + f] : 0.0);
+    Real nrm[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) nrm[d] = Real(a.cf.normal[((size_t)cj * NF + f) * D + d]);
+    Real mu = pbd * cj_own;
+    if (has_nb) { Real mo = pbd * Real(1.0 / a.lenscale[cn]); mu = (mo > mu) ? mo : mu; }
+    const Real scn = Real(a.scaling[cn]);
+    Real iJn[D][D];
+#pragma unroll
+    for (int e = 0; e < D; e++)
+#pragma unroll
+      for (int d = 0; d < D; d++) iJn[e][d] = Form::FACE_GRADU ? Real(a.invJ[(size_t)cn * D * D + e * D + d]) : Real(0);
+    const Real* nbU = a.U + (size_t)cn * NT;
+    const Real* nbX = TAN ? a.X + (size_t)cn * NT : nullptr;
+    auto Bnu = [&](int v, int kk) { int m = 4 * kk + grp; return (has_nb && m < N) ? nbU[v * N + m] * scn : Real(0); };
+    auto Bnx = [&](int v, int kk) { int m = 4 * kk + grp; return (has_nb && m < N) ? nbX[v * N + m] * scn : Real(0); };
+
+    for (int t = 0; t < TS::TE; t++) {
+      const int nodeE = t * 16 + col;
+      int qo[3], qn[3];
+      face_axes<D, K1>(f, nodeE < TS::KE ? nodeE : 0, qo);
+      face_axes<D, K1>(fo, nodeE < TS::KE ? nodeE : 0, qn);
+      acc_t uo[V], un[V], duo[V], dun[V], guo[D][V], gun[D][V], dguo[D][V], dgun[D][V];
+      t_eval<TS, Real, V>(tab, qo, -1, grp, Bu, uo);
+      t_eval<TS, Real, V>(tab, qn, -1, grp, Bnu, un);
+      if (TAN) {
+        t_eval<TS, Real, V>(tab, qo, -1, grp, Bx, duo);
+        t_eval<TS, Real, V>(tab, qn, -1, grp, Bnx, dun);
+      }
+      if (Form::FACE_GRADU) {
+#pragma unroll
+        for (int e = 0; e < D; e++) {
+          t_eval<TS, Real, V>(tab, qo, e, grp, Bu, guo[e]);
+          t_eval<TS, Real, V>(tab, qn, e, grp, Bnu, gun[e]);
+          if (TAN) {
+            t_eval<TS, Real, V>(tab, qo, e, grp, Bx, dguo[e]);
+            t_eval<TS, Real, V>(tab, qn, e, grp, Bnx, dgun[e]);
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int node = t * 16 + MM<Real>::row(grp, r);
+        const bool pad = node >= TS::KE;
+        int q[3];
+        face_axes<D, K1>(f, pad ? 0 : node, q);
+        const Real wq = pad ? Real(0) : face_weight<TS>(tab, node) * sg * sc;
+        T Uo[V], Un[V], Go[V][D], Gn[V][D];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          if constexpr (TAN) {
+            Uo[v].v = uo[v][r]; Uo[v].d[0] = duo[v][r];
+            Un[v].v = un[v][r]; Un[v].d[0] = dun[v][r];
+          } else { Uo[v] = uo[v][r]; Un[v] = un[v][r]; }
+          if (!has_nb) Un[v] = Uo[v];
+#pragma unroll
+          for (int d = 0; d < D; d++) {
+            if (Form::FACE_GRADU) {
+              Real go_ = 0, gn_ = 0, tgo = 0, tgn = 0;
+#pragma unroll
+              for (int e = 0; e < D; e++) {
+                go_ += guo[e][v][r] * iJ[e][d];
+                gn_ += gun[e][v][r] * iJn[e][d];
+                if (TAN) { tgo += dguo[e][v][r] * iJ[e][d]; tgn += dgun[e][v][r] * iJn[e][d]; }
+              }
+              if constexpr (TAN) { Go[v][d].v = go_; Go[v][d].d[0] = tgo; Gn[v][d].v = gn_; Gn[v][d].d[0] = tgn; }
+              else { Go[v][d] = go_; Gn[v][d] = gn_; }
+              if (!has_nb) Gn[v][d] = Go[v][d];
+            } else {
+              Go[v][d] = Uo[v]; Gn[v][d] = Uo[v];
+            }
+          }
+        }
+        T fo_[V], go_[V][D], fn_[V], gn_[V][D];
+        if (has_nb) {
+          Form::template inner<D, T, Real>(a.fp, nrm, mu, Uo, Un, Go, Gn, fo_, go_, fn_, gn_);
+        } else {
+#pragma unroll
+          for (int v = 0; v < V; v++) { fo_[v] = Uo[v]; for (int d = 0; d < D; d++) go_[v][d] = Uo[v]; }
+          Form::template boundary<D, T, Real>(a.fp, nrm, mu, Uo, Go, fo_, go_);
+        }
+        Real g[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          Real s;
+          if constexpr (TAN) s = fo_[v].d[0]; else s = fo_[v];
+          g[v] = pad ? Real(0) : s * wq;
+        }
+        t_test<TS, Real, V>(tab, q, -1, col, g, racc);
+        if (Form::FACE_TESTG) {
+#pragma unroll
+          for (int e = 0; e < D; e++) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+              Real s = 0;
+#pragma unroll
+              for (int d = 0; d < D; d++) {
+                if constexpr (TAN) s += go_[v][d].d[0] * iJ[e][d]; else s += go_[v][d] * iJ[e][d];
+              }
+              g[v] = pad ? Real(0) : s * wq;
+            }
+            t_test<TS, Real, V>(tab, q, e, col, g, racc);
+          }
+        }
+      }
+    }
+  }
+
+  if (valid) {
+    Real* Op = (TAN ? a.out : a.R) + (size_t)cell * NT;
+    const Real al = Real(a.alpha), be = Real(a.beta);
+#pragma unroll
+    for (int v = 0; v < V; v++)
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          int n = mt * 16 + MM<Real>::row(grp, r);
+          if (n < N) {
+            if (TAN) {
+              Op[v * N + n] = racc[v][mt][r];
+            } else {
+              Real old = (be != Real(0)) ? be * Op[v * N + n] : Real(0);
+              Op[v * N + n] = old + al * racc[v][mt][r];
+            }
+          }
+        }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Adjoint: out <- J(U)^T W  (FUSED: also R <- beta R + alpha Op(U)).
+// ----------------------------------------------------------------------------
+template <class Form, int D, int P, int K1, typename Real, bool FUSED>
+__global__ void __launch_bounds__(64 * TWAVES) kt_adjoint(TArgs<Real> a) {
+  typedef TShape<D, P, K1> TS;
+  typedef typename MM<Real>::acc acc_t;
+  constexpr int V = Form::V, N = TS::N, NT = V * N, NF = TS::NF, MT = TS::MT;
+  constexpr int STRIDE = NT | 1;
+  constexpr bool HAND = HandAdjoint<Form>::value;
+  constexpr bool GU_IN = Form::VOL_GRADU || Form::FACE_GRADU;
+  constexpr int M = GU_IN ? V * (1 + D) : V;
+  typedef Dual<M, Real> TD;
+
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Real* tab = reinterpret_cast<Real*>(smem_raw);
+  for (int i = threadIdx.x; i < TS::TAB; i += blockDim.x) tab[i] = a.tab[i];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, grp = lane >> 4;
+  const int cell0 = (blockIdx.x * TWAVES + wave) * 16;
+  Real* stU = tab + ((TS::TAB + 1) & ~1) + wave * (2 * 16 * STRIDE);
+  Real* stW = stU + 16 * STRIDE;
+  stage16<Real, NT>(stU, STRIDE, a.U, cell0, a.J, lane);
+  stage16<Real, NT>(stW, STRIDE, a.X, cell0, a.J, lane);
+  __syncthreads();
+
+  const int cell = cell0 + col;
+  const bool valid = cell < a.J;
+  const int cj = valid ? cell : 0;
+  const Real sc = Real(a.scaling[cj]);
+  const Real detJ = Real(a.detJ[cj]);
+  Real iJ[D][D];
+#pragma unroll
+  for (int e = 0; e < D; e++)
+#pragma unroll
+    for (int d = 0; d < D; d++) iJ[e][d] = Real(a.invJ[(size_t)cj * D * D + e * D + d]);
+
+  acc_t racc[V][MT], aacc[V][MT];
+#pragma unroll
+  for (int v = 0; v < V; v++)
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) { racc[v][mt] = acc_zero<Real>(); aacc[v][mt] = acc_zero<Real>(); }
+
+  const Real* ownU = stU + col * STRIDE;
+  const Real* ownW = stW + col * STRIDE;
+  auto Bu = [&](int v, int kk) { int m = 4 * kk + grp; return (m < N) ? ownU[v * N + m] * sc : Real(0); };
+  auto Bw = [&](int v, int kk) { int m = 4 * kk + grp; return (m < N) ? ownW[v * N + m] : Real(0); };
+
+  // ------------------------------------------------------------------ volume
+  if constexpr (HAND) {
+    // Hand-written adjoint: the cotangent of the volume flux is linear in grad(W), so the
+    // three reference-gradient directions are evaluated and pulled back one at a time
+    // (keeps one 4-node gradient array live instead of D).
+    for (int t = 0; t < TS::TV; t++) {
+      int qe[3];
+      const int nodeE = t * 16 + col;
+      vol_axes<D, K1>(nodeE < TS::KV ? nodeE : 0, qe);
+      acc_t u[V];
+      t_eval<TS, Real, V>(tab, qe, -1, grp, Bu, u);
+      Real ub[4][V];
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int v = 0; v < V; v++) ub[r][v] = Real(0);
+      if (FUSED) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int node = t * 16 + MM<Real>::row(grp, r);
+          const bool pad = node >= TS::KV;
+          int q[3];
+          vol_axes<D, K1>(pad ? 0 : node, q);
+          const Real wq = pad ? Real(0) : vol_weight<TS>(tab, q) * detJ * sc;
+          Real Uq[V], GUd[V][D], Fv[V][D];
+#pragma unroll
+          for (int v = 0; v < V; v++) { Uq[v] = u[v][r]; for (int d = 0; d < D; d++) GUd[v][d] = Uq[v]; }
+          Form::template volume<D, Real, Real>(a.fp, Uq, GUd, Fv);
+#pragma unroll
+          for (int e = 0; e < D; e++) {
+            Real g[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+              Real s = 0;
+#pragma unroll
+              for (int d = 0; d < D; d++) s += Fv[v][d] * iJ[e][d];
+              g[v] = pad ? Real(0) : s * wq;
+            }
+            t_test<TS, Real, V>(tab, q, e, col, g, racc);
+          }
+        }
+      }
+      for (int e = 0; e < D; e++) {
+        acc_t wg[V];
+        t_eval<TS, Real, V>(tab, qe, e, grp, Bw, wg);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int node = t * 16 + MM<Real>::row(grp, r);
+          const bool pad = node >= TS::KV;
+          int q[3];
+          vol_axes<D, K1>(pad ? 0 : node, q);
+          const Real wq = pad ? Real(0) : vol_weight<TS>(tab, q) * detJ * sc;
+          Real Uq[V], Fb[V][D];
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            Uq[v] = u[v][r];
+#pragma unroll
+            for (int d = 0; d < D; d++) Fb[v][d] = iJ[e][d] * wg[v][r] * wq;
+          }
+          EulerAdjoint<D, Real>::volume_vjp(a.fp, Uq, Fb, ub[r]);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int node = t * 16 + MM<Real>::row(grp, r);
+        const bool pad = node >= TS::KV;
+        int q[3];
+        vol_axes<D, K1>(pad ? 0 : node, q);
+        Real bv[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) bv[v] = pad ? Real(0) : ub[r][v] * sc;
+        t_test<TS, Real, V>(tab, q, -1, col, bv, aacc);
+      }
+    }
+  } else {
+  for (int t = 0; t < TS::TV; t++) {
+    int qe[3];
+    const int nodeE = t * 16 + col;
+    vol_axes<D, K1>(nodeE < TS::KV ? nodeE : 0, qe);
+    acc_t u[V], gu[D][V], wg[D][V];
+    t_eval<TS, Real, V>(tab, qe, -1, grp, Bu, u);
+#pragma unroll
+    for (int e = 0; e < D; e++) {
+      if (Form::VOL_GRADU) t_eval<TS, Real, V>(tab, qe, e, grp, Bu, gu[e]);
+      t_eval<TS, Real, V>(tab, qe, e, grp, Bw, wg[e]);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int node = t * 16 + MM<Real>::row(grp, r);
+      const bool pad = node >= TS::KV;
+      int q[3];
+      vol_axes<D, K1>(pad ? 0 : node, q);
+      const Real wq = pad ? Real(0) : vol_weight<TS>(tab, q) * detJ * sc;
+      // cotangent of the physical volume flux: Fbar[v][d] = wq * sum_e invJ[e][d] (grad_xi W)_e
+      Real Fb[V][D];
+#pragma unroll
+      for (int v = 0; v < V; v++)
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+          Real s = 0;
+#pragma unroll
+          for (int e = 0; e < D; e++) s += iJ[e][d] * wg[e][v][r];
+          Fb[v][d] = s * wq;
+        }
+      Real ub[V], gb[V][D];
+      Real Fv[V][D];
+      if constexpr (HAND) {
+        Real Uq[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) { Uq[v] = u[v][r]; ub[v] = 0; }
+        EulerAdjoint<D, Real>::volume_vjp(a.fp, Uq, Fb, ub);
+        if (FUSED) {
+          Real GUd[V][D];
+#pragma unroll
+          for (int v = 0; v < V; v++) for (int d = 0; d < D; d++) GUd[v][d] = Uq[v];
+          Form::template volume<D, Real, Real>(a.fp, Uq, GUd, Fv);
+        }
+      } else {
+        TD Uq[V], GU[V][D], F[V][D];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          Uq[v] = dconst<M, Real>(u[v][r]);
+          Uq[v].d[v] = Real(1);
+#pragma unroll
+          for (int d = 0; d < D; d++) {
+            if (Form::VOL_GRADU) {
+              Real gv = 0;
+#pragma unroll
+              for (int e = 0; e < D; e++) gv += gu[e][v][r] * iJ[e][d];
+              GU[v][d] = dconst<M, Real>(gv);
+              if (GU_IN) GU[v][d].d[(V + v * D + d) % M] = Real(1);
+            } else {
+              GU[v][d] = Uq[v];
+            }
+          }
+        }
+        Form::template volume<D, TD, Real>(a.fp, Uq, GU, F);
+#pragma unroll
+        for (int qq = 0; qq < V; qq++) { ub[qq] = 0; for (int dd = 0; dd < D; dd++) gb[qq][dd] = 0; }
+#pragma unroll
+        for (int v = 0; v < V; v++)
+#pragma unroll
+          for (int d = 0; d < D; d++) {
+            Fv[v][d] = F[v][d].v;
+#pragma unroll
+            for (int qq = 0; qq < V; qq++) {
+              ub[qq] += Fb[v][d] * F[v][d].d[qq];
+              if (Form::VOL_GRADU) {
+#pragma unroll
+                for (int dd = 0; dd < D; dd++) gb[qq][dd] += Fb[v][d] * F[v][d].d[(V + qq * D + dd) % M];
+              }
+            }
+          }
+      }
+      if (FUSED) {
+#pragma unroll
+        for (int e = 0; e < D; e++) {
+          Real g[V];
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            Real s = 0;
+#pragma unroll
+            for (int d = 0; d < D; d++) s += Fv[v][d] * iJ[e][d];
+            g[v] = pad ? Real(0) : s * wq;
+          }
+          t_test<TS, Real, V>(tab, q, e, col, g, racc);
+        }
+      }
+      Real bv[V];
+#pragma unroll
+      for (int v = 0; v < V; v++) bv[v] = pad ? Real(0) : ub[v] * sc;
+      t_test<TS, Real, V>(tab, q, -1, col, bv, aacc);
+      if constexpr (!HAND) {
+        if (Form::VOL_GRADU) {
+#pragma unroll
+          for (int e = 0; e < D; e++) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+              Real s = 0;
+#pragma unroll
+              for (int d = 0; d < D; d++) s += gb[v][d] * iJ[e][d];
+              bv[v] = pad ? Real(0) : s * sc;
+            }
+            t_test<TS, Real, V>(tab, q, e, col, bv, aacc);
+          }
+        }
+      }
+    }
+  }
+
+  }
+
+  // ------------------------------------------------------------------- faces
+  const Real pbd = Real(a.fp.penalty_base * a.fp.penalty_deg);
+  const Real cj_own = Real(1.0 / a.lenscale[cj]);
+  for (int f = 0; f < NF; f++) {
+    const int nb = valid ? a.cf.nb[(size_t)cj * NF + f] : -1;
+    const bool has_nb = nb >= 0;
+    const int cn = has_nb ? nb : cj;
+    const int fo = a.cf.pair[f];
+    const Real sg = Real(valid ? a.cf.sqrtg[(size_t)cj * NF + f] : 0.0);
+    Real nrm[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) nrm[d] = Real(a.cf.normal[((size_t)cj * NF + f) * D + d]);
+    Real mu = pbd * cj_own;
+    if (has_nb) { Real mo = pbd * Real(1.0 / a.lenscale[cn]); mu = (mo > mu) ? mo : mu; }
+    const Real scn = Real(a.scaling[cn]);
+    Real iJn[D][D];
+#pragma unroll
+    for (int e = 0; e < D; e++)
+#pragma unroll
+      for (int d = 0; d < D; d++) iJn[e][d] = HAND ? Real(0) : Real(a.invJ[(size_t)cn * D * D + e * D + d]);
+    const Real* nbU = a.U + (size_t)cn * NT;
+    const Real* nbW = a.X + (size_t)cn * NT;
+    auto Bnu = [&](int v, int kk) { int m = 4 * kk + grp; return (has_nb && m < N) ? nbU[v * N + m] * scn : Real(0); };
+    auto Bnw = [&](int v, int kk) { int m = 4 * kk + grp; return (has_nb && m < N) ? nbW[v * N + m] : Real(0); };
+
+    for (int t = 0; t < TS::TE; t++) {
+      const int nodeE = t * 16 + col;
+      int qo[3], qn[3];
+      face_axes<D, K1>(f, nodeE < TS::KE ? nodeE : 0, qo);
+      face_axes<D, K1>(fo, nodeE < TS::KE ? nodeE : 0, qn);
+      acc_t uo[V], un[V];
+      t_eval<TS, Real, V>(tab, qo, -1, grp, Bu, uo);
+      t_eval<TS, Real, V>(tab, qn, -1, grp, Bnu, un);
+      if constexpr (HAND) {
+        // merged face cotangent  gbar = sc W_own.trace - scn W_nb.trace  (fn = -fo)
+        acc_t gw[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) gw[v] = acc_zero<Real>();
+#pragma unroll
+        for (int kk = 0; kk < TS::NK; kk++) {
+          int code = tcode<TS>(tab, 4 * kk + grp);
+          Real ao = (code >= 0) ? tval<TS>(tab, qo, code, -1) : Real(0);
+          Real an = (code >= 0) ? tval<TS>(tab, qn, code, -1) : Real(0);
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            gw[v] = MM<Real>::mma(ao, Bw(v, kk) * sc, gw[v]);
+            gw[v] = MM<Real>::mma(an, -Bnw(v, kk) * scn, gw[v]);
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int node = t * 16 + MM<Real>::row(grp, r);
+          const bool pad = node >= TS::KE;
+          int q[3];
+          face_axes<D, K1>(f, pad ? 0 : node, q);
+          const Real wsg = pad ? Real(0) : face_weight<TS>(tab, node) * sg;
+          Real Uo[V], Un[V], gbar[V], fo_[V], ub[V];
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            Uo[v] = uo[v][r];
+            Un[v] = has_nb ? un[v][r] : uo[v][r];
+            gbar[v] = gw[v][r] * wsg;
+            ub[v] = 0;
+          }
+          EulerAdjoint<D, Real>::template inner_vjp<IsHLLC<Form>::value>(a.fp, nrm, Uo, Un, gbar, fo_, ub);
+          Real g[V];
+          if (FUSED) {
+#pragma unroll
+            for (int v = 0; v < V; v++) g[v] = pad ? Real(0) : fo_[v] * wsg * sc;
+            t_test<TS, Real, V>(tab, q, -1, col, g, racc);
+          }
+#pragma unroll
+          for (int v = 0; v < V; v++) g[v] = pad ? Real(0) : ub[v] * sc;
+          t_test<TS, Real, V>(tab, q, -1, col, g, aacc);
+        }
+      } else {
+        acc_t guo[D][V], gun[D][V], wo[V], wn[V], gwo[D][V], gwn[D][V];
+        t_eval<TS, Real, V>(tab, qo, -1, grp, Bw, wo);
+        t_eval<TS, Real, V>(tab, qn, -1, grp, Bnw, wn);
+#pragma unroll
+        for (int e = 0; e < D; e++) {
+          if (Form::FACE_GRADU) {
+            t_eval<TS, Real, V>(tab, qo, e, grp, Bu, guo[e]);
+            t_eval<TS, Real, V>(tab, qn, e, grp, Bnu, gun[e]);
+          }
+          if (Form::FACE_TESTG) {
+            t_eval<TS, Real, V>(tab, qo, e, grp, Bw, gwo[e]);
+            t_eval<TS, Real, V>(tab, qn, e, grp, Bnw, gwn[e]);
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int node = t * 16 + MM<Real>::row(grp, r);
+          const bool pad = node >= TS::KE;
+          int q[3];
+          face_axes<D, K1>(f, pad ? 0 : node, q);
+          const Real wsg = pad ? Real(0) : face_weight<TS>(tab, node) * sg;
+          TD Uo[V], Un[V], Go[V][D], Gn[V][D];
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            Uo[v] = dconst<M, Real>(uo[v][r]);
+            Uo[v].d[v] = Real(1);
+            Un[v] = has_nb ? dconst<M, Real>(un[v][r]) : Uo[v];
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+              if (Form::FACE_GRADU) {
+                Real go_ = 0, gn_ = 0;
+#pragma unroll
+                for (int e = 0; e < D; e++) { go_ += guo[e][v][r] * iJ[e][d]; gn_ += gun[e][v][r] * iJn[e][d]; }
+                Go[v][d] = dconst<M, Real>(go_);
+                if (GU_IN) Go[v][d].d[(V + v * D + d) % M] = Real(1);
+                Gn[v][d] = has_nb ? dconst<M, Real>(gn_) : Go[v][d];
+              } else {
+                Go[v][d] = Uo[v]; Gn[v][d] = Un[v];
+              }
+            }
+          }
+          TD fo_[V], go_[V][D], fn_[V], gn_[V][D];
+          if (has_nb) {
+            Form::template inner<D, TD, Real>(a.fp, nrm, mu, Uo, Un, Go, Gn, fo_, go_, fn_, gn_);
+          } else {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+              fo_[v] = Uo[v]; fn_[v] = dconst<M, Real>(Real(0));
+              for (int d = 0; d < D; d++) { go_[v][d] = Uo[v]; gn_[v][d] = dconst<M, Real>(Real(0)); }
+            }
+            Form::template boundary<D, TD, Real>(a.fp, nrm, mu, Uo, Go, fo_, go_);
+          }
+          Real ub[V], gb[V][D];
+#pragma unroll
+          for (int qq = 0; qq < V; qq++) { ub[qq] = 0; for (int d = 0; d < D; d++) gb[qq][d] = 0; }
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            Real fob = wsg * sc * wo[v][r];
+            Real fnb = has_nb ? wsg * scn * wn[v][r] : Real(0);
+#pragma unroll
+            for (int qq = 0; qq < V; qq++) {
+              ub[qq] += fob * fo_[v].d[qq] + fnb * fn_[v].d[qq];
+              if (GU_IN && Form::FACE_GRADU) {
+#pragma unroll
+                for (int dd = 0; dd < D; dd++)
+                  gb[qq][dd] += fob * fo_[v].d[(V + qq * D + dd) % M] + fnb * fn_[v].d[(V + qq * D + dd) % M];
+              }
+            }
+            if (Form::FACE_TESTG) {
+#pragma unroll
+              for (int d = 0; d < D; d++) {
+                Real gob = 0, gnb = 0;
+#pragma unroll
+                for (int e = 0; e < D; e++) { gob += iJ[e][d] * gwo[e][v][r]; gnb += iJn[e][d] * gwn[e][v][r]; }
+                gob *= wsg * sc;
+                gnb = has_nb ? gnb * wsg * scn : Real(0);
+#pragma unroll
+                for (int qq = 0; qq < V; qq++) {
+                  ub[qq] += gob * go_[v][d].d[qq] + gnb * gn_[v][d].d[qq];
+                  if (GU_IN && Form::FACE_GRADU) {
+#pragma unroll
+                    for (int dd = 0; dd < D; dd++)
+                      gb[qq][dd] += gob * go_[v][d].d[(V + qq * D + dd) % M] + gnb * gn_[v][d].d[(V + qq * D + dd) % M];
+                  }
+                }
+              }
+            }
+          }
+          Real g[V];
+          if (FUSED) {
+            const Real wq = wsg * sc;
+#pragma unroll
+            for (int v = 0; v < V; v++) g[v] = pad ? Real(0) : fo_[v].v * wq;
+            t_test<TS, Real, V>(tab, q, -1, col, g, racc);
+            if (Form::FACE_TESTG) {
+#pragma unroll
+              for (int e = 0; e < D; e++) {
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                  Real s = 0;
+#pragma unroll
+                  for (int d = 0; d < D; d++) s += go_[v][d].v * iJ[e][d];
+                  g[v] = pad ? Real(0) : s * wq;
+                }
+                t_test<TS, Real, V>(tab, q, e, col, g, racc);
+              }
+            }
+          }
+#pragma unroll
+          for (int v = 0; v < V; v++) g[v] = pad ? Real(0) : ub[v] * sc;
+          t_test<TS, Real, V>(tab, q, -1, col, g, aacc);
+          if (Form::FACE_GRADU) {
+#pragma unroll
+            for (int e = 0; e < D; e++) {
+#pragma unroll
+              for (int v = 0; v < V; v++) {
+                Real s = 0;
+#pragma unroll
+                for (int d = 0; d < D; d++) s += gb[v][d] * iJ[e][d];
+                g[v] = pad ? Real(0) : s * sc;
+              }
+              t_test<TS, Real, V>(tab, q, e, col, g, aacc);
+            }
+          }
+        }
+      }
+    }
+  }
+
+  if (valid) {
+    Real* Rp = FUSED ? a.R + (size_t)cell * NT : nullptr;
+    Real* Op = a.out + (size_t)cell * NT;
+    const Real al = Real(a.alpha), be = Real(a.beta);
+#pragma unroll
+    for (int v = 0; v < V; v++)
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          int n = mt * 16 + MM<Real>::row(grp, r);
+          if (n < N) {
+            if (FUSED) {
+              Real old = (be != Real(0)) ? be * Rp[v * N + n] : Real(0);
+              Rp[v * N + n] = old + al * racc[v][mt][r];
+            }
+            Op[v * N + n] = aacc[v][mt][r];
+          }
+        }
+  }
+}
+
+// LDS bytes of a launch: 1D tables + per-wave staging of the own coefficients
+template <int D, int P, int K1, int V, typename Real>
+constexpr size_t kt_lds_bytes(int narrays) {
+  typedef TShape<D, P, K1> TS;
+  return (size_t)(((TS::TAB + 1) & ~1) + TWAVES * narrays * 16 * ((V * TS::N) | 1)) * sizeof(Real);
+}
+
+}  // namespace bosss

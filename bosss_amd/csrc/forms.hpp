@@ -28,13 +28,13 @@ struct Dual {
   R v;
   R d[M];
 };
-template <int M, typename R> __device__ __forceinline__ Dual<M, R> dconst(R x) { Dual<M, R> r; r.v = x;
+template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> dconst(R x) { Dual<M, R> r; r.v = x;
 #pragma unroll
   for (int i = 0; i < M; i++) r.d[i] = R(0); return r; }
 
 #define BOSSS_DUAL_BINOP(OP, VEXPR, DEXPR)                                                            \
   template <int M, typename R>                                                                       \
-  __device__ __forceinline__ Dual<M, R> operator OP(const Dual<M, R>& a, const Dual<M, R>& b) {      \
+  __host__ __device__ __forceinline__ Dual<M, R> operator OP(const Dual<M, R>& a, const Dual<M, R>& b) {      \
     Dual<M, R> r; r.v = VEXPR;                                                                       \
     _Pragma("unroll") for (int i = 0; i < M; i++) r.d[i] = DEXPR;                                    \
     return r;                                                                                        \
@@ -43,56 +43,56 @@ BOSSS_DUAL_BINOP(+, a.v + b.v, a.d[i] + b.d[i])
 BOSSS_DUAL_BINOP(-, a.v - b.v, a.d[i] - b.d[i])
 BOSSS_DUAL_BINOP(*, a.v * b.v, a.d[i] * b.v + a.v * b.d[i])
 #undef BOSSS_DUAL_BINOP
-template <int M, typename R> __device__ __forceinline__ Dual<M, R> operator/(const Dual<M, R>& a, const Dual<M, R>& b) {
+template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> operator/(const Dual<M, R>& a, const Dual<M, R>& b) {
   Dual<M, R> r; R ib = R(1) / b.v; r.v = a.v * ib;
 #pragma unroll
   for (int i = 0; i < M; i++) r.d[i] = (a.d[i] - r.v * b.d[i]) * ib;
   return r;
 }
-template <int M, typename R> __device__ __forceinline__ Dual<M, R> operator-(const Dual<M, R>& a) {
+template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> operator-(const Dual<M, R>& a) {
   Dual<M, R> r; r.v = -a.v;
 #pragma unroll
   for (int i = 0; i < M; i++) r.d[i] = -a.d[i];
   return r;
 }
-template <int M, typename R> __device__ __forceinline__ Dual<M, R> operator*(const Dual<M, R>& a, R s) {
+template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> operator*(const Dual<M, R>& a, R s) {
   Dual<M, R> r; r.v = a.v * s;
 #pragma unroll
   for (int i = 0; i < M; i++) r.d[i] = a.d[i] * s;
   return r;
 }
-template <int M, typename R> __device__ __forceinline__ Dual<M, R> operator*(R s, const Dual<M, R>& a) { return a * s; }
-template <int M, typename R> __device__ __forceinline__ Dual<M, R> operator+(const Dual<M, R>& a, R s) { Dual<M, R> r = a; r.v += s; return r; }
-template <int M, typename R> __device__ __forceinline__ Dual<M, R> operator+(R s, const Dual<M, R>& a) { Dual<M, R> r = a; r.v += s; return r; }
-template <int M, typename R> __device__ __forceinline__ Dual<M, R> operator-(const Dual<M, R>& a, R s) { Dual<M, R> r = a; r.v -= s; return r; }
-template <int M, typename R> __device__ __forceinline__ Dual<M, R> operator-(R s, const Dual<M, R>& a) { Dual<M, R> r = -a; r.v += s; return r; }
-template <int M, typename R> __device__ __forceinline__ Dual<M, R> operator/(const Dual<M, R>& a, R s) { return a * (R(1) / s); }
-template <int M, typename R> __device__ __forceinline__ Dual<M, R> operator/(R s, const Dual<M, R>& b) { return dconst<M, R>(s) / b; }
-template <int M, typename R> __device__ __forceinline__ Dual<M, R> dsqrt(const Dual<M, R>& a) {
+template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> operator*(R s, const Dual<M, R>& a) { return a * s; }
+template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> operator+(const Dual<M, R>& a, R s) { Dual<M, R> r = a; r.v += s; return r; }
+template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> operator+(R s, const Dual<M, R>& a) { Dual<M, R> r = a; r.v += s; return r; }
+template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> operator-(const Dual<M, R>& a, R s) { Dual<M, R> r = a; r.v -= s; return r; }
+template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> operator-(R s, const Dual<M, R>& a) { Dual<M, R> r = -a; r.v += s; return r; }
+template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> operator/(const Dual<M, R>& a, R s) { return a * (R(1) / s); }
+template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> operator/(R s, const Dual<M, R>& b) { return dconst<M, R>(s) / b; }
+template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> dsqrt(const Dual<M, R>& a) {
   Dual<M, R> r; r.v = sqrt(a.v); R f = R(0.5) / r.v;
 #pragma unroll
   for (int i = 0; i < M; i++) r.d[i] = a.d[i] * f;
   return r;
 }
-template <int M, typename R> __device__ __forceinline__ Dual<M, R> dabs(const Dual<M, R>& a) { return a.v < R(0) ? -a : a; }
-template <int M, typename R> __device__ __forceinline__ R value(const Dual<M, R>& a) { return a.v; }
+template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> dabs(const Dual<M, R>& a) { return a.v < R(0) ? -a : a; }
+template <int M, typename R> __host__ __device__ __forceinline__ R value(const Dual<M, R>& a) { return a.v; }
 
-__device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
-__device__ __forceinline__ float dsqrt(float a) { return sqrtf(a); }
-__device__ __forceinline__ double dabs(double a) { return fabs(a); }
-__device__ __forceinline__ float dabs(float a) { return fabsf(a); }
-__device__ __forceinline__ double value(double a) { return a; }
-__device__ __forceinline__ float value(float a) { return a; }
+__host__ __device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
+__host__ __device__ __forceinline__ float dsqrt(float a) { return sqrtf(a); }
+__host__ __device__ __forceinline__ double dabs(double a) { return fabs(a); }
+__host__ __device__ __forceinline__ float dabs(float a) { return fabsf(a); }
+__host__ __device__ __forceinline__ double value(double a) { return a; }
+__host__ __device__ __forceinline__ float value(float a) { return a; }
 
 template <class T> struct Scalar { typedef T type; };
 template <int M, typename R> struct Scalar<Dual<M, R>> { typedef R type; };
 
-template <class T> struct Make { static __device__ __forceinline__ T c(typename Scalar<T>::type x) { return T(x); } };
-template <int M, typename R> struct Make<Dual<M, R>> { static __device__ __forceinline__ Dual<M, R> c(R x) { return dconst<M, R>(x); } };
-template <class T> __device__ __forceinline__ T tconst(typename Scalar<T>::type x) { return Make<T>::c(x); }
+template <class T> struct Make { static __host__ __device__ __forceinline__ T c(typename Scalar<T>::type x) { return T(x); } };
+template <int M, typename R> struct Make<Dual<M, R>> { static __host__ __device__ __forceinline__ Dual<M, R> c(R x) { return dconst<M, R>(x); } };
+template <class T> __host__ __device__ __forceinline__ T tconst(typename Scalar<T>::type x) { return Make<T>::c(x); }
 
 // Math.Max(a, b) keeps a on ties (b wins only when strictly greater).
-template <class T> __device__ __forceinline__ T vmax(const T& a, const T& b) { return value(b) > value(a) ? b : a; }
+template <class T> __host__ __device__ __forceinline__ T vmax(const T& a, const T& b) { return value(b) > value(a) ? b : a; }
 
 
 // ------------------------------------------------------------------------- SIP
@@ -105,13 +105,13 @@ struct FormSIP {
   static constexpr bool HAS_BOUNDARY = true;
 
   template <int D, class T, class R>
-  __device__ static void volume(const FormParams& P, const T* U, const T (*GU)[D], T (*F)[D]) {
+  __host__ __device__ static void volume(const FormParams& P, const T* U, const T (*GU)[D], T (*F)[D]) {
     (void)U;
 #pragma unroll
     for (int d = 0; d < D; d++) F[0][d] = GU[0][d] * R(-P.nu);   // -Nu * grad(u) . grad(v)
   }
   template <int D, class T, class R>
-  __device__ static void inner(const FormParams& P, const R* n, R mu, const T* Uo, const T* Un, const T (*Go)[D],
+  __host__ __device__ static void inner(const FormParams& P, const R* n, R mu, const T* Uo, const T* Un, const T (*Go)[D],
                                const T (*Gn)[D], T* fo, T (*go)[D], T* fn, T (*gn)[D]) {
     const R nu = R(P.nu);
     T jump = Uo[0] - Un[0];
@@ -128,7 +128,7 @@ struct FormSIP {
     }
   }
   template <int D, class T, class R>
-  __device__ static void boundary(const FormParams& P, const R* n, R mu, const T* Uo, const T (*Go)[D], T* fo, T (*go)[D]) {
+  __host__ __device__ static void boundary(const FormParams& P, const R* n, R mu, const T* Uo, const T (*Go)[D], T* fo, T (*go)[D]) {
     const R nu = R(P.nu);
     T flux = Go[0][0] * (nu * n[0]);
 #pragma unroll
@@ -149,12 +149,12 @@ struct FormAdvDiff {
   static constexpr bool HAS_BOUNDARY = true;
 
   template <int D, class T, class R>
-  __device__ static void volume(const FormParams& P, const T* U, const T (*GU)[D], T (*F)[D]) {
+  __host__ __device__ static void volume(const FormParams& P, const T* U, const T (*GU)[D], T (*F)[D]) {
 #pragma unroll
     for (int d = 0; d < D; d++) F[0][d] = U[0] * R(-P.vel[d]) + GU[0][d] * R(-P.nu);
   }
   template <int D, class T, class R>
-  __device__ static void inner(const FormParams& P, const R* n, R mu, const T* Uo, const T* Un, const T (*Go)[D],
+  __host__ __device__ static void inner(const FormParams& P, const R* n, R mu, const T* Uo, const T* Un, const T (*Go)[D],
                                const T (*Gn)[D], T* fo, T (*go)[D], T* fn, T (*gn)[D]) {
     FormSIP::inner<D, T, R>(P, n, mu, Uo, Un, Go, Gn, fo, go, fn, gn);
     R un = R(P.vel[0]) * n[0];
@@ -165,7 +165,7 @@ struct FormAdvDiff {
     fn[0] = fn[0] - up;
   }
   template <int D, class T, class R>
-  __device__ static void boundary(const FormParams& P, const R* n, R mu, const T* Uo, const T (*Go)[D], T* fo, T (*go)[D]) {
+  __host__ __device__ static void boundary(const FormParams& P, const R* n, R mu, const T* Uo, const T (*Go)[D], T* fo, T (*go)[D]) {
     FormSIP::boundary<D, T, R>(P, n, mu, Uo, Go, fo, go);
     R un = R(P.vel[0]) * n[0];
 #pragma unroll
@@ -179,7 +179,7 @@ struct FormAdvDiff {
 template <int D, class T, class R>
 struct EulerState {
   T rho, m[D], E, u[D], p, a;
-  __device__ __forceinline__ EulerState(const T* U, const FormParams& P) {
+  __host__ __device__ __forceinline__ EulerState(const T* U, const FormParams& P) {
     rho = U[0];
     T irho = R(1) / rho;
 #pragma unroll
@@ -192,14 +192,14 @@ struct EulerState {
     p = (E - msq * irho * (R(0.5) * ms)) * R(P.gamma - 1.0);
     a = dsqrt(p * irho) * R(1.0 / P.mach);
   }
-  __device__ __forceinline__ T un(const R* n) const {
+  __host__ __device__ __forceinline__ T un(const R* n) const {
     T r = u[0] * n[0];
 #pragma unroll
     for (int d = 1; d < D; d++) r = r + u[d] * n[d];
     return r;
   }
   // F(U).n for all V = D+2 components
-  __device__ __forceinline__ void flux_n(const R* n, const T& unn, R ims, T* f) const {
+  __host__ __device__ __forceinline__ void flux_n(const R* n, const T& unn, R ims, T* f) const {
     f[0] = rho * unn;
 #pragma unroll
     for (int d = 0; d < D; d++) f[1 + d] = m[d] * unn + p * (ims * n[d]);
@@ -216,7 +216,7 @@ struct FormEuler {
   static constexpr bool HAS_BOUNDARY = false;
 
   template <int DD, class T, class R>
-  __device__ static void volume(const FormParams& P, const T* U, const T (*GU)[DD], T (*F)[DD]) {
+  __host__ __device__ static void volume(const FormParams& P, const T* U, const T (*GU)[DD], T (*F)[DD]) {
     (void)GU;
     EulerState<D, T, R> s(U, P);
     const R ims = R(1.0 / (P.gamma * P.mach * P.mach));
@@ -235,7 +235,7 @@ struct FormEuler {
   }
 
   template <int DD, class T, class R>
-  __device__ static void inner(const FormParams& P, const R* n, R mu, const T* Uo, const T* Un, const T (*Go)[DD],
+  __host__ __device__ static void inner(const FormParams& P, const R* n, R mu, const T* Uo, const T* Un, const T (*Go)[DD],
                                const T (*Gn)[DD], T* fo, T (*go)[DD], T* fn, T (*gn)[DD]) {
     (void)mu; (void)Go; (void)Gn; (void)go; (void)gn;
     EulerState<D, T, R> si(Uo, P), so(Un, P);
@@ -295,7 +295,7 @@ struct FormEuler {
     for (int c = 0; c < V; c++) fn[c] = -fo[c];
   }
   template <int DD, class T, class R>
-  __device__ static void boundary(const FormParams&, const R*, R, const T*, const T (*)[DD], T* fo, T (*)[DD]) {
+  __host__ __device__ static void boundary(const FormParams&, const R*, R, const T*, const T (*)[DD], T* fo, T (*)[DD]) {
 #pragma unroll
     for (int c = 0; c < V; c++) fo[c] = fo[c] * R(0);
   }

@@ -28,6 +28,8 @@ int KernelSet::prepare() const {
   for (int i = 0; i < NVARIANT; i++) {
     if (k64[i] && hipFuncSetAttribute((const void*)k64[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds64) != hipSuccess) return 1;
     if (k32[i] && hipFuncSetAttribute((const void*)k32[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds32) != hipSuccess) return 2;
+    if (kt64[i] && hipFuncSetAttribute((const void*)kt64[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)ktlds64[i]) != hipSuccess) return 3;
+    if (kt32[i] && hipFuncSetAttribute((const void*)kt32[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)ktlds32[i]) != hipSuccess) return 4;
   }
   return 0;
 }

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dg_kernels.hpp"
+#include "dg_tensor.hpp"
 
 namespace bosss {
 
@@ -16,8 +17,19 @@ struct KernelSet {
   size_t lds64, lds32;
   void (*k64[NVARIANT])(KArgs<double>);
   void (*k32[NVARIANT])(KArgs<float>);
+  // tensor-table kernels (dg_tensor.hpp): 1D tables in LDS, own coefficients staged in LDS
+  int P, ttab_size, t_o_l, t_o_dl, t_o_w, t_o_code, t_k2;
+  void (*kt64[NVARIANT])(TArgs<double>);
+  void (*kt32[NVARIANT])(TArgs<float>);
+  size_t ktlds64[NVARIANT], ktlds32[NVARIANT];
   int prepare() const;
 };
+
+constexpr int degree_from_modes(int D, int N) {
+  int p = 0;
+  while (tdeg_count(D, p) < N) p++;
+  return p;
+}
 
 const KernelSet* find_kernels(int kind, int D, int N, int K1, int precision);
 
@@ -35,8 +47,29 @@ KernelSet make_kernel_set(int kind) {
   k.k64[KernelSet::JVP] = k_forward<Form, D, N, K1, double, MODE_JVP>;
   k.k64[KernelSet::VJP] = k_adjoint<Form, D, N, K1, double, false>;
   k.k64[KernelSet::RES_VJP] = k_adjoint<Form, D, N, K1, double, true>;
-  for (int i = 0; i < KernelSet::NVARIANT; i++) k.k32[i] = nullptr;
+  for (int i = 0; i < KernelSet::NVARIANT; i++) { k.k32[i] = nullptr; k.kt32[i] = nullptr; k.ktlds32[i] = 0; }
+  constexpr int P = degree_from_modes(D, N);
+  static_assert(tdeg_count(D, P) == N, "N must be a full total-degree basis");
+  typedef TShape<D, P, K1> TSh;
+  k.P = P; k.ttab_size = TSh::TAB; k.t_o_l = TSh::O_L; k.t_o_dl = TSh::O_DL; k.t_o_w = TSh::O_W; k.t_o_code = TSh::O_CODE;
+  k.t_k2 = TSh::K2;
+  k.kt64[KernelSet::RES] = kt_forward<Form, D, P, K1, double, MODE_RES>;
+  k.kt64[KernelSet::JVP] = kt_forward<Form, D, P, K1, double, MODE_JVP>;
+  k.kt64[KernelSet::VJP] = kt_adjoint<Form, D, P, K1, double, false>;
+  k.kt64[KernelSet::RES_VJP] = kt_adjoint<Form, D, P, K1, double, true>;
+  k.ktlds64[KernelSet::RES] = kt_lds_bytes<D, P, K1, Form::V, double>(1);
+  k.ktlds64[KernelSet::JVP] = kt_lds_bytes<D, P, K1, Form::V, double>(2);
+  k.ktlds64[KernelSet::VJP] = kt_lds_bytes<D, P, K1, Form::V, double>(2);
+  k.ktlds64[KernelSet::RES_VJP] = kt_lds_bytes<D, P, K1, Form::V, double>(2);
   if constexpr (WITH32) {
+    k.kt32[KernelSet::RES] = kt_forward<Form, D, P, K1, float, MODE_RES>;
+    k.kt32[KernelSet::JVP] = kt_forward<Form, D, P, K1, float, MODE_JVP>;
+    k.kt32[KernelSet::VJP] = kt_adjoint<Form, D, P, K1, float, false>;
+    k.kt32[KernelSet::RES_VJP] = kt_adjoint<Form, D, P, K1, float, true>;
+    k.ktlds32[KernelSet::RES] = kt_lds_bytes<D, P, K1, Form::V, float>(1);
+    k.ktlds32[KernelSet::JVP] = kt_lds_bytes<D, P, K1, Form::V, float>(2);
+    k.ktlds32[KernelSet::VJP] = kt_lds_bytes<D, P, K1, Form::V, float>(2);
+    k.ktlds32[KernelSet::RES_VJP] = kt_lds_bytes<D, P, K1, Form::V, float>(2);
     k.k32[KernelSet::RES] = k_forward<Form, D, N, K1, float, MODE_RES>;
     k.k32[KernelSet::JVP] = k_forward<Form, D, N, K1, float, MODE_JVP>;
     k.k32[KernelSet::VJP] = k_adjoint<Form, D, N, K1, float, false>;

@@ -175,6 +175,52 @@ class DifferentialOperator:
                                                self._stream(stream)))
 
 
+    # ------------------------------------------------ operator matrix (block-ELL)
+    def matrix_layout(self):
+        """(blocks per row S, total entries, number of probe colours, cols[J][S])."""
+        S = ctypes.c_int32(); n = ctypes.c_int64(); nc = ctypes.c_int32()
+        check(lib().bosss_hip_op_matrix_layout(self.h, ctypes.byref(S), ctypes.byref(n), ctypes.byref(nc), None))
+        cols = np.zeros((self.grid.J, S.value), np.int32)
+        check(lib().bosss_hip_op_matrix_layout(self.h, None, None, None, cols.ctypes.data_as(ctypes.c_void_p)))
+        return S.value, n.value, nc.value, cols
+
+    def assemble_dev(self, U, blocks, alpha=1.0, beta=0.0, stream=None):
+        """blocks <- beta*blocks + alpha*dR/dU(U) (the operator matrix for linear forms)."""
+        self._check_dev(U)
+        check(lib().bosss_hip_op_assemble_dev(self.h, self._ptr(U), alpha, beta, self._ptr(blocks), self._stream(stream)))
+
+    def spmv_dev(self, blocks, x, y, alpha=1.0, beta=0.0, stream=None):
+        self._check_dev(x, y)
+        check(lib().bosss_hip_bell_spmv_dev(self.h, self._ptr(blocks), alpha, self._ptr(x), beta, self._ptr(y), self._stream(stream)))
+
+    def GetMatrixBuilder(self):
+        return EvaluatorLinear(self)
+
+
+class EvaluatorLinear:
+    """IEvaluatorLinear: ComputeMatrix into a device block-ELL matrix (EvaluatorLinear, DifferentialOperator.cs:1721-1838)."""
+
+    def __init__(self, op):
+        self.op = op
+        self.S, self.entries, self.ncolors, self.cols = op.matrix_layout()
+
+    def new_matrix(self):
+        import torch
+        return torch.zeros(self.entries, dtype=torch.float64, device="cuda:%d" % self.op.ctx.device)
+
+    def ComputeMatrix(self, blocks, alpha=1.0, U=None):
+        """Accumulate alpha * (operator matrix) into blocks; U is the linearisation point (zero for linear forms)."""
+        import torch
+        if U is None:
+            U = torch.zeros(self.op.ndof, dtype=torch.float64, device=blocks.device)
+        self.op.assemble_dev(U, blocks, alpha=alpha, beta=1.0)
+        return blocks
+
+    def ComputeAffine(self):
+        """Affine part b of R(U) = M U + b (R at U = 0)."""
+        return self.op.evaluate(np.zeros(self.op.ndof))
+
+
 class EvaluatorNonLin:
     """IEvaluatorNonLin: Evaluate(alpha, beta, output) with the domain state fixed at creation."""
 

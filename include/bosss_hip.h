@@ -118,6 +118,18 @@ int bosss_hip_vjp_dev(bosss_op* op, const double* U, const double* W, double* Jt
 int bosss_hip_evaluate_vjp_dev(bosss_op* op, double alpha, double beta, const double* U, double* R, const double* W,
                                double* JtW, void* stream);
 
+/* ---- operator matrix (replaces EvaluatorLinear.ComputeMatrix, DifferentialOperator.cs:1721-1838,
+ *      into a BlockMsrMatrix, BlockMsrMatrix.cs:186/1388, and the coloured
+ *      FDJacobianBuilder, DifferentialOperator.cs:1905-2880, with exact JVPs).
+ *      Layout "block-ELL": blocks[j][s] is the Nt x Nt (Nt = V*N) row-major block of
+ *      cell row j and column cell cols[j][s]; s = 0 is the diagonal, s = 1 + f the
+ *      neighbour across local face f (-1: no block).  entries = J*(1+2D)*Nt*Nt. ---- */
+int bosss_hip_op_matrix_layout(bosss_op* op, int32_t* blocks_per_row, int64_t* entries, int32_t* ncolors, int32_t* cols);
+/* blocks <- beta*blocks + alpha*dR/dU(U)  (device pointers; the Jacobian of a linear form is its matrix) */
+int bosss_hip_op_assemble_dev(bosss_op* op, const double* U, double alpha, double beta, double* blocks, void* stream);
+/* y <- alpha*A x + beta*y with A in the operator's block-ELL layout (BlockMsrMatrix.SpMV, BlockMsrMatrix.cs:2554) */
+int bosss_hip_bell_spmv_dev(bosss_op* op, const double* blocks, double alpha, const double* x, double beta, double* y, void* stream);
+
 /* ---- batched uneven cell-block factorisation (replaces BlockJacobi.InitImpl's
  *      ReadBlock -> InvertInPlace -> LAPACK DGETRF/DGETRI per block, BlockJacobi.cs:63-120,
  *      IMatrixExtensions.cs:2128-2230, and Schwarz's LAPACK block solver, Schwarz.cs:827-915).

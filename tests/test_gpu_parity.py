@@ -12,6 +12,16 @@ from oracle.bosss_ref import RefModel, params_vector, OP_SIP, OP_ADVDIFF, OP_EUL
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(params=["tensor", "generic"], autouse=True)
+def table_path(request, monkeypatch):
+    """Run every case through both kernel families: 1D tensor factors (default) and full LDS tables."""
+    if request.param == "generic":
+        monkeypatch.setenv("BOSSS_HIP_GENERIC_TABLES", "1")
+    else:
+        monkeypatch.delenv("BOSSS_HIP_GENERIC_TABLES", raising=False)
+    return request.param
+
 KIND = {"sip": OP_SIP, "advdiff": OP_ADVDIFF, "euler_lf": OP_EULER_LF, "euler_hllc": OP_EULER_HLLC}
 TOL = 1e-12
 
