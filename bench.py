@@ -65,6 +65,43 @@ def cpu_baseline(form, p, budget_s=15.0, n_max=64):
     }
 
 
+def kernel_label(form, p, split):
+    """Name of the dominant kernel of one step, as rocprofv3 lists it (template arguments shortened)."""
+    if split:
+        return "residual kernel + VJP kernel (%s, p=%d)" % (form, p)
+    if form.startswith("euler"):
+        return "kt_vjp_euler<FormEuler<3,%s>,D=3,P=%d,K1=%d,double,fused>" % (
+            "HLLC" if form == "euler_hllc" else "LF", p, (3 * p + 2) // 2)   # Gauss points of order 3p
+    return "kt_adjoint<%s,D=3,P=%d,double,fused>" % (form, p)
+
+
+def hbm_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` on the n^3 mesh, from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (profiles/hbm_traffic.json), or None if none was measured for it."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for r in rec.get("kernels", []):
+        if r.get("kernel") == kernel and r.get("mesh") == n:
+            return r.get("bytes_per_launch")
+    return None
+
+
+def copy_calibration(dev, nbytes=1 << 30):
+    """One device-to-device copy of nbytes (torch copy kernel): the known-size reference that
+    rocprofv3 FETCH_SIZE / WRITE_SIZE readings of the DG kernel are scaled by."""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
 def newton_comparison(grid, op64, Un, form, p):
     """One implicit Euler step solved to ||F|| <= 1e-10: FP64 Newton-GMRES vs FP64 defect
     correction around FP32 JVPs (same driver, bosss_amd.newton)."""
@@ -99,6 +136,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--split", action="store_true", help="time separate residual and VJP launches instead of the fused one")
     ap.add_argument("--no-newton", action="store_true", help="skip the implicit-step Newton comparison (FP64 vs mixed)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 1 GiB device copy first (known-size reference for HBM counter readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU sharding is a design, not a code path); "
@@ -123,6 +162,8 @@ def main():
     Rt = torch.zeros_like(Ut)
     Gt = torch.zeros_like(Ut)
     stream = torch.cuda.current_stream()
+    if args.copy_calibration:
+        copy_calibration(dev)
 
     def step():
         if args.split:
@@ -168,8 +209,9 @@ def main():
                                "fused residual + VJP per step" % (form, n, p, op.N, op.nvars, op.ndof),
                    "mesh": [n, n, n], "degree": p, "form": form, "dof": op.ndof},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                     "kernel": "kt_adjoint<FormEuler<3,HLLC>,D=3,P=3,K1=5,double,fused>" if form == "euler_hllc" else form,
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "traffic": hbm_traffic(kernel_label(form, p, args.split), n),
+                     "kernel": kernel_label(form, p, args.split),
                      "kernel_ms": round(kernel_ms, 4), "alg_flops_per_launch": flops},
     }
     print("bench: %s kernels, %s, %.3f ms/step, %.3f ms kernel" % (

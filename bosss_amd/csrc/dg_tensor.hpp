@@ -422,7 +422,7 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_adjoint(TArgs<Real> a) {
   typedef typename MM<Real>::acc acc_t;
   constexpr int V = Form::V, N = TS::N, NT = V * N, NF = TS::NF, MT = TS::MT;
   constexpr int STRIDE = NT | 1;
-  constexpr bool HAND = HandAdjoint<Form>::value;
+  static_assert(!HandAdjoint<Form>::value, "the Euler forms use kt_vjp_euler");
   constexpr bool GU_IN = Form::VOL_GRADU || Form::FACE_GRADU;
   constexpr int M = GU_IN ? V * (1 + D) : V;
   typedef Dual<M, Real> TD;
@@ -462,80 +462,6 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_adjoint(TArgs<Real> a) {
   auto Bw = [&](int v, int kk) { int m = 4 * kk + grp; return (m < N) ? ownW[v * N + m] : Real(0); };
 
   // ------------------------------------------------------------------ volume
-  if constexpr (HAND) {
-    // Hand-written adjoint: the cotangent of the volume flux is linear in grad(W), so the
-    // three reference-gradient directions are evaluated and pulled back one at a time
-    // (keeps one 4-node gradient array live instead of D).
-    for (int t = 0; t < TS::TV; t++) {
-      int qe[3];
-      const int nodeE = t * 16 + col;
-      vol_axes<D, K1>(nodeE < TS::KV ? nodeE : 0, qe);
-      acc_t u[V];
-      t_eval<TS, Real, V>(tab, qe, -1, grp, Bu, u);
-      Real ub[4][V];
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int v = 0; v < V; v++) ub[r][v] = Real(0);
-      if (FUSED) {
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int node = t * 16 + MM<Real>::row(grp, r);
-          const bool pad = node >= TS::KV;
-          int q[3];
-          vol_axes<D, K1>(pad ? 0 : node, q);
-          const Real wq = pad ? Real(0) : vol_weight<TS>(tab, q) * detJ * sc;
-          Real Uq[V], GUd[V][D], Fv[V][D];
-#pragma unroll
-          for (int v = 0; v < V; v++) { Uq[v] = u[v][r]; for (int d = 0; d < D; d++) GUd[v][d] = Uq[v]; }
-          Form::template volume<D, Real, Real>(a.fp, Uq, GUd, Fv);
-#pragma unroll
-          for (int e = 0; e < D; e++) {
-            Real g[V];
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-              Real s = 0;
-#pragma unroll
-              for (int d = 0; d < D; d++) s += Fv[v][d] * iJ[e][d];
-              g[v] = pad ? Real(0) : s * wq;
-            }
-            t_test<TS, Real, V>(tab, q, e, col, g, racc);
-          }
-        }
-      }
-      for (int e = 0; e < D; e++) {
-        acc_t wg[V];
-        t_eval<TS, Real, V>(tab, qe, e, grp, Bw, wg);
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int node = t * 16 + MM<Real>::row(grp, r);
-          const bool pad = node >= TS::KV;
-          int q[3];
-          vol_axes<D, K1>(pad ? 0 : node, q);
-          const Real wq = pad ? Real(0) : vol_weight<TS>(tab, q) * detJ * sc;
-          Real Uq[V], Fb[V][D];
-#pragma unroll
-          for (int v = 0; v < V; v++) {
-            Uq[v] = u[v][r];
-#pragma unroll
-            for (int d = 0; d < D; d++) Fb[v][d] = iJ[e][d] * wg[v][r] * wq;
-          }
-          EulerAdjoint<D, Real>::volume_vjp(a.fp, Uq, Fb, ub[r]);
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int node = t * 16 + MM<Real>::row(grp, r);
-        const bool pad = node >= TS::KV;
-        int q[3];
-        vol_axes<D, K1>(pad ? 0 : node, q);
-        Real bv[V];
-#pragma unroll
-        for (int v = 0; v < V; v++) bv[v] = pad ? Real(0) : ub[r][v] * sc;
-        t_test<TS, Real, V>(tab, q, -1, col, bv, aacc);
-      }
-    }
-  } else {
   for (int t = 0; t < TS::TV; t++) {
     int qe[3];
     const int nodeE = t * 16 + col;
@@ -567,18 +493,7 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_adjoint(TArgs<Real> a) {
         }
       Real ub[V], gb[V][D];
       Real Fv[V][D];
-      if constexpr (HAND) {
-        Real Uq[V];
-#pragma unroll
-        for (int v = 0; v < V; v++) { Uq[v] = u[v][r]; ub[v] = 0; }
-        EulerAdjoint<D, Real>::volume_vjp(a.fp, Uq, Fb, ub);
-        if (FUSED) {
-          Real GUd[V][D];
-#pragma unroll
-          for (int v = 0; v < V; v++) for (int d = 0; d < D; d++) GUd[v][d] = Uq[v];
-          Form::template volume<D, Real, Real>(a.fp, Uq, GUd, Fv);
-        }
-      } else {
+      {
         TD Uq[V], GU[V][D], F[V][D];
 #pragma unroll
         for (int v = 0; v < V; v++) {
@@ -633,24 +548,20 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_adjoint(TArgs<Real> a) {
 #pragma unroll
       for (int v = 0; v < V; v++) bv[v] = pad ? Real(0) : ub[v] * sc;
       t_test<TS, Real, V>(tab, q, -1, col, bv, aacc);
-      if constexpr (!HAND) {
-        if (Form::VOL_GRADU) {
+      if (Form::VOL_GRADU) {
 #pragma unroll
-          for (int e = 0; e < D; e++) {
+        for (int e = 0; e < D; e++) {
 #pragma unroll
-            for (int v = 0; v < V; v++) {
-              Real s = 0;
+          for (int v = 0; v < V; v++) {
+            Real s = 0;
 #pragma unroll
-              for (int d = 0; d < D; d++) s += gb[v][d] * iJ[e][d];
-              bv[v] = pad ? Real(0) : s * sc;
-            }
-            t_test<TS, Real, V>(tab, q, e, col, bv, aacc);
+            for (int d = 0; d < D; d++) s += gb[v][d] * iJ[e][d];
+            bv[v] = pad ? Real(0) : s * sc;
           }
+          t_test<TS, Real, V>(tab, q, e, col, bv, aacc);
         }
       }
     }
-  }
-
   }
 
   // ------------------------------------------------------------------- faces
@@ -672,7 +583,7 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_adjoint(TArgs<Real> a) {
 #pragma unroll
     for (int e = 0; e < D; e++)
 #pragma unroll
-      for (int d = 0; d < D; d++) iJn[e][d] = HAND ? Real(0) : Real(a.invJ[(size_t)cn * D * D + e * D + d]);
+      for (int d = 0; d < D; d++) iJn[e][d] = Real(a.invJ[(size_t)cn * D * D + e * D + d]);
     const Real* nbU = a.U + (size_t)cn * NT;
     const Real* nbW = a.X + (size_t)cn * NT;
     auto Bnu = [&](int v, int kk) { int m = 4 * kk + grp; return (has_nb && m < N) ? nbU[v * N + m] * scn : Real(0); };
@@ -686,172 +597,128 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_adjoint(TArgs<Real> a) {
       acc_t uo[V], un[V];
       t_eval<TS, Real, V>(tab, qo, -1, grp, Bu, uo);
       t_eval<TS, Real, V>(tab, qn, -1, grp, Bnu, un);
-      if constexpr (HAND) {
-        // merged face cotangent  gbar = sc W_own.trace - scn W_nb.trace  (fn = -fo)
-        acc_t gw[V];
+      acc_t guo[D][V], gun[D][V], wo[V], wn[V], gwo[D][V], gwn[D][V];
+      t_eval<TS, Real, V>(tab, qo, -1, grp, Bw, wo);
+      t_eval<TS, Real, V>(tab, qn, -1, grp, Bnw, wn);
 #pragma unroll
-        for (int v = 0; v < V; v++) gw[v] = acc_zero<Real>();
+      for (int e = 0; e < D; e++) {
+        if (Form::FACE_GRADU) {
+          t_eval<TS, Real, V>(tab, qo, e, grp, Bu, guo[e]);
+          t_eval<TS, Real, V>(tab, qn, e, grp, Bnu, gun[e]);
+        }
+        if (Form::FACE_TESTG) {
+          t_eval<TS, Real, V>(tab, qo, e, grp, Bw, gwo[e]);
+          t_eval<TS, Real, V>(tab, qn, e, grp, Bnw, gwn[e]);
+        }
+      }
 #pragma unroll
-        for (int kk = 0; kk < TS::NK; kk++) {
-          int code = tcode<TS>(tab, 4 * kk + grp);
-          Real ao = (code >= 0) ? tval<TS>(tab, qo, code, -1) : Real(0);
-          Real an = (code >= 0) ? tval<TS>(tab, qn, code, -1) : Real(0);
+      for (int r = 0; r < 4; r++) {
+        const int node = t * 16 + MM<Real>::row(grp, r);
+        const bool pad = node >= TS::KE;
+        int q[3];
+        face_axes<D, K1>(f, pad ? 0 : node, q);
+        const Real wsg = pad ? Real(0) : face_weight<TS>(tab, node) * sg;
+        TD Uo[V], Un[V], Go[V][D], Gn[V][D];
 #pragma unroll
-          for (int v = 0; v < V; v++) {
-            gw[v] = MM<Real>::mma(ao, Bw(v, kk) * sc, gw[v]);
-            gw[v] = MM<Real>::mma(an, -Bnw(v, kk) * scn, gw[v]);
+        for (int v = 0; v < V; v++) {
+          Uo[v] = dconst<M, Real>(uo[v][r]);
+          Uo[v].d[v] = Real(1);
+          Un[v] = has_nb ? dconst<M, Real>(un[v][r]) : Uo[v];
+#pragma unroll
+          for (int d = 0; d < D; d++) {
+            if (Form::FACE_GRADU) {
+              Real go_ = 0, gn_ = 0;
+#pragma unroll
+              for (int e = 0; e < D; e++) { go_ += guo[e][v][r] * iJ[e][d]; gn_ += gun[e][v][r] * iJn[e][d]; }
+              Go[v][d] = dconst<M, Real>(go_);
+              if (GU_IN) Go[v][d].d[(V + v * D + d) % M] = Real(1);
+              Gn[v][d] = has_nb ? dconst<M, Real>(gn_) : Go[v][d];
+            } else {
+              Go[v][d] = Uo[v]; Gn[v][d] = Un[v];
+            }
           }
         }
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int node = t * 16 + MM<Real>::row(grp, r);
-          const bool pad = node >= TS::KE;
-          int q[3];
-          face_axes<D, K1>(f, pad ? 0 : node, q);
-          const Real wsg = pad ? Real(0) : face_weight<TS>(tab, node) * sg;
-          Real Uo[V], Un[V], gbar[V], fo_[V], ub[V];
+        TD fo_[V], go_[V][D], fn_[V], gn_[V][D];
+        if (has_nb) {
+          Form::template inner<D, TD, Real>(a.fp, nrm, mu, Uo, Un, Go, Gn, fo_, go_, fn_, gn_);
+        } else {
 #pragma unroll
           for (int v = 0; v < V; v++) {
-            Uo[v] = uo[v][r];
-            Un[v] = has_nb ? un[v][r] : uo[v][r];
-            gbar[v] = gw[v][r] * wsg;
-            ub[v] = 0;
+            fo_[v] = Uo[v]; fn_[v] = dconst<M, Real>(Real(0));
+            for (int d = 0; d < D; d++) { go_[v][d] = Uo[v]; gn_[v][d] = dconst<M, Real>(Real(0)); }
           }
-          EulerAdjoint<D, Real>::template inner_vjp<IsHLLC<Form>::value>(a.fp, nrm, Uo, Un, gbar, fo_, ub);
-          Real g[V];
-          if (FUSED) {
-#pragma unroll
-            for (int v = 0; v < V; v++) g[v] = pad ? Real(0) : fo_[v] * wsg * sc;
-            t_test<TS, Real, V>(tab, q, -1, col, g, racc);
-          }
-#pragma unroll
-          for (int v = 0; v < V; v++) g[v] = pad ? Real(0) : ub[v] * sc;
-          t_test<TS, Real, V>(tab, q, -1, col, g, aacc);
+          Form::template boundary<D, TD, Real>(a.fp, nrm, mu, Uo, Go, fo_, go_);
         }
-      } else {
-        acc_t guo[D][V], gun[D][V], wo[V], wn[V], gwo[D][V], gwn[D][V];
-        t_eval<TS, Real, V>(tab, qo, -1, grp, Bw, wo);
-        t_eval<TS, Real, V>(tab, qn, -1, grp, Bnw, wn);
+        Real ub[V], gb[V][D];
 #pragma unroll
-        for (int e = 0; e < D; e++) {
-          if (Form::FACE_GRADU) {
-            t_eval<TS, Real, V>(tab, qo, e, grp, Bu, guo[e]);
-            t_eval<TS, Real, V>(tab, qn, e, grp, Bnu, gun[e]);
+        for (int qq = 0; qq < V; qq++) { ub[qq] = 0; for (int d = 0; d < D; d++) gb[qq][d] = 0; }
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          Real fob = wsg * sc * wo[v][r];
+          Real fnb = has_nb ? wsg * scn * wn[v][r] : Real(0);
+#pragma unroll
+          for (int qq = 0; qq < V; qq++) {
+            ub[qq] += fob * fo_[v].d[qq] + fnb * fn_[v].d[qq];
+            if (GU_IN && Form::FACE_GRADU) {
+#pragma unroll
+              for (int dd = 0; dd < D; dd++)
+                gb[qq][dd] += fob * fo_[v].d[(V + qq * D + dd) % M] + fnb * fn_[v].d[(V + qq * D + dd) % M];
+            }
           }
           if (Form::FACE_TESTG) {
-            t_eval<TS, Real, V>(tab, qo, e, grp, Bw, gwo[e]);
-            t_eval<TS, Real, V>(tab, qn, e, grp, Bnw, gwn[e]);
-          }
-        }
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int node = t * 16 + MM<Real>::row(grp, r);
-          const bool pad = node >= TS::KE;
-          int q[3];
-          face_axes<D, K1>(f, pad ? 0 : node, q);
-          const Real wsg = pad ? Real(0) : face_weight<TS>(tab, node) * sg;
-          TD Uo[V], Un[V], Go[V][D], Gn[V][D];
-#pragma unroll
-          for (int v = 0; v < V; v++) {
-            Uo[v] = dconst<M, Real>(uo[v][r]);
-            Uo[v].d[v] = Real(1);
-            Un[v] = has_nb ? dconst<M, Real>(un[v][r]) : Uo[v];
 #pragma unroll
             for (int d = 0; d < D; d++) {
-              if (Form::FACE_GRADU) {
-                Real go_ = 0, gn_ = 0;
+              Real gob = 0, gnb = 0;
 #pragma unroll
-                for (int e = 0; e < D; e++) { go_ += guo[e][v][r] * iJ[e][d]; gn_ += gun[e][v][r] * iJn[e][d]; }
-                Go[v][d] = dconst<M, Real>(go_);
-                if (GU_IN) Go[v][d].d[(V + v * D + d) % M] = Real(1);
-                Gn[v][d] = has_nb ? dconst<M, Real>(gn_) : Go[v][d];
-              } else {
-                Go[v][d] = Uo[v]; Gn[v][d] = Un[v];
-              }
-            }
-          }
-          TD fo_[V], go_[V][D], fn_[V], gn_[V][D];
-          if (has_nb) {
-            Form::template inner<D, TD, Real>(a.fp, nrm, mu, Uo, Un, Go, Gn, fo_, go_, fn_, gn_);
-          } else {
+              for (int e = 0; e < D; e++) { gob += iJ[e][d] * gwo[e][v][r]; gnb += iJn[e][d] * gwn[e][v][r]; }
+              gob *= wsg * sc;
+              gnb = has_nb ? gnb * wsg * scn : Real(0);
 #pragma unroll
-            for (int v = 0; v < V; v++) {
-              fo_[v] = Uo[v]; fn_[v] = dconst<M, Real>(Real(0));
-              for (int d = 0; d < D; d++) { go_[v][d] = Uo[v]; gn_[v][d] = dconst<M, Real>(Real(0)); }
-            }
-            Form::template boundary<D, TD, Real>(a.fp, nrm, mu, Uo, Go, fo_, go_);
-          }
-          Real ub[V], gb[V][D];
+              for (int qq = 0; qq < V; qq++) {
+                ub[qq] += gob * go_[v][d].d[qq] + gnb * gn_[v][d].d[qq];
+                if (GU_IN && Form::FACE_GRADU) {
 #pragma unroll
-          for (int qq = 0; qq < V; qq++) { ub[qq] = 0; for (int d = 0; d < D; d++) gb[qq][d] = 0; }
-#pragma unroll
-          for (int v = 0; v < V; v++) {
-            Real fob = wsg * sc * wo[v][r];
-            Real fnb = has_nb ? wsg * scn * wn[v][r] : Real(0);
-#pragma unroll
-            for (int qq = 0; qq < V; qq++) {
-              ub[qq] += fob * fo_[v].d[qq] + fnb * fn_[v].d[qq];
-              if (GU_IN && Form::FACE_GRADU) {
-#pragma unroll
-                for (int dd = 0; dd < D; dd++)
-                  gb[qq][dd] += fob * fo_[v].d[(V + qq * D + dd) % M] + fnb * fn_[v].d[(V + qq * D + dd) % M];
-              }
-            }
-            if (Form::FACE_TESTG) {
-#pragma unroll
-              for (int d = 0; d < D; d++) {
-                Real gob = 0, gnb = 0;
-#pragma unroll
-                for (int e = 0; e < D; e++) { gob += iJ[e][d] * gwo[e][v][r]; gnb += iJn[e][d] * gwn[e][v][r]; }
-                gob *= wsg * sc;
-                gnb = has_nb ? gnb * wsg * scn : Real(0);
-#pragma unroll
-                for (int qq = 0; qq < V; qq++) {
-                  ub[qq] += gob * go_[v][d].d[qq] + gnb * gn_[v][d].d[qq];
-                  if (GU_IN && Form::FACE_GRADU) {
-#pragma unroll
-                    for (int dd = 0; dd < D; dd++)
-                      gb[qq][dd] += gob * go_[v][d].d[(V + qq * D + dd) % M] + gnb * gn_[v][d].d[(V + qq * D + dd) % M];
-                  }
+                  for (int dd = 0; dd < D; dd++)
+                    gb[qq][dd] += gob * go_[v][d].d[(V + qq * D + dd) % M] + gnb * gn_[v][d].d[(V + qq * D + dd) % M];
                 }
               }
             }
           }
-          Real g[V];
-          if (FUSED) {
-            const Real wq = wsg * sc;
+        }
+        Real g[V];
+        if (FUSED) {
+          const Real wq = wsg * sc;
 #pragma unroll
-            for (int v = 0; v < V; v++) g[v] = pad ? Real(0) : fo_[v].v * wq;
-            t_test<TS, Real, V>(tab, q, -1, col, g, racc);
-            if (Form::FACE_TESTG) {
-#pragma unroll
-              for (int e = 0; e < D; e++) {
-#pragma unroll
-                for (int v = 0; v < V; v++) {
-                  Real s = 0;
-#pragma unroll
-                  for (int d = 0; d < D; d++) s += go_[v][d].v * iJ[e][d];
-                  g[v] = pad ? Real(0) : s * wq;
-                }
-                t_test<TS, Real, V>(tab, q, e, col, g, racc);
-              }
-            }
-          }
-#pragma unroll
-          for (int v = 0; v < V; v++) g[v] = pad ? Real(0) : ub[v] * sc;
-          t_test<TS, Real, V>(tab, q, -1, col, g, aacc);
-          if (Form::FACE_GRADU) {
+          for (int v = 0; v < V; v++) g[v] = pad ? Real(0) : fo_[v].v * wq;
+          t_test<TS, Real, V>(tab, q, -1, col, g, racc);
+          if (Form::FACE_TESTG) {
 #pragma unroll
             for (int e = 0; e < D; e++) {
 #pragma unroll
               for (int v = 0; v < V; v++) {
                 Real s = 0;
 #pragma unroll
-                for (int d = 0; d < D; d++) s += gb[v][d] * iJ[e][d];
-                g[v] = pad ? Real(0) : s * sc;
+                for (int d = 0; d < D; d++) s += go_[v][d].v * iJ[e][d];
+                g[v] = pad ? Real(0) : s * wq;
               }
-              t_test<TS, Real, V>(tab, q, e, col, g, aacc);
+              t_test<TS, Real, V>(tab, q, e, col, g, racc);
             }
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < V; v++) g[v] = pad ? Real(0) : ub[v] * sc;
+        t_test<TS, Real, V>(tab, q, -1, col, g, aacc);
+        if (Form::FACE_GRADU) {
+#pragma unroll
+          for (int e = 0; e < D; e++) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+              Real s = 0;
+#pragma unroll
+              for (int d = 0; d < D; d++) s += gb[v][d] * iJ[e][d];
+              g[v] = pad ? Real(0) : s * sc;
+            }
+            t_test<TS, Real, V>(tab, q, e, col, g, aacc);
           }
         }
       }
@@ -885,6 +752,420 @@ template <int D, int P, int K1, int V, typename Real>
 constexpr size_t kt_lds_bytes(int narrays) {
   typedef TShape<D, P, K1> TS;
   return (size_t)(((TS::TAB + 1) & ~1) + TWAVES * narrays * 16 * ((V * TS::N) | 1)) * sizeof(Real);
+}
+
+}  // namespace bosss
+
+namespace bosss {
+
+// ----------------------------------------------------------------------------
+// Euler VJP with an LDS node buffer (tensor tables).
+//
+// Same contractions and pointwise adjoints as kt_adjoint's hand-written path, but
+// the 4-nodes-per-lane MFMA outputs (states and cotangents of a 16-node tile) are
+// parked in a per-lane slice of LDS right after each evaluation, and the nodes are
+// then processed one at a time from there.  LDS is private per lane here (slot-major,
+// lane-minor: conflict-free, no barriers), so it acts as an explicitly managed
+// spill area at LDS speed; the own cell's coefficients are read just in time from
+// global memory (L1/L2) instead of being staged.  This keeps the FP64 Euler VJP
+// inside the register file.
+// ----------------------------------------------------------------------------
+template <class Form, int D, int P, int K1, typename Real, bool FUSED>
+__global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
+  typedef TShape<D, P, K1> TS;
+  typedef typename MM<Real>::acc acc_t;
+  constexpr int V = Form::V, N = TS::N, NT = V * N, NF = TS::NF, MT = TS::MT;
+  constexpr int NSLOT = 3 * V * 4;
+
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Real* tab = reinterpret_cast<Real*>(smem_raw);
+  for (int i = threadIdx.x; i < TS::TAB; i += blockDim.x) tab[i] = a.tab[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, grp = lane >> 4;
+  Real* buf = tab + ((TS::TAB + 1) & ~1) + wave * (NSLOT * 64);
+  auto slot = [&](int arr, int v, int r) -> Real& { return buf[((arr * V + v) * 4 + r) * 64 + lane]; };
+
+  const int cell = (blockIdx.x * TWAVES + wave) * 16 + col;
+  const bool valid = cell < a.J;
+  const int cj = valid ? cell : 0;
+  const Real sc = Real(a.scaling[cj]);
+  const Real detJ = Real(a.detJ[cj]);
+  Real iJ[D][D];
+#pragma unroll
+  for (int e = 0; e < D; e++)
+#pragma unroll
+    for (int d = 0; d < D; d++) iJ[e][d] = Real(a.invJ[(size_t)cj * D * D + e * D + d]);
+
+  acc_t racc[V][MT], aacc[V][MT];
+#pragma unroll
+  for (int v = 0; v < V; v++)
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) { racc[v][mt] = acc_zero<Real>(); aacc[v][mt] = acc_zero<Real>(); }
+
+  const Real* ownU = a.U + (size_t)cj * NT;
+  const Real* ownW = a.X + (size_t)cj * NT;
+  auto Bu = [&](int v, int kk) { int m = 4 * kk + grp; return (valid && m < N) ? ownU[v * N + m] * sc : Real(0); };
+  auto Bw = [&](int v, int kk) { int m = 4 * kk + grp; return (valid && m < N) ? ownW[v * N + m] : Real(0); };
+  auto park = [&](int arr, const acc_t (&acc)[V]) {
+#pragma unroll
+    for (int v = 0; v < V; v++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) slot(arr, v, r) = acc[v][r];
+  };
+
+  // ------------------------------------------------------------------ volume
+  for (int t = 0; t < TS::TV; t++) {
+    int qe[3];
+    const int nodeE = t * 16 + col;
+    vol_axes<D, K1>(nodeE < TS::KV ? nodeE : 0, qe);
+    {
+      acc_t acc[V];
+      t_eval<TS, Real, V>(tab, qe, -1, grp, Bu, acc);
+      park(0, acc);
+    }
+    if (FUSED) {
+      for (int r = 0; r < 4; r++) {
+        const int node = t * 16 + MM<Real>::row(grp, r);
+        const bool pad = node >= TS::KV;
+        int q[3];
+        vol_axes<D, K1>(pad ? 0 : node, q);
+        const Real wq = pad ? Real(0) : vol_weight<TS>(tab, q) * detJ * sc;
+        Real Uq[V], GUd[V][D], Fv[V][D];
+#pragma unroll
+        for (int v = 0; v < V; v++) { Uq[v] = slot(0, v, r); for (int d = 0; d < D; d++) GUd[v][d] = Uq[v]; }
+        Form::template volume<D, Real, Real>(a.fp, Uq, GUd, Fv);
+#pragma unroll
+        for (int e = 0; e < D; e++) {
+          Real g[V];
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            Real s = 0;
+#pragma unroll
+            for (int d = 0; d < D; d++) s += Fv[v][d] * iJ[e][d];
+            g[v] = pad ? Real(0) : s * wq;
+          }
+          t_test<TS, Real, V>(tab, q, e, col, g, racc);
+        }
+      }
+    }
+    // Ubar accumulated per node in the buffer's third array
+#pragma unroll
+    for (int v = 0; v < V; v++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) slot(2, v, r) = Real(0);
+    for (int e = 0; e < D; e++) {
+      {
+        acc_t acc[V];
+        t_eval<TS, Real, V>(tab, qe, e, grp, Bw, acc);
+        park(1, acc);
+      }
+      for (int r = 0; r < 4; r++) {
+        const int node = t * 16 + MM<Real>::row(grp, r);
+        const bool pad = node >= TS::KV;
+        int q[3];
+        vol_axes<D, K1>(pad ? 0 : node, q);
+        const Real wq = pad ? Real(0) : vol_weight<TS>(tab, q) * detJ * sc;
+        Real Uq[V], Fb[V][D], ub[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          Uq[v] = slot(0, v, r);
+          ub[v] = slot(2, v, r);
+#pragma unroll
+          for (int d = 0; d < D; d++) Fb[v][d] = iJ[e][d] * slot(1, v, r) * wq;
+        }
+        EulerAdjoint<D, Real>::volume_vjp(a.fp, Uq, Fb, ub);
+#pragma unroll
+        for (int v = 0; v < V; v++) slot(2, v, r) = ub[v];
+      }
+    }
+    for (int r = 0; r < 4; r++) {
+      const int node = t * 16 + MM<Real>::row(grp, r);
+      const bool pad = node >= TS::KV;
+      int q[3];
+      vol_axes<D, K1>(pad ? 0 : node, q);
+      Real bv[V];
+#pragma unroll
+      for (int v = 0; v < V; v++) bv[v] = pad ? Real(0) : slot(2, v, r) * sc;
+      t_test<TS, Real, V>(tab, q, -1, col, bv, aacc);
+    }
+  }
+
+  // ------------------------------------------------------------------- faces
+  for (int f = 0; f < NF; f++) {
+    const int nb = valid ? a.cf.nb[(size_t)cj * NF + f] : -1;
+    const bool has_nb = nb >= 0;
+    const int cn = has_nb ? nb : cj;
+    const int fo = a.cf.pair[f];
+    const Real sg = Real(valid ? a.cf.sqrtg[(size_t)cj * NF + f] : 0.0);
+    Real nrm[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) nrm[d] = Real(a.cf.normal[((size_t)cj * NF + f) * D + d]);
+    const Real scn = Real(a.scaling[cn]);
+    const Real* nbU = a.U + (size_t)cn * NT;
+    const Real* nbW = a.X + (size_t)cn * NT;
+    auto Bnu = [&](int v, int kk) { int m = 4 * kk + grp; return (has_nb && m < N) ? nbU[v * N + m] * scn : Real(0); };
+
+    for (int t = 0; t < TS::TE; t++) {
+      const int nodeE = t * 16 + col;
+      int qo[3], qn[3];
+      face_axes<D, K1>(f, nodeE < TS::KE ? nodeE : 0, qo);
+      face_axes<D, K1>(fo, nodeE < TS::KE ? nodeE : 0, qn);
+      {
+        acc_t acc[V];
+        t_eval<TS, Real, V>(tab, qo, -1, grp, Bu, acc);
+        park(0, acc);
+      }
+      {
+        acc_t acc[V];
+        t_eval<TS, Real, V>(tab, qn, -1, grp, Bnu, acc);
+        park(1, acc);
+      }
+      {
+        // merged face cotangent  sc W_own.trace - scn W_nb.trace  (fn = -fo)
+        acc_t acc[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) acc[v] = acc_zero<Real>();
+#pragma unroll
+        for (int kk = 0; kk < TS::NK; kk++) {
+          int m = 4 * kk + grp;
+          int code = tcode<TS>(tab, m);
+          Real ao = (code >= 0) ? tval<TS>(tab, qo, code, -1) : Real(0);
+          Real an = (code >= 0) ? tval<TS>(tab, qn, code, -1) : Real(0);
+#pragma unroll
+          for (int v = 0; v < V; v++) {
+            Real wo_ = (valid && m < N) ? ownW[v * N + m] * sc : Real(0);
+            Real wn_ = (has_nb && m < N) ? nbW[v * N + m] * scn : Real(0);
+            acc[v] = MM<Real>::mma(ao, wo_, acc[v]);
+            acc[v] = MM<Real>::mma(an, -wn_, acc[v]);
+          }
+        }
+        park(2, acc);
+      }
+      for (int r = 0; r < 4; r++) {
+        const int node = t * 16 + MM<Real>::row(grp, r);
+        const bool pad = node >= TS::KE;
+        int q[3];
+        face_axes<D, K1>(f, pad ? 0 : node, q);
+        const Real wsg = pad ? Real(0) : face_weight<TS>(tab, node) * sg;
+        Real Uo[V], Un[V], gbar[V], fo_[V], ub[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          Uo[v] = slot(0, v, r);
+          Un[v] = has_nb ? slot(1, v, r) : Uo[v];
+          gbar[v] = slot(2, v, r) * wsg;
+          ub[v] = 0;
+        }
+        EulerAdjoint<D, Real>::template inner_vjp<IsHLLC<Form>::value>(a.fp, nrm, Uo, Un, gbar, fo_, ub);
+        Real g[V];
+        if (FUSED) {
+#pragma unroll
+          for (int v = 0; v < V; v++) g[v] = pad ? Real(0) : fo_[v] * wsg * sc;
+          t_test<TS, Real, V>(tab, q, -1, col, g, racc);
+        }
+#pragma unroll
+        for (int v = 0; v < V; v++) g[v] = pad ? Real(0) : ub[v] * sc;
+        t_test<TS, Real, V>(tab, q, -1, col, g, aacc);
+      }
+    }
+  }
+
+  if (valid) {
+    Real* Rp = FUSED ? a.R + (size_t)cell * NT : nullptr;
+    Real* Op = a.out + (size_t)cell * NT;
+    const Real al = Real(a.alpha), be = Real(a.beta);
+#pragma unroll
+    for (int v = 0; v < V; v++)
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          int n = mt * 16 + MM<Real>::row(grp, r);
+          if (n < N) {
+            if (FUSED) {
+              Real old = (be != Real(0)) ? be * Rp[v * N + n] : Real(0);
+              Rp[v * N + n] = old + al * racc[v][mt][r];
+            }
+            Op[v * N + n] = aacc[v][mt][r];
+          }
+        }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Euler JVP with the same LDS node buffer: own trace, neighbour trace and own
+// tangent trace of a 16-node face tile are parked in LDS, the neighbour tangent
+// trace stays in registers; the dual-number flux is then evaluated node by node.
+// ----------------------------------------------------------------------------
+template <class Form, int D, int P, int K1, typename Real>
+__global__ void __launch_bounds__(64 * TWAVES) kt_jvp_euler(TArgs<Real> a) {
+  typedef TShape<D, P, K1> TS;
+  typedef typename MM<Real>::acc acc_t;
+  typedef Dual<1, Real> T;
+  constexpr int V = Form::V, N = TS::N, NT = V * N, NF = TS::NF, MT = TS::MT;
+  constexpr int NSLOT = 3 * V * 4;
+
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Real* tab = reinterpret_cast<Real*>(smem_raw);
+  for (int i = threadIdx.x; i < TS::TAB; i += blockDim.x) tab[i] = a.tab[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, grp = lane >> 4;
+  Real* buf = tab + ((TS::TAB + 1) & ~1) + wave * (NSLOT * 64);
+  auto slot = [&](int arr, int v, int r) -> Real& { return buf[((arr * V + v) * 4 + r) * 64 + lane]; };
+
+  const int cell = (blockIdx.x * TWAVES + wave) * 16 + col;
+  const bool valid = cell < a.J;
+  const int cj = valid ? cell : 0;
+  const Real sc = Real(a.scaling[cj]);
+  const Real detJ = Real(a.detJ[cj]);
+  Real iJ[D][D];
+#pragma unroll
+  for (int e = 0; e < D; e++)
+#pragma unroll
+    for (int d = 0; d < D; d++) iJ[e][d] = Real(a.invJ[(size_t)cj * D * D + e * D + d]);
+
+  acc_t racc[V][MT];
+#pragma unroll
+  for (int v = 0; v < V; v++)
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) racc[v][mt] = acc_zero<Real>();
+
+  const Real* ownU = a.U + (size_t)cj * NT;
+  const Real* ownX = a.X + (size_t)cj * NT;
+  auto Bu = [&](int v, int kk) { int m = 4 * kk + grp; return (valid && m < N) ? ownU[v * N + m] * sc : Real(0); };
+  auto Bx = [&](int v, int kk) { int m = 4 * kk + grp; return (valid && m < N) ? ownX[v * N + m] * sc : Real(0); };
+  auto park = [&](int arr, const acc_t (&acc)[V]) {
+#pragma unroll
+    for (int v = 0; v < V; v++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) slot(arr, v, r) = acc[v][r];
+  };
+
+  // ------------------------------------------------------------------ volume
+  for (int t = 0; t < TS::TV; t++) {
+    int qe[3];
+    const int nodeE = t * 16 + col;
+    vol_axes<D, K1>(nodeE < TS::KV ? nodeE : 0, qe);
+    {
+      acc_t acc[V];
+      t_eval<TS, Real, V>(tab, qe, -1, grp, Bu, acc);
+      park(0, acc);
+      t_eval<TS, Real, V>(tab, qe, -1, grp, Bx, acc);
+      park(1, acc);
+    }
+    for (int r = 0; r < 4; r++) {
+      const int node = t * 16 + MM<Real>::row(grp, r);
+      const bool pad = node >= TS::KV;
+      int q[3];
+      vol_axes<D, K1>(pad ? 0 : node, q);
+      const Real wq = pad ? Real(0) : vol_weight<TS>(tab, q) * detJ * sc;
+      T Uq[V], GU[V][D], F[V][D];
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        Uq[v].v = slot(0, v, r);
+        Uq[v].d[0] = slot(1, v, r);
+#pragma unroll
+        for (int d = 0; d < D; d++) GU[v][d] = Uq[v];
+      }
+      Form::template volume<D, T, Real>(a.fp, Uq, GU, F);
+#pragma unroll
+      for (int e = 0; e < D; e++) {
+        Real g[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          Real s = 0;
+#pragma unroll
+          for (int d = 0; d < D; d++) s += F[v][d].d[0] * iJ[e][d];
+          g[v] = pad ? Real(0) : s * wq;
+        }
+        t_test<TS, Real, V>(tab, q, e, col, g, racc);
+      }
+    }
+  }
+
+  // ------------------------------------------------------------------- faces
+  for (int f = 0; f < NF; f++) {
+    const int nb = valid ? a.cf.nb[(size_t)cj * NF + f] : -1;
+    const bool has_nb = nb >= 0;
+    const int cn = has_nb ? nb : cj;
+    const int fo = a.cf.pair[f];
+    const Real sg = Real(valid ? a.cf.sqrtg[(size_t)cj * NF + f] : 0.0);
+    Real nrm[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) nrm[d] = Real(a.cf.normal[((size_t)cj * NF + f) * D + d]);
+    const Real scn = Real(a.scaling[cn]);
+    const Real* nbU = a.U + (size_t)cn * NT;
+    const Real* nbX = a.X + (size_t)cn * NT;
+    auto Bnu = [&](int v, int kk) { int m = 4 * kk + grp; return (has_nb && m < N) ? nbU[v * N + m] * scn : Real(0); };
+    auto Bnx = [&](int v, int kk) { int m = 4 * kk + grp; return (has_nb && m < N) ? nbX[v * N + m] * scn : Real(0); };
+
+    for (int t = 0; t < TS::TE; t++) {
+      const int nodeE = t * 16 + col;
+      int qo[3], qn[3];
+      face_axes<D, K1>(f, nodeE < TS::KE ? nodeE : 0, qo);
+      face_axes<D, K1>(fo, nodeE < TS::KE ? nodeE : 0, qn);
+      acc_t dun[V];
+      {
+        acc_t acc[V];
+        t_eval<TS, Real, V>(tab, qo, -1, grp, Bu, acc);
+        park(0, acc);
+        t_eval<TS, Real, V>(tab, qn, -1, grp, Bnu, acc);
+        park(1, acc);
+        t_eval<TS, Real, V>(tab, qo, -1, grp, Bx, acc);
+        park(2, acc);
+      }
+      t_eval<TS, Real, V>(tab, qn, -1, grp, Bnx, dun);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int node = t * 16 + MM<Real>::row(grp, r);
+        const bool pad = node >= TS::KE;
+        int q[3];
+        face_axes<D, K1>(f, pad ? 0 : node, q);
+        const Real wq = pad ? Real(0) : face_weight<TS>(tab, node) * sg * sc;
+        T Uo[V], Un[V], Go[V][D], Gn[V][D];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          Uo[v].v = slot(0, v, r); Uo[v].d[0] = slot(2, v, r);
+          Un[v].v = slot(1, v, r); Un[v].d[0] = dun[v][r];
+          if (!has_nb) Un[v] = Uo[v];
+#pragma unroll
+          for (int d = 0; d < D; d++) { Go[v][d] = Uo[v]; Gn[v][d] = Uo[v]; }
+        }
+        T fo_[V], go_[V][D], fn_[V], gn_[V][D];
+        if (has_nb) {
+          Form::template inner<D, T, Real>(a.fp, nrm, Real(0), Uo, Un, Go, Gn, fo_, go_, fn_, gn_);
+        } else {
+#pragma unroll
+          for (int v = 0; v < V; v++) { fo_[v] = Uo[v]; for (int d = 0; d < D; d++) go_[v][d] = Uo[v]; }
+          Form::template boundary<D, T, Real>(a.fp, nrm, Real(0), Uo, Go, fo_, go_);
+        }
+        Real g[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) g[v] = pad ? Real(0) : fo_[v].d[0] * wq;
+        t_test<TS, Real, V>(tab, q, -1, col, g, racc);
+      }
+    }
+  }
+
+  if (valid) {
+    Real* Op = a.out + (size_t)cell * NT;
+#pragma unroll
+    for (int v = 0; v < V; v++)
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          int n = mt * 16 + MM<Real>::row(grp, r);
+          if (n < N) Op[v * N + n] = racc[v][mt][r];
+        }
+  }
+}
+
+template <int D, int P, int K1, int V, typename Real>
+constexpr size_t kt_vjp_euler_lds_bytes() {
+  typedef TShape<D, P, K1> TS;
+  return (size_t)(((TS::TAB + 1) & ~1) + TWAVES * 3 * V * 4 * 64) * sizeof(Real);
 }
 
 }  // namespace bosss

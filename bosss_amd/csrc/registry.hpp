@@ -33,6 +33,23 @@ constexpr int degree_from_modes(int D, int N) {
 
 const KernelSet* find_kernels(int kind, int D, int N, int K1, int precision);
 
+// JVP, VJP and fused residual+VJP kernels: the Euler forms (hand-written adjoint)
+// use the LDS node-buffer kernels, the forms with forward-mode duals the generic ones.
+template <class Form, int D, int P, int K1, typename Real>
+void set_tensor_linearisations(void (**kt)(TArgs<Real>), size_t* lds) {
+  if constexpr (HandAdjoint<Form>::value) {
+    kt[KernelSet::JVP] = kt_jvp_euler<Form, D, P, K1, Real>;
+    kt[KernelSet::VJP] = kt_vjp_euler<Form, D, P, K1, Real, false>;
+    kt[KernelSet::RES_VJP] = kt_vjp_euler<Form, D, P, K1, Real, true>;
+    lds[KernelSet::JVP] = lds[KernelSet::VJP] = lds[KernelSet::RES_VJP] = kt_vjp_euler_lds_bytes<D, P, K1, Form::V, Real>();
+  } else {
+    kt[KernelSet::JVP] = kt_forward<Form, D, P, K1, Real, MODE_JVP>;
+    kt[KernelSet::VJP] = kt_adjoint<Form, D, P, K1, Real, false>;
+    kt[KernelSet::RES_VJP] = kt_adjoint<Form, D, P, K1, Real, true>;
+    lds[KernelSet::JVP] = lds[KernelSet::VJP] = lds[KernelSet::RES_VJP] = kt_lds_bytes<D, P, K1, Form::V, Real>(2);
+  }
+}
+
 template <class Form, int D, int N, int K1, bool WITH32>
 KernelSet make_kernel_set(int kind) {
   typedef Shape<D, N, K1, Form::FACE_GRADU || Form::FACE_TESTG> S;
@@ -54,22 +71,12 @@ KernelSet make_kernel_set(int kind) {
   k.P = P; k.ttab_size = TSh::TAB; k.t_o_l = TSh::O_L; k.t_o_dl = TSh::O_DL; k.t_o_w = TSh::O_W; k.t_o_code = TSh::O_CODE;
   k.t_k2 = TSh::K2;
   k.kt64[KernelSet::RES] = kt_forward<Form, D, P, K1, double, MODE_RES>;
-  k.kt64[KernelSet::JVP] = kt_forward<Form, D, P, K1, double, MODE_JVP>;
-  k.kt64[KernelSet::VJP] = kt_adjoint<Form, D, P, K1, double, false>;
-  k.kt64[KernelSet::RES_VJP] = kt_adjoint<Form, D, P, K1, double, true>;
   k.ktlds64[KernelSet::RES] = kt_lds_bytes<D, P, K1, Form::V, double>(1);
-  k.ktlds64[KernelSet::JVP] = kt_lds_bytes<D, P, K1, Form::V, double>(2);
-  k.ktlds64[KernelSet::VJP] = kt_lds_bytes<D, P, K1, Form::V, double>(2);
-  k.ktlds64[KernelSet::RES_VJP] = kt_lds_bytes<D, P, K1, Form::V, double>(2);
+  set_tensor_linearisations<Form, D, P, K1, double>(k.kt64, k.ktlds64);
   if constexpr (WITH32) {
     k.kt32[KernelSet::RES] = kt_forward<Form, D, P, K1, float, MODE_RES>;
-    k.kt32[KernelSet::JVP] = kt_forward<Form, D, P, K1, float, MODE_JVP>;
-    k.kt32[KernelSet::VJP] = kt_adjoint<Form, D, P, K1, float, false>;
-    k.kt32[KernelSet::RES_VJP] = kt_adjoint<Form, D, P, K1, float, true>;
     k.ktlds32[KernelSet::RES] = kt_lds_bytes<D, P, K1, Form::V, float>(1);
-    k.ktlds32[KernelSet::JVP] = kt_lds_bytes<D, P, K1, Form::V, float>(2);
-    k.ktlds32[KernelSet::VJP] = kt_lds_bytes<D, P, K1, Form::V, float>(2);
-    k.ktlds32[KernelSet::RES_VJP] = kt_lds_bytes<D, P, K1, Form::V, float>(2);
+    set_tensor_linearisations<Form, D, P, K1, float>(k.kt32, k.ktlds32);
     k.k32[KernelSet::RES] = k_forward<Form, D, N, K1, float, MODE_RES>;
     k.k32[KernelSet::JVP] = k_forward<Form, D, N, K1, float, MODE_JVP>;
     k.k32[KernelSet::VJP] = k_adjoint<Form, D, N, K1, float, false>;
