@@ -1,6 +1,7 @@
 // C ABI of the gfx950 DG operator library (declarations: include/bosss_hip.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -49,6 +50,7 @@ struct bosss_ctx {
   int device;
   hipStream_t stream;
   int num_cu;
+  int nxcd;   // XCDs of the device (gfx950: 32 CUs each); BOSSS_HIP_XCDS overrides, 1 disables the block remapping
 };
 
 struct bosss_grid {
@@ -106,6 +108,8 @@ int bosss_hip_ctx_create(int device, bosss_ctx** ctx) {
   bosss_ctx* c = new bosss_ctx();
   c->device = device;
   c->num_cu = prop.multiProcessorCount;
+  c->nxcd = c->num_cu >= 64 ? c->num_cu / 32 : 1;
+  if (const char* x = std::getenv("BOSSS_HIP_XCDS")) c->nxcd = std::max(1, std::atoi(x));
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete c; return fail(BOSSS_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
   *ctx = c;
@@ -438,6 +442,7 @@ static TArgs<Real> make_targs(const bosss_op* op, double alpha, double beta, con
   a.tab = (const Real*)op->ttab;
   a.fp = k.fp; a.alpha = alpha; a.beta = beta;
   a.U = (const Real*)U; a.X = (const Real*)X; a.R = (Real*)R; a.out = (Real*)out;
+  a.nxcd = op->ctx->nxcd;
   return a;
 }
 

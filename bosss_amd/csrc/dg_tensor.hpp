@@ -163,7 +163,19 @@ struct TArgs {
   const Real* X;        // tangent (JVP) or cotangent W (VJP)
   Real* R;
   Real* out;
+  int nxcd;             // XCDs the workgroups are dealt over (1: no remapping)
 };
+
+// Workgroup -> cell-block mapping.  The dispatcher deals consecutive workgroups
+// round-robin over the XCDs, each with its own L2; renumbering the blocks so that
+// XCD x walks one contiguous range of cells keeps a cell's face neighbours (the
+// adjacent cell blocks) in the same L2.  Bijective for any grid size.
+__device__ __forceinline__ int xcd_block(int nxcd) {
+  const int b = blockIdx.x, nb = gridDim.x;
+  if (nxcd <= 1) return b;
+  const int per = nb / nxcd, rem = nb % nxcd, x = b % nxcd, i = b / nxcd;
+  return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
 
 template <class Form> struct HandAdjoint : std::false_type {};
 template <int D, bool H> struct HandAdjoint<FormEuler<D, H>> : std::true_type {};
@@ -190,7 +202,7 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_forward(TArgs<Real> a) {
   for (int i = threadIdx.x; i < TS::TAB; i += blockDim.x) tab[i] = a.tab[i];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, grp = lane >> 4;
-  const int cell0 = (blockIdx.x * TWAVES + wave) * 16;
+  const int cell0 = (xcd_block(a.nxcd) * TWAVES + wave) * 16;
   Real* stU = tab + ((TS::TAB + 1) & ~1) + wave * (NARR * 16 * STRIDE);
   Real* stX = stU + 16 * STRIDE;
   stage16<Real, NT>(stU, STRIDE, a.U, cell0, a.J, lane);
@@ -432,7 +444,7 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_adjoint(TArgs<Real> a) {
   for (int i = threadIdx.x; i < TS::TAB; i += blockDim.x) tab[i] = a.tab[i];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, grp = lane >> 4;
-  const int cell0 = (blockIdx.x * TWAVES + wave) * 16;
+  const int cell0 = (xcd_block(a.nxcd) * TWAVES + wave) * 16;
   Real* stU = tab + ((TS::TAB + 1) & ~1) + wave * (2 * 16 * STRIDE);
   Real* stW = stU + 16 * STRIDE;
   stage16<Real, NT>(stU, STRIDE, a.U, cell0, a.J, lane);
@@ -786,7 +798,7 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
   Real* buf = tab + ((TS::TAB + 1) & ~1) + wave * (NSLOT * 64);
   auto slot = [&](int arr, int v, int r) -> Real& { return buf[((arr * V + v) * 4 + r) * 64 + lane]; };
 
-  const int cell = (blockIdx.x * TWAVES + wave) * 16 + col;
+  const int cell = (xcd_block(a.nxcd) * TWAVES + wave) * 16 + col;
   const bool valid = cell < a.J;
   const int cj = valid ? cell : 0;
   const Real sc = Real(a.scaling[cj]);
@@ -1014,7 +1026,7 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_jvp_euler(TArgs<Real> a) {
   Real* buf = tab + ((TS::TAB + 1) & ~1) + wave * (NSLOT * 64);
   auto slot = [&](int arr, int v, int r) -> Real& { return buf[((arr * V + v) * 4 + r) * 64 + lane]; };
 
-  const int cell = (blockIdx.x * TWAVES + wave) * 16 + col;
+  const int cell = (xcd_block(a.nxcd) * TWAVES + wave) * 16 + col;
   const bool valid = cell < a.J;
   const int cj = valid ? cell : 0;
   const Real sc = Real(a.scaling[cj]);

@@ -83,6 +83,8 @@ CASES = [
     ("euler_lf", 3, 3, 3, True),  # config-5 kernels
     ("euler_hllc", 3, 3, 3, True),
     ("euler_hllc", 2, 5, 2, True),
+    ("euler_hllc", 3, 9, 2, True),  # 729 cells = 12 workgroups: XCD block renumbering with a remainder
+    ("sip", 2, 30, 2, False),       # 900 cells = 15 workgroups
 ]
 
 
