@@ -102,6 +102,101 @@ def copy_calibration(dev, nbytes=1 << 30):
     del a, b
 
 
+def _time_ms(fn, steps, warmup):
+    """Mean duration of fn() over `steps` launches (HIP events on torch's current stream)."""
+    import torch
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for a, b in ev:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    return sum(a.elapsed_time(b) for a, b in ev) / steps
+
+
+def secondary_workload(args):
+    """The other measured configurations of the DG operator path (one JSON line each):
+
+    sip2d    SIP Poisson, 64^2 Cartesian, p=2, Dirichlet: residual + assembled block-sparse apply
+    advdiff  upwind + SIP (nu 0.01), 64^3 periodic, p=3: matrix-free FP64 and FP32 vs assembled SpMV
+    euler32  Euler LF and HLLC, 32^3 periodic, p=2: fused residual + VJP
+    """
+    import numpy as np
+    import torch
+    from bosss_amd.grid import CartesianGrid
+    from bosss_amd.operator import DifferentialOperator
+    from bosss_amd.workloads import euler_state, uniform_coefficients, contraction_flops
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    K, W0 = args.steps, args.warmup
+    res = {}
+    if args.workload == "euler32":
+        grid = CartesianGrid((32,) * 3, periodic=[True] * 3)
+        for form in ("euler_lf", "euler_hllc"):
+            op = DifferentialOperator(grid, form, 2, gamma=1.4)
+            U = torch.from_numpy(euler_state(grid, op.N, seed=0)).to(dev)
+            Wt = torch.from_numpy(uniform_coefficients(op.ndof, 1)).to(dev)
+            R, G = torch.zeros_like(U), torch.zeros_like(U)
+            ms = _time_ms(lambda: op.evaluate_vjp_dev(U, R, Wt, G), K, W0)
+            ms_r = _time_ms(lambda: op.evaluate_dev(U, R), K, W0)
+            fl = contraction_flops(form, 3, 2, grid.J, grid.E)
+            res[form] = {"residual_vjp_ms": round(ms, 4), "gdof_s": round(op.ndof / ms / 1e6, 4),
+                         "residual_ms": round(ms_r, 4), "residual_gdof_s": round(op.ndof / ms_r / 1e6, 4),
+                         "tflops_alg": round(2 * fl / ms / 1e9, 3)}
+            op.close()
+        dof, desc = grid.J * 50, "3D Euler LF/HLLC, 32^3 periodic, p=2 (10 modes x 5 vars), fused residual + VJP"
+    else:
+        if args.workload == "sip2d":
+            grid = CartesianGrid((64, 64), periodic=[False, False])
+            form, p, nu = "sip", 2, 1.0
+            desc = "2D SIP Poisson, 64^2 Cartesian, p=2, Dirichlet: residual and assembled block-sparse apply"
+        else:
+            grid = CartesianGrid((64,) * 3, periodic=[True] * 3)
+            form, p, nu = "advdiff", 3, -0.01
+            desc = "3D upwind advection + SIP, 64^3 periodic, p=3: matrix-free FP64/FP32 vs assembled SpMV"
+        vel = (1.0, 0.5, 0.25)
+        op = DifferentialOperator(grid, form, p, penalty=1.3, nu=nu, velocity=vel)
+        dof = op.ndof
+        U = torch.from_numpy(uniform_coefficients(dof, 0)).to(dev)
+        R = torch.zeros_like(U)
+        ms64 = _time_ms(lambda: op.evaluate_dev(U, R), K, W0)
+        fl = contraction_flops(form, grid.D, p, grid.J, grid.E)
+        res["matrix_free_fp64"] = {"ms": round(ms64, 4), "gdof_s": round(dof / ms64 / 1e6, 4),
+                                   "tflops_alg": round(fl / ms64 / 1e9, 3)}
+        if args.workload == "advdiff":
+            op32 = DifferentialOperator(grid, form, p, penalty=1.3, nu=nu, velocity=vel, precision="fp32")
+            U32, R32 = U.float(), torch.zeros(dof, dtype=torch.float32, device=dev)
+            ms32 = _time_ms(lambda: op32.evaluate_dev(U32, R32), K, W0)
+            err = float((R32.double() - R).abs().max() / R.abs().max())
+            res["matrix_free_fp32"] = {"ms": round(ms32, 4), "gdof_s": round(dof / ms32 / 1e6, 4),
+                                       "tflops_alg": round(fl / ms32 / 1e9, 3), "max_rel_err_vs_fp64": err}
+            op32.close()
+        mb = op.GetMatrixBuilder()
+        blocks = mb.new_matrix()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        mb.ComputeMatrix(blocks)
+        torch.cuda.synchronize()
+        t_asm = time.perf_counter() - t0
+        b = torch.from_numpy(np.asarray(mb.ComputeAffine())).to(dev)      # R(U) = M U + b
+        Y = torch.zeros_like(U)
+        ms_mv = _time_ms(lambda: op.spmv_dev(blocks, U, Y), K, W0)
+        err = float((Y + b - R).abs().max() / R.abs().max())
+        nbytes = blocks.numel() * 8 + 2 * dof * 8
+        res["assembled"] = {"assembly_s": round(t_asm, 4), "probe_colours": mb.ncolors, "matrix_bytes": blocks.numel() * 8,
+                            "spmv_ms": round(ms_mv, 4), "spmv_gdof_s": round(dof / ms_mv / 1e6, 4),
+                            "spmv_gb_s": round(nbytes / ms_mv / 1e6, 1), "spmv_frac_of_8TBs": round(nbytes / ms_mv / 1e6 / 8000, 4),
+                            "max_rel_diff_MU_plus_b_vs_residual": err}
+        op.close()
+    out = {"metric": "DG operator apply, secondary configuration", "workload": args.workload, "config": desc,
+           "dof": dof, "steps": K, "warmup": W0, "dtype": "f64", "data": "synthetic", "results": res}
+    print(json.dumps(out), flush=True)
+
+
 def newton_comparison(grid, op64, Un, form, p):
     """One implicit Euler step solved to ||F|| <= 1e-10: FP64 Newton-GMRES vs FP64 defect
     correction around FP32 JVPs (same driver, bosss_amd.newton)."""
@@ -136,6 +231,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--split", action="store_true", help="time separate residual and VJP launches instead of the fused one")
     ap.add_argument("--no-newton", action="store_true", help="skip the implicit-step Newton comparison (FP64 vs mixed)")
+    ap.add_argument("--workload", default="euler64", choices=["euler64", "sip2d", "advdiff", "euler32"],
+                    help="euler64: the headline line; the others print the secondary configurations")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device copy first (known-size reference for HBM counter readings)")
     args = ap.parse_args()
@@ -143,6 +240,9 @@ def main():
         print("bench.py: this build runs on one GPU only (the multi-GPU sharding is a design, not a code path); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "euler64":
+        secondary_workload(args)
+        return
 
     import numpy as np
     import torch

@@ -34,6 +34,21 @@ class FormParams(ctypes.Structure):
                 ("gamma", ctypes.c_double), ("mach", ctypes.c_double)]
 
 
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+class XdgDesc(ctypes.Structure):
+    _fields_ = [("D", ctypes.c_int32), ("degree", ctypes.c_int32), ("N", ctypes.c_int32), ("J", ctypes.c_int32),
+                ("mode_exponents", _ip), ("nspec", _ip), ("spec0", _ip), ("invJ", _dp), ("scaling", _dp), ("lscale", _dp),
+                ("penalty_base", ctypes.c_double), ("mu", ctypes.c_double * 2), ("source", ctypes.c_double * 2),
+                ("S", ctypes.c_int32), ("cols", _ip), ("boff", _i64p),
+                ("nvol", ctypes.c_int32), ("vol_cell", _ip), ("vol_spec", _ip), ("vol_off", _i64p), ("vol_nodes", _dp), ("vol_w", _dp),
+                ("nface", ctypes.c_int32), ("face_cells", _ip), ("face_slot", _ip), ("face_spec", _ip), ("face_off", _i64p),
+                ("face_nodes_in", _dp), ("face_nodes_out", _dp), ("face_w", _dp), ("face_normal", _dp), ("face_gD", _dp),
+                ("nifc", ctypes.c_int32), ("ifc_cell", _ip), ("ifc_off", _i64p), ("ifc_nodes", _dp), ("ifc_w", _dp),
+                ("ifc_normal", _dp)]
+
+
 # symbol -> (restype, argtypes); must match include/bosss_hip.h
 SIGNATURES = {
     "bosss_hip_last_error": (ctypes.c_char_p, []),
@@ -63,6 +78,15 @@ SIGNATURES = {
     "bosss_hip_block_solve_dev": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "bosss_hip_block_gemv_dev": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp]),
     "bosss_hip_block_last_error": (ctypes.c_char_p, []),
+    "bosss_hip_xdg_create": (ctypes.c_int, [_vp, ctypes.POINTER(XdgDesc), ctypes.POINTER(_vp)]),
+    "bosss_hip_xdg_destroy": (ctypes.c_int, [_vp]),
+    "bosss_hip_xdg_info": (ctypes.c_int, [_vp, _i64p, _i64p]),
+    "bosss_hip_xdg_assemble_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "bosss_hip_xdg_spmv_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp]),
+    "bosss_hip_xdg_solve_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                               ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                               _dp, _vp]),
+    "bosss_hip_xdg_last_error": (ctypes.c_char_p, []),
 }
 
 _lib = None

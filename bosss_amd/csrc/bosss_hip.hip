@@ -91,6 +91,15 @@ void launch_bell_spmv(int J, int Nt, int S, const int32_t* cols, const double* b
                       double* y, hipStream_t s);
 }
 
+namespace bosss {
+int ctx_device_stream(bosss_ctx* c, int* device, hipStream_t* stream) {
+  if (!c) return fail(BOSSS_ERR_ARG, "ctx is NULL");
+  *device = c->device;
+  *stream = c->stream;
+  return BOSSS_OK;
+}
+}  // namespace bosss
+
 extern "C" {
 
 const char* bosss_hip_last_error(void) { return g_err.c_str(); }

@@ -1,0 +1,801 @@
+// Two-species XDG Poisson operator (agglomeration threshold 0) on precomputed cut-cell
+// quadrature items: matrix assembly, uneven block-sparse apply, and the block-Jacobi
+// preconditioned CG solver (FP64, and FP32 with FP64 iterative refinement), captured in
+// a HIP graph.
+//
+// Forms (restated, see the oracle for the line-by-line reading):
+//   XLaplace_Bulk   XLaplace.cs:50-200  (SIPLaplace with Nu = species mu, ipLaplace.cs:183-249)
+//   XLaplace_Interface (mode SIP)  XLaplace.cs:205-373
+//   RHSSource       XdgPoisson3Main.cs:328-358
+// Layout: cell j holds nspec[j] species blocks of N coefficients (species spec0[j] first,
+// XDGBasis.cs:92-113); the matrix is block-ELL with uneven blocks: row j, slot s (0 diagonal,
+// 1+f neighbour across local face f) is the row-major n_j x n_col block at boff[j*S+s].
+//
+// One wavefront per quadrature item.  The basis (orthonormal Legendre products, the
+// reference's mode order given by exponents[N][D]) and its physical gradients are staged
+// for 64 nodes at a time in LDS; lanes own (test, trial) mode pairs.  Face items add into
+// blocks that other faces of the same cell also touch, so they use FP64 atomic adds; volume
+// and interface items own their sub-blocks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bosss_hip.h"
+
+namespace bosss {
+namespace {
+
+constexpr int XN = 20;          // max modes (p <= 3 in 3D)
+constexpr int XD = 3;
+constexpr int XCHUNK = 32;      // nodes staged per pass
+constexpr int XPAIRS = (XN * XN + 63) / 64;
+
+std::string g_xdg_err;
+int xfail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_xdg_err = buf;
+  return code;
+}
+#define XHIP(x)                                                                     \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) return xfail(BOSSS_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+struct XArgs {
+  int D, N, P, J, S;
+  const int32_t* nspec;
+  const int32_t* spec0;
+  const int32_t* expo;      // [N][D]
+  const double* invJ;       // [J][D][D]
+  const double* scaling;    // [J]
+  const double* lscale;     // [J][2]
+  const int32_t* cols;      // [J][S]
+  const int64_t* boff;      // [J][S]
+  const int64_t* dof;       // [J+1]
+  double penalty_base, pdeg, mu[2], src[2];
+};
+
+// normalised Legendre values/derivatives L_k(x) = sqrt((2k+1)/2) P_k(x), k <= P
+__device__ inline void legendre(double x, int P, double* L, double* dL) {
+  double p0 = 1.0, p1 = x, d0 = 0.0, d1 = 1.0;
+  L[0] = p0; dL[0] = d0;
+  if (P >= 1) { L[1] = p1; dL[1] = d1; }
+  for (int k = 1; k < P; k++) {
+    double p2 = ((2 * k + 1) * x * p1 - k * p0) / (k + 1);
+    double d2 = d0 + (2 * k + 1) * p1;
+    p0 = p1; p1 = p2; d0 = d1; d1 = d2;
+    L[k + 1] = p2; dL[k + 1] = d2;
+  }
+  for (int k = 0; k <= P; k++) { double s = sqrt((2 * k + 1) * 0.5); L[k] *= s; dL[k] *= s; }
+}
+
+// physical basis values v[n] and gradients g[n][d] of cell j at reference point xi
+__device__ inline void phys_basis(const XArgs& a, int j, const double* xi, double* v, double (*g)[XD]) {
+  double L[XD][4], dL[XD][4];
+  for (int d = 0; d < a.D; d++) legendre(xi[d], a.P, L[d], dL[d]);
+  const double sc = a.scaling[j];
+  const double* iJ = a.invJ + (size_t)j * a.D * a.D;
+  for (int n = 0; n < a.N; n++) {
+    const int32_t* e = a.expo + n * a.D;
+    double val = 1.0, gr[XD];
+    for (int d = 0; d < a.D; d++) val *= L[d][e[d]];
+    for (int k = 0; k < a.D; k++) {
+      double t = dL[k][e[k]];
+      for (int d = 0; d < a.D; d++) if (d != k) t *= L[d][e[d]];
+      gr[k] = t;
+    }
+    v[n] = sc * val;
+    for (int d = 0; d < a.D; d++) {
+      double s = 0.0;
+      for (int k = 0; k < a.D; k++) s += gr[k] * iJ[k * a.D + d];
+      g[n][d] = sc * s;
+    }
+  }
+}
+
+__device__ inline int spec_off(const XArgs& a, int j, int sp) { return a.spec0[j] == sp ? 0 : a.N; }
+__device__ inline int nrow(const XArgs& a, int j) { return a.nspec[j] * a.N; }
+__device__ inline double* block_at(const XArgs& a, double* blocks, int j, int slot, int r, int c) {
+  const int col = a.cols[j * a.S + slot];
+  return blocks + a.boff[(size_t)j * a.S + slot] + (int64_t)r * nrow(a, col) + c;
+}
+
+// staged basis of up to 64 nodes: values [64][XN], gradients [64][XN][XD]
+struct Stage {
+  double v[XCHUNK][XN];
+  double g[XCHUNK][XN][XD];
+};
+
+__global__ void __launch_bounds__(64) k_xdg_volume(XArgs a, const int32_t* __restrict__ cell, const int32_t* __restrict__ spec,
+                                                   const int64_t* __restrict__ off, const double* __restrict__ nodes,
+                                                   const double* __restrict__ w, double* __restrict__ blocks, double* __restrict__ affine) {
+  __shared__ Stage st;
+  __shared__ double sw[XCHUNK];
+  const int it = blockIdx.x, lane = threadIdx.x;
+  const int j = cell[it], sp = spec[it], o = spec_off(a, j, sp);
+  const int N = a.N, NP = N * N;
+  const double mu = a.mu[sp];
+  double acc[XPAIRS];
+  for (int i = 0; i < XPAIRS; i++) acc[i] = 0.0;
+  double aff = 0.0;
+  for (int64_t q0 = off[it]; q0 < off[it + 1]; q0 += XCHUNK) {
+    const int nq = (int)std::min<int64_t>(XCHUNK, off[it + 1] - q0);
+    if (lane < nq) {
+      phys_basis(a, j, nodes + (q0 + lane) * a.D, st.v[lane], st.g[lane]);
+      sw[lane] = w[q0 + lane];
+    }
+    __syncthreads();
+    for (int i = 0; i < XPAIRS; i++) {
+      const int pr = lane + 64 * i;
+      if (pr >= NP) break;
+      const int r = pr / N, c = pr % N;
+      double s = 0.0;
+      for (int q = 0; q < nq; q++) {
+        double dot = 0.0;
+        for (int d = 0; d < a.D; d++) dot += st.g[q][c][d] * st.g[q][r][d];
+        s -= sw[q] * mu * dot;
+      }
+      acc[i] += s;
+    }
+    if (lane < N)
+      for (int q = 0; q < nq; q++) aff -= sw[q] * a.src[sp] * st.v[q][lane];
+    __syncthreads();
+  }
+  for (int i = 0; i < XPAIRS; i++) {
+    const int pr = lane + 64 * i;
+    if (pr >= NP) break;
+    *block_at(a, blocks, j, 0, o + pr / N, o + pr % N) += acc[i];
+  }
+  if (lane < N) affine[a.dof[j] + o + lane] += aff;
+}
+
+__global__ void __launch_bounds__(64) k_xdg_face(XArgs a, const int32_t* __restrict__ cells, const int32_t* __restrict__ fslot,
+                                                 const int32_t* __restrict__ spec, const int64_t* __restrict__ off,
+                                                 const double* __restrict__ nin, const double* __restrict__ nout,
+                                                 const double* __restrict__ w, const double* __restrict__ normal,
+                                                 const double* __restrict__ gD, double* __restrict__ blocks,
+                                                 double* __restrict__ affine) {
+  __shared__ Stage si, so;
+  __shared__ double sw[XCHUNK], sg[XCHUNK];
+  const int it = blockIdx.x, lane = threadIdx.x;
+  const int ji = cells[2 * it], jo = cells[2 * it + 1], sp = spec[it];
+  const int N = a.N, NP = N * N, D = a.D;
+  const double mu = a.mu[sp];
+  double nrm[XD];
+  for (int d = 0; d < D; d++) nrm[d] = normal[(size_t)it * D + d];
+  const bool inner = jo >= 0;
+  const double eta = inner ? a.penalty_base * fmax(1.0 / a.lscale[2 * ji + sp], 1.0 / a.lscale[2 * jo + sp]) * mu * a.pdeg : 0.0;
+  const double pn = 2.0 * a.penalty_base * a.pdeg / a.lscale[2 * ji + sp];
+  double acc[XPAIRS][4];
+  for (int i = 0; i < XPAIRS; i++) for (int k = 0; k < 4; k++) acc[i][k] = 0.0;
+  double aff = 0.0;
+  for (int64_t q0 = off[it]; q0 < off[it + 1]; q0 += XCHUNK) {
+    const int nq = (int)std::min<int64_t>(XCHUNK, off[it + 1] - q0);
+    if (lane < nq) {
+      phys_basis(a, ji, nin + (q0 + lane) * D, si.v[lane], si.g[lane]);
+      if (inner) phys_basis(a, jo, nout + (q0 + lane) * D, so.v[lane], so.g[lane]);
+      sw[lane] = w[q0 + lane];
+      sg[lane] = gD ? gD[q0 + lane] : 0.0;
+    }
+    __syncthreads();
+    for (int i = 0; i < XPAIRS; i++) {
+      const int pr = lane + 64 * i;
+      if (pr >= NP) break;
+      const int r = pr / N, c = pr % N;
+      for (int q = 0; q < nq; q++) {
+        double gri = 0, gci = 0, gro = 0, gco = 0;
+        for (int d = 0; d < D; d++) {
+          gri += si.g[q][r][d] * nrm[d];
+          gci += si.g[q][c][d] * nrm[d];
+          if (inner) { gro += so.g[q][r][d] * nrm[d]; gco += so.g[q][c][d] * nrm[d]; }
+        }
+        const double ww = sw[q];
+        if (inner) {
+          // (test side ts, trial side us): vA - vB = s_t phi_r, uA - uB = s_u phi_c
+          const double vr[2] = {si.v[q][r], so.v[q][r]}, vc[2] = {si.v[q][c], so.v[q][c]};
+          const double gr[2] = {gri, gro}, gc[2] = {gci, gco};
+#pragma unroll
+          for (int ts = 0; ts < 2; ts++)
+#pragma unroll
+            for (int us = 0; us < 2; us++) {
+              const double st_ = ts ? -1.0 : 1.0, su = us ? -1.0 : 1.0;
+              double f = 0.5 * mu * gc[us] * st_ * vr[ts] + 0.5 * mu * gr[ts] * su * vc[us] - eta * su * st_ * vc[us] * vr[ts];
+              acc[i][ts * 2 + us] += ww * f;
+            }
+        } else {
+          const double vr = si.v[q][r], vc = si.v[q][c];
+          acc[i][0] += ww * (mu * gci * vr + mu * gri * vc - mu * vc * vr * pn);
+        }
+      }
+    }
+    if (!inner && lane < N)
+      for (int q = 0; q < nq; q++) {
+        double gvn = 0;
+        for (int d = 0; d < D; d++) gvn += si.g[q][lane][d] * nrm[d];
+        aff += sw[q] * (-mu * gvn * sg[q] + mu * sg[q] * si.v[q][lane] * pn);
+      }
+    __syncthreads();
+  }
+  const int oi = spec_off(a, ji, sp), oo = inner ? spec_off(a, jo, sp) : 0;
+  for (int i = 0; i < XPAIRS; i++) {
+    const int pr = lane + 64 * i;
+    if (pr >= NP) break;
+    const int r = pr / N, c = pr % N;
+    if (inner) {
+      unsafeAtomicAdd(block_at(a, blocks, ji, 0, oi + r, oi + c), acc[i][0]);
+      unsafeAtomicAdd(block_at(a, blocks, ji, fslot[2 * it], oi + r, oo + c), acc[i][1]);
+      unsafeAtomicAdd(block_at(a, blocks, jo, fslot[2 * it + 1], oo + r, oi + c), acc[i][2]);
+      unsafeAtomicAdd(block_at(a, blocks, jo, 0, oo + r, oo + c), acc[i][3]);
+    } else {
+      unsafeAtomicAdd(block_at(a, blocks, ji, 0, oi + r, oi + c), acc[i][0]);
+    }
+  }
+  if (!inner && lane < N) unsafeAtomicAdd(affine + a.dof[ji] + oi + lane, aff);
+}
+
+__global__ void __launch_bounds__(64) k_xdg_interface(XArgs a, const int32_t* __restrict__ cell, const int64_t* __restrict__ off,
+                                                      const double* __restrict__ nodes, const double* __restrict__ w,
+                                                      const double* __restrict__ normal, double* __restrict__ blocks) {
+  __shared__ Stage st;
+  __shared__ double sw[XCHUNK], sn[XCHUNK][XD];
+  const int it = blockIdx.x, lane = threadIdx.x;
+  const int j = cell[it];
+  if (a.nspec[j] != 2) return;
+  const int N = a.N, NP = N * N, D = a.D;
+  const double muA = a.mu[0], muB = a.mu[1];
+  const double pmu = fmax(fabs(muA), fabs(muB)) * (muA < 0 ? -1.0 : 1.0);
+  const double eta = a.penalty_base * fmax(1.0 / a.lscale[2 * j], 1.0 / a.lscale[2 * j + 1]) * pmu * a.pdeg;
+  const double muS[2] = {muA, muB};
+  double acc[XPAIRS][4];
+  for (int i = 0; i < XPAIRS; i++) for (int k = 0; k < 4; k++) acc[i][k] = 0.0;
+  for (int64_t q0 = off[it]; q0 < off[it + 1]; q0 += XCHUNK) {
+    const int nq = (int)std::min<int64_t>(XCHUNK, off[it + 1] - q0);
+    if (lane < nq) {
+      phys_basis(a, j, nodes + (q0 + lane) * D, st.v[lane], st.g[lane]);
+      sw[lane] = w[q0 + lane];
+      for (int d = 0; d < D; d++) sn[lane][d] = normal[(q0 + lane) * D + d];
+    }
+    __syncthreads();
+    for (int i = 0; i < XPAIRS; i++) {
+      const int pr = lane + 64 * i;
+      if (pr >= NP) break;
+      const int r = pr / N, c = pr % N;
+      for (int q = 0; q < nq; q++) {
+        double gr = 0, gc = 0;
+        for (int d = 0; d < D; d++) { gr += st.g[q][r][d] * sn[q][d]; gc += st.g[q][c][d] * sn[q][d]; }
+        const double vr = st.v[q][r], vc = st.v[q][c], ww = sw[q];
+#pragma unroll
+        for (int ts = 0; ts < 2; ts++)
+#pragma unroll
+          for (int us = 0; us < 2; us++) {
+            const double st_ = ts ? -1.0 : 1.0, su = us ? -1.0 : 1.0;
+            double f = muS[us] * 0.5 * gc * st_ * vr + muS[ts] * 0.5 * gr * su * vc - eta * su * st_ * vc * vr;
+            acc[i][ts * 2 + us] += ww * f;
+          }
+      }
+    }
+    __syncthreads();
+  }
+  const int o[2] = {spec_off(a, j, 0), spec_off(a, j, 1)};
+  for (int i = 0; i < XPAIRS; i++) {
+    const int pr = lane + 64 * i;
+    if (pr >= NP) break;
+    const int r = pr / N, c = pr % N;
+    for (int ts = 0; ts < 2; ts++)
+      for (int us = 0; us < 2; us++) *block_at(a, blocks, j, 0, o[ts] + r, o[us] + c) += acc[i][ts * 2 + us];
+  }
+}
+
+template <typename T>
+__global__ void k_fill(int64_t n, T* __restrict__ x, T v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] = v;
+}
+
+__global__ void k_to_float(int64_t n, const double* __restrict__ x, float* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = (float)x[i];
+}
+
+// y <- alpha A x + beta y, one wavefront per block row, lane per row of the block
+template <typename Real>
+__global__ void __launch_bounds__(64) k_xdg_spmv(int J, int S, const int32_t* __restrict__ nrows, const int32_t* __restrict__ cols,
+                                                 const int64_t* __restrict__ boff, const int64_t* __restrict__ dof,
+                                                 const Real* __restrict__ A, double alpha, const Real* __restrict__ x, double beta,
+                                                 Real* __restrict__ y) {
+  const int j = blockIdx.x, r = threadIdx.x;
+  const int n = nrows[j];
+  if (r >= n) return;
+  double acc = 0.0;
+  for (int s = 0; s < S; s++) {
+    const int k = cols[j * S + s];
+    if (k < 0) continue;
+    const int m = nrows[k];
+    const Real* row = A + boff[(size_t)j * S + s] + (int64_t)r * m;
+    const Real* xk = x + dof[k];
+    for (int c = 0; c < m; c++) acc += (double)row[c] * (double)xk[c];
+  }
+  Real* yr = y + dof[j] + r;
+  *yr = (Real)(alpha * acc + (beta != 0.0 ? beta * (double)*yr : 0.0));
+}
+
+// gather the diagonal blocks into a compact batch (offsets doff) for the block LU
+__global__ void k_gather_diag(int J, int S, const int32_t* __restrict__ nrows, const int64_t* __restrict__ boff,
+                              const int64_t* __restrict__ doff, const double* __restrict__ A, double* __restrict__ Dg) {
+  const int j = blockIdx.x;
+  const int n = nrows[j];
+  for (int i = threadIdx.x; i < n * n; i += blockDim.x) Dg[doff[j] + i] = A[boff[(size_t)j * S] + i];
+}
+
+// z_j <- Dinv_j r_j
+template <typename Real>
+__global__ void __launch_bounds__(64) k_bdiag_apply(int J, const int32_t* __restrict__ nrows, const int64_t* __restrict__ doff,
+                                                    const int64_t* __restrict__ dof, const Real* __restrict__ Dinv,
+                                                    const Real* __restrict__ r, Real* __restrict__ z) {
+  const int j = blockIdx.x, i = threadIdx.x;
+  const int n = nrows[j];
+  if (i >= n) return;
+  const Real* row = Dinv + doff[j] + (int64_t)i * n;
+  const Real* rj = r + dof[j];
+  double acc = 0.0;
+  for (int c = 0; c < n; c++) acc += (double)row[c] * (double)rj[c];
+  z[dof[j] + i] = (Real)acc;
+}
+
+// ------------------------------------------------------------ reductions
+constexpr int RED_BLOCKS = 256, RED_THREADS = 256;
+
+__device__ inline double block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) t += sh[i];
+  return t;
+}
+
+// partial[2*b + {0,1}] = sums over this block of x.y and x.x (second only if want2)
+template <typename Real>
+__global__ void __launch_bounds__(RED_THREADS) k_dot_partial(int64_t n, const Real* __restrict__ x, const Real* __restrict__ y,
+                                                             int want2, double* __restrict__ partial) {
+  __shared__ double sh[RED_THREADS / 64];
+  double s1 = 0.0, s2 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double xi = (double)x[i];
+    s1 += xi * (double)y[i];
+    if (want2) s2 += xi * xi;
+  }
+  double t1 = block_sum(s1, sh);
+  __syncthreads();
+  double t2 = want2 ? block_sum(s2, sh) : 0.0;
+  if (threadIdx.x == 0) { partial[2 * blockIdx.x] = t1; partial[2 * blockIdx.x + 1] = t2; }
+}
+
+// scal[slot1] = sum of partial sums 0, scal[slot2] = sum of partial sums 1 (slot2 < 0: skip)
+__global__ void __launch_bounds__(RED_THREADS) k_dot_final(int nb, const double* __restrict__ partial, double* __restrict__ scal,
+                                                           int slot1, int slot2) {
+  __shared__ double sh[RED_THREADS / 64];
+  double s1 = 0.0, s2 = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) { s1 += partial[2 * i]; s2 += partial[2 * i + 1]; }
+  double t1 = block_sum(s1, sh);
+  __syncthreads();
+  double t2 = block_sum(s2, sh);
+  if (threadIdx.x == 0) { scal[slot1] = t1; if (slot2 >= 0) scal[slot2] = t2; }
+}
+
+// PCG scalar slots
+enum { S_RZ = 0, S_PQ = 1, S_RZN = 2, S_RR = 3, S_BB = 4, S_NSLOT = 8 };
+
+// x += alpha p, r -= alpha q with alpha = rz / pq (0 if pq == 0)
+template <typename Real>
+__global__ void k_pcg_xr(int64_t n, const double* __restrict__ scal, const Real* __restrict__ p, const Real* __restrict__ q,
+                         Real* __restrict__ x, Real* __restrict__ r) {
+  const double pq = scal[S_PQ];
+  const double al = (pq != 0.0) ? scal[S_RZ] / pq : 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] = (Real)((double)x[i] + al * (double)p[i]);
+    r[i] = (Real)((double)r[i] - al * (double)q[i]);
+  }
+}
+// p = z + beta p with beta = rz_new / rz (0 if rz == 0)
+template <typename Real>
+__global__ void k_pcg_p(int64_t n, const double* __restrict__ scal, const Real* __restrict__ z, Real* __restrict__ p) {
+  const double rz = scal[S_RZ];
+  const double be = (rz != 0.0) ? scal[S_RZN] / rz : 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (Real)((double)z[i] + be * (double)p[i]);
+}
+__global__ void k_scal_copy(double* scal, int dst, int src) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) scal[dst] = scal[src];
+}
+// y = (Real)(s * x)   (FP64 residual -> scaled FP32 right-hand side, and back)
+template <typename A, typename B>
+__global__ void k_scale_copy(int64_t n, double s, const A* __restrict__ x, B* __restrict__ y, int accumulate) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (B)((accumulate ? (double)y[i] : 0.0) + s * (double)x[i]);
+}
+
+}  // namespace
+}  // namespace bosss
+
+using namespace bosss;
+
+struct bosss_xdg {
+  bosss_ctx* ctx = nullptr;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  XArgs a{};
+  int nvol = 0, nface = 0, nifc = 0;
+  int64_t ndof = 0, nentries = 0, ndiag = 0;
+  std::vector<void*> allocs;
+  int32_t *d_nspec, *d_spec0, *d_expo, *d_cols, *d_nrows;
+  int64_t *d_boff, *d_dof, *d_doff;
+  double *d_invJ, *d_scal, *d_ls;
+  int32_t *v_cell, *v_spec, *f_cells, *f_slot, *f_spec, *i_cell;
+  int64_t *v_off, *f_off, *i_off;
+  double *v_nodes, *v_w, *f_nin, *f_nout, *f_w, *f_normal, *f_gD, *i_nodes, *i_w, *i_normal;
+  // solver workspace (allocated on first solve)
+  bool ws = false;
+  double *Dinv = nullptr, *scal = nullptr, *partial = nullptr;
+  int32_t* info = nullptr;
+  double *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *q64 = nullptr;
+  float *A32 = nullptr, *Dinv32 = nullptr, *b32 = nullptr, *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  const void* graph_key[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  int graph_k[2] = {0, 0};
+  ~bosss_xdg() {
+    for (int i = 0; i < 2; i++) if (graph[i]) (void)hipGraphExecDestroy(graph[i]);
+    for (void* p : allocs) (void)hipFree(p);
+  }
+};
+
+namespace bosss {
+// the context's device and internal stream (bosss_hip.hip)
+int ctx_device_stream(bosss_ctx* ctx, int* device, hipStream_t* stream);
+}
+
+namespace {
+
+template <typename T>
+int upload(bosss_xdg* x, T** dst, const T* src, size_t n) {
+  *dst = nullptr;
+  if (n == 0) return BOSSS_OK;
+  if (!src) return xfail(BOSSS_ERR_ARG, "missing array (%zu elements)", n);
+  XHIP(hipMalloc((void**)dst, n * sizeof(T)));
+  x->allocs.push_back(*dst);
+  XHIP(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return BOSSS_OK;
+}
+template <typename T>
+int alloc(bosss_xdg* x, T** dst, size_t n) {
+  XHIP(hipMalloc((void**)dst, std::max<size_t>(n, 1) * sizeof(T)));
+  x->allocs.push_back(*dst);
+  return BOSSS_OK;
+}
+#define XCHK(x)                    \
+  do {                             \
+    int rc_ = (x);                 \
+    if (rc_ != BOSSS_OK) return rc_; \
+  } while (0)
+
+inline unsigned grid_for(int64_t n, int threads = 256) {
+  int64_t b = (n + threads - 1) / threads;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, 4096));
+}
+
+template <typename Real>
+void launch_dot(bosss_xdg* X, int64_t n, const Real* x, const Real* y, int want2, int slot1, int slot2, hipStream_t s) {
+  hipLaunchKernelGGL(k_dot_partial<Real>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, n, x, y, want2, X->partial);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, s, RED_BLOCKS, X->partial, X->scal, slot1, slot2);
+}
+
+template <typename Real>
+void launch_spmv(bosss_xdg* X, const Real* A, double alpha, const Real* x, double beta, Real* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_xdg_spmv<Real>, dim3(X->a.J), dim3(64), 0, s, X->a.J, X->a.S, X->d_nrows, X->d_cols, X->d_boff, X->d_dof, A,
+                     alpha, x, beta, y);
+}
+
+// one PCG iteration (Real = double or float), all scalars on the device
+template <typename Real>
+void pcg_iteration(bosss_xdg* X, const Real* A, const Real* Dinv, Real* x, Real* r, Real* z, Real* p, Real* q, hipStream_t s) {
+  const int64_t n = X->ndof;
+  launch_spmv<Real>(X, A, 1.0, p, 0.0, q, s);
+  launch_dot<Real>(X, n, p, q, 0, S_PQ, -1, s);
+  hipLaunchKernelGGL(k_pcg_xr<Real>, dim3(grid_for(n)), dim3(256), 0, s, n, X->scal, p, q, x, r);
+  hipLaunchKernelGGL(k_bdiag_apply<Real>, dim3(X->a.J), dim3(64), 0, s, X->a.J, X->d_nrows, X->d_doff, X->d_dof, Dinv, r, z);
+  launch_dot<Real>(X, n, r, z, 1, S_RZN, S_RR, s);
+  hipLaunchKernelGGL(k_pcg_p<Real>, dim3(grid_for(n)), dim3(256), 0, s, n, X->scal, z, p);
+  hipLaunchKernelGGL(k_scal_copy, dim3(1), dim3(64), 0, s, X->scal, (int)S_RZ, (int)S_RZN);
+}
+
+// r = b - A x, z = Dinv r, p = z, rz = r.z, rr = r.r, bb = b.b
+template <typename Real>
+void pcg_start(bosss_xdg* X, const Real* A, const Real* Dinv, const Real* b, Real* x, Real* r, Real* z, Real* p, hipStream_t s) {
+  const int64_t n = X->ndof;
+  hipLaunchKernelGGL((k_scale_copy<Real, Real>), dim3(grid_for(n)), dim3(256), 0, s, n, 1.0, b, r, 0);
+  launch_spmv<Real>(X, A, -1.0, x, 1.0, r, s);
+  hipLaunchKernelGGL(k_bdiag_apply<Real>, dim3(X->a.J), dim3(64), 0, s, X->a.J, X->d_nrows, X->d_doff, X->d_dof, Dinv, r, z);
+  hipLaunchKernelGGL((k_scale_copy<Real, Real>), dim3(grid_for(n)), dim3(256), 0, s, n, 1.0, z, p, 0);
+  launch_dot<Real>(X, n, r, z, 1, S_RZ, S_RR, s);
+  launch_dot<Real>(X, n, b, b, 0, S_BB, -1, s);
+}
+
+// Run PCG until sqrt(rr/bb) <= tol or maxit; K iterations per check (one graph launch).
+template <typename Real>
+int pcg_solve(bosss_xdg* X, int gslot, const Real* A, const Real* Dinv, const Real* b, Real* x, Real* r, Real* z, Real* p, Real* q,
+              double tol, int maxit, int K, int use_graph, int* iters, double* relres) {
+  hipStream_t s = X->stream;
+  pcg_start<Real>(X, A, Dinv, b, x, r, z, p, s);
+  double h[S_NSLOT];
+  XHIP(hipMemcpyAsync(h, X->scal, sizeof h, hipMemcpyDeviceToHost, s));
+  XHIP(hipStreamSynchronize(s));
+  double bb = h[S_BB], rel = bb > 0 ? std::sqrt(h[S_RR] / bb) : 0.0;
+  int it = 0;
+  if (use_graph) {
+    // the graph bakes in the matrix and solution pointers: rebuild when they change
+    const void* key0 = (const void*)A;
+    const void* key1 = (const void*)x;
+    if (!X->graph[gslot] || X->graph_key[gslot][0] != key0 || X->graph_key[gslot][1] != key1 || X->graph_k[gslot] != K) {
+      if (X->graph[gslot]) { XHIP(hipGraphExecDestroy(X->graph[gslot])); X->graph[gslot] = nullptr; }
+      hipGraph_t g;
+      XHIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < K; k++) pcg_iteration<Real>(X, A, Dinv, x, r, z, p, q, s);
+      XHIP(hipStreamEndCapture(s, &g));
+      hipError_t e = hipGraphInstantiate(&X->graph[gslot], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return xfail(BOSSS_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+      X->graph_key[gslot][0] = key0;
+      X->graph_key[gslot][1] = key1;
+      X->graph_k[gslot] = K;
+    }
+  }
+  // The recurrence residual drifts from b - A x in finite precision: on convergence the true
+  // residual is recomputed (pcg_start) and CG restarted from the current iterate if it misses.
+  for (int restart = 0; restart < 8; restart++) {
+    while (rel > tol && it < maxit) {
+      if (use_graph) {
+        XHIP(hipGraphLaunch(X->graph[gslot], s));
+      } else {
+        for (int k = 0; k < K; k++) pcg_iteration<Real>(X, A, Dinv, x, r, z, p, q, s);
+      }
+      it += K;
+      XHIP(hipMemcpyAsync(h, X->scal, sizeof h, hipMemcpyDeviceToHost, s));
+      XHIP(hipStreamSynchronize(s));
+      rel = std::sqrt(std::max(h[S_RR], 0.0) / bb);
+      if (!std::isfinite(rel)) return xfail(BOSSS_ERR_NONFINITE, "PCG residual is not finite after %d iterations", it);
+    }
+    if (it == 0) break;
+    pcg_start<Real>(X, A, Dinv, b, x, r, z, p, s);        // true residual r = b - A x
+    XHIP(hipMemcpyAsync(h, X->scal, sizeof h, hipMemcpyDeviceToHost, s));
+    XHIP(hipStreamSynchronize(s));
+    rel = std::sqrt(std::max(h[S_RR], 0.0) / bb);
+    if (rel <= tol || it >= maxit) break;
+  }
+  *iters = it;
+  *relres = rel;
+  return BOSSS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bosss_hip_xdg_last_error(void) { return g_xdg_err.c_str(); }
+
+int bosss_hip_xdg_create(bosss_ctx* ctx, const bosss_xdg_desc* d, bosss_xdg** out) {
+  if (!ctx || !d || !out) return xfail(BOSSS_ERR_ARG, "NULL argument");
+  if (d->D < 2 || d->D > 3) return xfail(BOSSS_ERR_ARG, "D must be 2 or 3");
+  if (d->N < 1 || d->N > XN || d->degree < 0 || d->degree > 3) return xfail(BOSSS_ERR_UNSUPPORTED, "XDG kernels cover p <= 3 (N <= %d)", XN);
+  if (d->S != 1 + 2 * d->D) return xfail(BOSSS_ERR_ARG, "S must be 1 + 2D");
+  for (int n = 0; n < d->N * d->D; n++)
+    if (d->mode_exponents[n] < 0 || d->mode_exponents[n] > d->degree) return xfail(BOSSS_ERR_ARG, "mode exponent out of range");
+  for (int j = 0; j < d->J; j++)
+    if (d->nspec[j] < 1 || d->nspec[j] > 2 || d->spec0[j] < 0 || d->spec0[j] > 1) return xfail(BOSSS_ERR_ARG, "bad species layout at cell %d", j);
+  int dev = 0;
+  hipStream_t st = nullptr;
+  int rc = ctx_device_stream(ctx, &dev, &st);
+  if (rc != BOSSS_OK) return rc;
+  XHIP(hipSetDevice(dev));
+  bosss_xdg* x = new bosss_xdg();
+  x->ctx = ctx; x->device = dev; x->stream = st;
+  const int J = d->J, D = d->D, S = d->S, N = d->N;
+  x->nvol = d->nvol; x->nface = d->nface; x->nifc = d->nifc;
+  // host-side derived layout
+  std::vector<int32_t> nrows(J);
+  std::vector<int64_t> dof(J + 1, 0), doff(J + 1, 0);
+  for (int j = 0; j < J; j++) { nrows[j] = d->nspec[j] * N; dof[j + 1] = dof[j] + nrows[j]; doff[j + 1] = doff[j] + (int64_t)nrows[j] * nrows[j]; }
+  int64_t ent = 0;
+  for (int j = 0; j < J; j++)
+    for (int s = 0; s < S; s++) {
+      int k = d->cols[j * S + s];
+      if (k < -1 || k >= J || (s == 0 && k != j)) { delete x; return xfail(BOSSS_ERR_ARG, "bad column cell at (%d, %d)", j, s); }
+      if (k >= 0) {
+        if (d->boff[(size_t)j * S + s] < 0) { delete x; return xfail(BOSSS_ERR_ARG, "bad block offset at (%d, %d)", j, s); }
+        ent = std::max<int64_t>(ent, d->boff[(size_t)j * S + s] + (int64_t)nrows[j] * nrows[k]);
+      }
+    }
+  x->ndof = dof[J]; x->nentries = ent; x->ndiag = doff[J];
+  // bounds checks of the quadrature items against the layout
+  auto bad_off = [](const int64_t* o, int n) { if (n && o[0] != 0) return true; for (int i = 0; i < n; i++) if (o[i + 1] < o[i]) return true; return false; };
+  if (bad_off(d->vol_off, d->nvol) || bad_off(d->face_off, d->nface) || bad_off(d->ifc_off, d->nifc)) {
+    delete x; return xfail(BOSSS_ERR_ARG, "node offsets must start at 0 and increase");
+  }
+  for (int i = 0; i < d->nvol; i++)
+    if (d->vol_cell[i] < 0 || d->vol_cell[i] >= J || d->vol_spec[i] < 0 || d->vol_spec[i] > 1 ||
+        !(d->spec0[d->vol_cell[i]] == d->vol_spec[i] || d->nspec[d->vol_cell[i]] == 2)) { delete x; return xfail(BOSSS_ERR_ARG, "volume item %d: cell/species not in the layout", i); }
+  for (int i = 0; i < d->nface; i++) {
+    int ji = d->face_cells[2 * i], jo = d->face_cells[2 * i + 1], sp = d->face_spec[i];
+    auto ok = [&](int j) { return j >= 0 && j < J && (d->spec0[j] == sp || d->nspec[j] == 2); };
+    if (sp < 0 || sp > 1 || !ok(ji) || (jo >= 0 && !ok(jo))) { delete x; return xfail(BOSSS_ERR_ARG, "face item %d: cell/species not in the layout", i); }
+    if (jo >= 0) {
+      int s1 = d->face_slot[2 * i], s2 = d->face_slot[2 * i + 1];
+      if (s1 < 1 || s1 >= S || s2 < 1 || s2 >= S || d->cols[ji * S + s1] != jo || d->cols[jo * S + s2] != ji) {
+        delete x; return xfail(BOSSS_ERR_ARG, "face item %d: slots do not match the block columns", i);
+      }
+    }
+  }
+  for (int i = 0; i < d->nifc; i++)
+    if (d->ifc_cell[i] < 0 || d->ifc_cell[i] >= J) { delete x; return xfail(BOSSS_ERR_ARG, "interface item %d: bad cell", i); }
+  const size_t KV = d->nvol ? (size_t)d->vol_off[d->nvol] : 0, KF = d->nface ? (size_t)d->face_off[d->nface] : 0,
+               KI = d->nifc ? (size_t)d->ifc_off[d->nifc] : 0;
+  rc = BOSSS_OK;
+#define UP(dst, src, n) if (rc == BOSSS_OK) rc = upload(x, &x->dst, src, n)
+  UP(d_nspec, d->nspec, J); UP(d_spec0, d->spec0, J); UP(d_expo, d->mode_exponents, (size_t)N * D);
+  UP(d_cols, d->cols, (size_t)J * S); UP(d_boff, d->boff, (size_t)J * S);
+  UP(d_nrows, nrows.data(), J); UP(d_dof, dof.data(), J + 1); UP(d_doff, doff.data(), J + 1);
+  UP(d_invJ, d->invJ, (size_t)J * D * D); UP(d_scal, d->scaling, J); UP(d_ls, d->lscale, (size_t)J * 2);
+  UP(v_cell, d->vol_cell, d->nvol); UP(v_spec, d->vol_spec, d->nvol); UP(v_off, d->vol_off, d->nvol + 1);
+  UP(v_nodes, d->vol_nodes, KV * D); UP(v_w, d->vol_w, KV);
+  UP(f_cells, d->face_cells, (size_t)2 * d->nface); UP(f_slot, d->face_slot, (size_t)2 * d->nface); UP(f_spec, d->face_spec, d->nface);
+  UP(f_off, d->face_off, d->nface + 1); UP(f_nin, d->face_nodes_in, KF * D); UP(f_nout, d->face_nodes_out, KF * D);
+  UP(f_w, d->face_w, KF); UP(f_normal, d->face_normal, (size_t)d->nface * D);
+  if (d->face_gD) { UP(f_gD, d->face_gD, KF); } else { x->f_gD = nullptr; }
+  UP(i_cell, d->ifc_cell, d->nifc); UP(i_off, d->ifc_off, d->nifc + 1); UP(i_nodes, d->ifc_nodes, KI * D);
+  UP(i_w, d->ifc_w, KI); UP(i_normal, d->ifc_normal, KI * D);
+#undef UP
+  if (rc != BOSSS_OK) { delete x; return rc; }
+  XArgs& a = x->a;
+  a.D = D; a.N = N; a.P = d->degree; a.J = J; a.S = S;
+  a.nspec = x->d_nspec; a.spec0 = x->d_spec0; a.expo = x->d_expo; a.invJ = x->d_invJ; a.scaling = x->d_scal; a.lscale = x->d_ls;
+  a.cols = x->d_cols; a.boff = x->d_boff; a.dof = x->d_dof;
+  a.penalty_base = d->penalty_base;
+  const double p = d->degree;
+  a.pdeg = std::max((p + 1) * (p + D) / D, (p + 1.0) * (p + 1.0));
+  a.mu[0] = d->mu[0]; a.mu[1] = d->mu[1]; a.src[0] = d->source[0]; a.src[1] = d->source[1];
+  *out = x;
+  return BOSSS_OK;
+}
+
+int bosss_hip_xdg_destroy(bosss_xdg* x) {
+  if (x) { (void)hipSetDevice(x->device); delete x; }
+  return BOSSS_OK;
+}
+
+int bosss_hip_xdg_info(bosss_xdg* x, int64_t* ndof, int64_t* nentries) {
+  if (!x) return xfail(BOSSS_ERR_ARG, "NULL operator");
+  if (ndof) *ndof = x->ndof;
+  if (nentries) *nentries = x->nentries;
+  return BOSSS_OK;
+}
+
+int bosss_hip_xdg_assemble_dev(bosss_xdg* x, double* blocks, double* affine, void* stream) {
+  if (!x || !blocks || !affine) return xfail(BOSSS_ERR_ARG, "NULL argument");
+  XHIP(hipSetDevice(x->device));
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(x->nentries)), dim3(256), 0, s, x->nentries, blocks, 0.0);
+  hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(x->ndof)), dim3(256), 0, s, x->ndof, affine, 0.0);
+  if (x->nvol)
+    hipLaunchKernelGGL(k_xdg_volume, dim3(x->nvol), dim3(64), 0, s, x->a, x->v_cell, x->v_spec, x->v_off, x->v_nodes, x->v_w, blocks, affine);
+  if (x->nface)
+    hipLaunchKernelGGL(k_xdg_face, dim3(x->nface), dim3(64), 0, s, x->a, x->f_cells, x->f_slot, x->f_spec, x->f_off, x->f_nin, x->f_nout,
+                       x->f_w, x->f_normal, x->f_gD, blocks, affine);
+  if (x->nifc)
+    hipLaunchKernelGGL(k_xdg_interface, dim3(x->nifc), dim3(64), 0, s, x->a, x->i_cell, x->i_off, x->i_nodes, x->i_w, x->i_normal, blocks);
+  XHIP(hipGetLastError());
+  return BOSSS_OK;
+}
+
+int bosss_hip_xdg_spmv_dev(bosss_xdg* x, const double* blocks, double alpha, const double* v, double beta, double* y, void* stream) {
+  if (!x || !blocks || !v || !y) return xfail(BOSSS_ERR_ARG, "NULL argument");
+  XHIP(hipSetDevice(x->device));
+  launch_spmv<double>(x, blocks, alpha, v, beta, y, (hipStream_t)stream);
+  XHIP(hipGetLastError());
+  return BOSSS_OK;
+}
+
+int bosss_hip_xdg_solve_dev(bosss_xdg* x, const double* blocks, const double* b, double* sol, double tol, int32_t maxit,
+                            int32_t precision, int32_t use_graph, int32_t check_every, int32_t* iterations, int32_t* outer,
+                            double* relres, void* stream) {
+  if (!x || !blocks || !b || !sol || !iterations || !relres) return xfail(BOSSS_ERR_ARG, "NULL argument");
+  if (precision != BOSSS_FP64 && precision != BOSSS_FP32) return xfail(BOSSS_ERR_ARG, "precision must be BOSSS_FP64 or BOSSS_FP32");
+  if (check_every < 1 || check_every > 256) return xfail(BOSSS_ERR_ARG, "check_every must be 1..256");
+  if (!(tol > 0)) return xfail(BOSSS_ERR_ARG, "tol must be positive");
+  XHIP(hipSetDevice(x->device));
+  hipStream_t s = x->stream;
+  {
+    // order after the caller's stream (blocks, b and sol may have been written there)
+    hipEvent_t ev;
+    XHIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    hipError_t e1 = hipEventRecord(ev, (hipStream_t)stream);
+    hipError_t e2 = (e1 == hipSuccess) ? hipStreamWaitEvent(s, ev, 0) : e1;
+    (void)hipEventDestroy(ev);
+    if (e2 != hipSuccess) return xfail(BOSSS_ERR_HIP, "stream ordering: %s", hipGetErrorString(e2));
+  }
+  const int64_t n = x->ndof;
+  if (!x->ws) {
+    XCHK(alloc(x, &x->Dinv, x->ndiag)); XCHK(alloc(x, &x->scal, S_NSLOT)); XCHK(alloc(x, &x->partial, 2 * RED_BLOCKS));
+    XCHK(alloc(x, &x->info, x->a.J));
+    XCHK(alloc(x, &x->r64, n)); XCHK(alloc(x, &x->z64, n)); XCHK(alloc(x, &x->p64, n)); XCHK(alloc(x, &x->q64, n));
+    x->ws = true;
+  }
+  if (precision == BOSSS_FP32 && !x->A32) {
+    XCHK(alloc(x, &x->A32, x->nentries)); XCHK(alloc(x, &x->Dinv32, x->ndiag));
+    XCHK(alloc(x, &x->b32, n)); XCHK(alloc(x, &x->r32, n)); XCHK(alloc(x, &x->z32, n)); XCHK(alloc(x, &x->p32, n)); XCHK(alloc(x, &x->q32, n)); XCHK(alloc(x, &x->x32, n));
+  }
+  // block-Jacobi: inverse diagonal blocks (batched uneven LU + DGETRI)
+  hipLaunchKernelGGL(k_gather_diag, dim3(x->a.J), dim3(256), 0, s, x->a.J, x->a.S, x->d_nrows, x->d_boff, x->d_doff, blocks, x->Dinv);
+  int nmax = 2 * x->a.N;
+  int rc = bosss_hip_block_lu_dev(x->ctx, x->a.J, nmax, x->d_nrows, x->d_doff, x->Dinv, nullptr, nullptr, x->info, 1, (void*)s);
+  if (rc != BOSSS_OK) return xfail(rc, "block LU: %s", bosss_hip_block_last_error());
+  {
+    std::vector<int32_t> info(x->a.J);
+    XHIP(hipMemcpyAsync(info.data(), x->info, info.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    XHIP(hipStreamSynchronize(s));
+    for (int j = 0; j < x->a.J; j++)
+      if (info[j] != 0) return info[j] > 0 ? j + 1 : xfail(BOSSS_ERR_ARG, "block %d larger than %d", j, nmax);
+  }
+  if (precision == BOSSS_FP64) {
+    *outer = 1;
+    return pcg_solve<double>(x, 0, blocks, x->Dinv, b, sol, x->r64, x->z64, x->p64, x->q64, tol, maxit, check_every, use_graph, iterations, relres);
+  }
+  // FP32 PCG inside FP64 iterative refinement: r = b - A x (FP64), solve A d = r/|r| in FP32, x += |r| d
+  hipLaunchKernelGGL(k_to_float, dim3(grid_for(x->nentries)), dim3(256), 0, s, x->nentries, blocks, x->A32);
+  hipLaunchKernelGGL(k_to_float, dim3(grid_for(x->ndiag)), dim3(256), 0, s, x->ndiag, x->Dinv, x->Dinv32);
+  double h[S_NSLOT];
+  launch_dot<double>(x, n, b, b, 0, S_BB, -1, s);
+  XHIP(hipMemcpyAsync(h, x->scal, sizeof h, hipMemcpyDeviceToHost, s));
+  XHIP(hipStreamSynchronize(s));
+  const double bb = h[S_BB];
+  int total = 0, k = 0;
+  double rel = 0.0;
+  const double inner_tol = 1e-5;
+  for (k = 0; k < 40; k++) {
+    hipLaunchKernelGGL((k_scale_copy<double, double>), dim3(grid_for(n)), dim3(256), 0, s, n, 1.0, b, x->r64, 0);
+    launch_spmv<double>(x, blocks, -1.0, sol, 1.0, x->r64, s);
+    launch_dot<double>(x, n, x->r64, x->r64, 0, S_RR, -1, s);
+    XHIP(hipMemcpyAsync(h, x->scal, sizeof h, hipMemcpyDeviceToHost, s));
+    XHIP(hipStreamSynchronize(s));
+    rel = bb > 0 ? std::sqrt(h[S_RR] / bb) : 0.0;
+    if (!std::isfinite(rel)) return xfail(BOSSS_ERR_NONFINITE, "refinement residual is not finite");
+    if (rel <= tol || total >= maxit) break;
+    const double rn = std::sqrt(h[S_RR]);
+    hipLaunchKernelGGL((k_scale_copy<double, float>), dim3(grid_for(n)), dim3(256), 0, s, n, 1.0 / rn, x->r64, x->b32, 0);
+    hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n)), dim3(256), 0, s, n, x->x32, 0.0f);
+    int it = 0;
+    double r_in = 0.0;
+    rc = pcg_solve<float>(x, 1, x->A32, x->Dinv32, x->b32, x->x32, x->r32, x->z32, x->p32, x->q32, inner_tol,
+                          std::max(1, maxit - total), check_every, use_graph, &it, &r_in);
+    if (rc != BOSSS_OK) return rc;
+    total += it;
+    hipLaunchKernelGGL((k_scale_copy<float, double>), dim3(grid_for(n)), dim3(256), 0, s, n, rn, x->x32, sol, 1);
+  }
+  *iterations = total;
+  *outer = k;
+  *relres = rel;
+  XHIP(hipGetLastError());
+  return BOSSS_OK;
+}
+
+}  // extern "C"

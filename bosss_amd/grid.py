@@ -17,26 +17,36 @@ _FACE = {(0, 0): 0, (0, 1): 1, (1, 0): 3, (1, 1): 2, (2, 0): 5, (2, 1): 4}
 
 
 class CartesianGrid:
-    def __init__(self, ncells, lo=None, hi=None, periodic=None):
-        self.ncells = tuple(int(n) for n in ncells)
+    """Tensor-product grid: uniform (ncells, lo, hi) or from explicit node coordinates per
+    direction (``nodes``, as Cartesian2DGrid/Cartesian3DGrid take them)."""
+
+    def __init__(self, ncells=None, lo=None, hi=None, periodic=None, nodes=None):
+        if nodes is None:
+            D = len(ncells)
+            lo = np.zeros(D) if lo is None else np.asarray(lo, float)
+            hi = np.ones(D) if hi is None else np.asarray(hi, float)
+            nodes = [np.linspace(lo[d], hi[d], int(ncells[d]) + 1) for d in range(D)]
+        self.nodes = [np.asarray(x, float) for x in nodes]
+        self.ncells = tuple(len(x) - 1 for x in self.nodes)
         D = self.D = len(self.ncells)
         assert D in (2, 3)
-        self.lo = np.zeros(D) if lo is None else np.asarray(lo, float)
-        self.hi = np.ones(D) if hi is None else np.asarray(hi, float)
+        self.lo = np.array([x[0] for x in self.nodes])
+        self.hi = np.array([x[-1] for x in self.nodes])
         self.periodic = (False,) * D if periodic is None else tuple(bool(p) for p in periodic)
         n = np.array(self.ncells)
         self.J = int(np.prod(n))
-        h = (self.hi - self.lo) / n
-        self.h = h
         idx = np.indices(self.ncells).reshape(D, -1).T      # x slowest, last index fastest
-        self.centers = self.lo + (idx + 0.5) * h
-        dj = np.prod(h / 2)
-        self.detJ = np.full(self.J, dj)
+        self.cell_lo = np.stack([self.nodes[d][idx[:, d]] for d in range(D)], 1)
+        self.cell_hi = np.stack([self.nodes[d][idx[:, d] + 1] for d in range(D)], 1)
+        hc = self.cell_hi - self.cell_lo                      # [J, D]
+        self.hc = hc
+        self.centers = 0.5 * (self.cell_lo + self.cell_hi)
+        self.detJ = np.prod(hc / 2, axis=1)
         self.invJ = np.zeros((self.J, D, D))
         for d in range(D):
-            self.invJ[:, d, d] = 2.0 / h[d]
-        self.scaling = np.full(self.J, 1.0 / np.sqrt(dj))
-        farea = np.array([np.prod(np.delete(h, d)) for d in range(D)])
+            self.invJ[:, d, d] = 2.0 / hc[:, d]
+        self.scaling = 1.0 / np.sqrt(self.detJ)
+        farea = np.stack([np.prod(np.delete(hc, d, axis=1), axis=1) for d in range(D)], 1)   # [J, D]
         strides = np.array([int(np.prod(n[d + 1:])) for d in range(D)])
         cells, trafo, normals, sqrtg = [], [], [], []
         bnd_area = np.zeros(self.J)
@@ -54,7 +64,7 @@ class CartesianGrid:
             trafo.append(np.tile([_FACE[(d, 1)], _FACE[(d, 0)]], (len(ci), 1)))
             nd = np.zeros((len(ci), D)); nd[:, d] = 1.0
             normals.append(nd)
-            sqrtg.append(np.full(len(ci), farea[d] / 2 ** (D - 1)))
+            sqrtg.append(farea[ci, d] / 2 ** (D - 1))
         for d in range(D):
             if self.periodic[d]:
                 continue
@@ -64,16 +74,16 @@ class CartesianGrid:
                 trafo.append(np.tile([_FACE[(d, side)], -1], (len(sel), 1)))
                 nd = np.zeros((len(sel), D)); nd[:, d] = 1.0 if side else -1.0
                 normals.append(nd)
-                sqrtg.append(np.full(len(sel), farea[d] / 2 ** (D - 1)))
-                bnd_area[sel] += farea[d]
+                sqrtg.append(farea[sel, d] / 2 ** (D - 1))
+                bnd_area[sel] += farea[sel, d]
         self.cells = np.ascontiguousarray(np.concatenate(cells).astype(np.int32))
         self.trafo = np.ascontiguousarray(np.concatenate(trafo).astype(np.int32))
         self.normals = np.ascontiguousarray(np.concatenate(normals))
         self.sqrtgram = np.ascontiguousarray(np.concatenate(sqrtg))
         self.E = len(self.cells)
-        vol = np.prod(h)
-        surf = 2 * farea.sum()
-        self.lenscale = 1.0 / ((surf + bnd_area) * 0.5 / vol)
+        self.volume = np.prod(hc, axis=1)
+        surf = 2 * farea.sum(axis=1)
+        self.lenscale = 2.0 * self.volume / (surf + bnd_area)
 
     def cell_index(self, ijk):
         n = self.ncells

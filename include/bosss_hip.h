@@ -148,6 +148,66 @@ int bosss_hip_block_gemv_dev(bosss_ctx* ctx, int nblk, const int32_t* n, const i
                              const int64_t* xoff, double alpha, const double* x, double beta, double* y, void* stream);
 const char* bosss_hip_block_last_error(void);
 
+/* ---- two-species XDG Poisson (XdgPoisson3 with XLaplace_Bulk + XLaplace_Interface + RHSSource,
+ *      XdgPoisson3Main.cs:148-178, XLaplace.cs:50-373; agglomeration threshold 0) on
+ *      precomputed cut-cell quadrature items (the Saye rules of the level-set tracker,
+ *      computed on the host like the reference does).  Replaces the XDG matrix assembly of
+ *      XEvaluatorLinear (XSpatialOperatorMk2_XEvaluators.cs:48-600), the block-Jacobi
+ *      smoother's block inversion (BlockJacobi.cs:63-197) and the Krylov solve around it.
+ *
+ *      Species: 0 = A (level set < 0), 1 = B.  Cell j holds nspec[j] (1 or 2) species blocks
+ *      of N coefficients, species spec0[j] first (XDGBasis.cs:92-113).  Matrix: block-ELL
+ *      with uneven blocks, S = 1 + 2D slots per cell row (0 diagonal, 1 + f the neighbour
+ *      across local face f, cols[j][s] = -1: none); block (j, s) is row-major
+ *      (nspec[j] N) x (nspec[cols[j][s]] N) at element offset boff[j][s].
+ *      Quadrature nodes are in the reference coordinates of the cell(s) they belong to,
+ *      weights are physical (volume / area), normals unit: face normals IN -> OUT, interface
+ *      normals from A to B.  face_slot[i] = {slot of OUT in IN's row, slot of IN in OUT's row}. ---- */
+typedef struct bosss_xdg bosss_xdg;
+typedef struct {
+  int32_t D, degree, N, J;
+  const int32_t* mode_exponents;   /* [N][D] exponents of the Legendre factors of each mode (Basis order) */
+  const int32_t* nspec;            /* [J] */
+  const int32_t* spec0;            /* [J] */
+  const double* invJ;              /* [J][D][D] */
+  const double* scaling;           /* [J] */
+  const double* lscale;            /* [J][2] species cell length scales (MultiphaseAgglomerator.cs:746-960) */
+  double penalty_base, mu[2], source[2];
+  int32_t S;
+  const int32_t* cols;             /* [J][S] */
+  const int64_t* boff;             /* [J][S] */
+  int32_t nvol;                    /* volume items: (cell, species) */
+  const int32_t *vol_cell, *vol_spec;
+  const int64_t* vol_off;          /* [nvol+1] node offsets */
+  const double *vol_nodes, *vol_w; /* [K][D], [K] */
+  int32_t nface;                   /* face items: (edge, species) */
+  const int32_t *face_cells, *face_slot, *face_spec;   /* [nface][2], [nface][2], [nface] */
+  const int64_t* face_off;
+  const double *face_nodes_in, *face_nodes_out, *face_w, *face_normal, *face_gD;  /* [K][D] x2, [K], [nface][D], [K] (NULL: g_D = 0) */
+  int32_t nifc;                    /* interface items: cut cells */
+  const int32_t* ifc_cell;
+  const int64_t* ifc_off;
+  const double *ifc_nodes, *ifc_w, *ifc_normal;         /* [K][D], [K], [K][D] */
+} bosss_xdg_desc;
+
+int bosss_hip_xdg_create(bosss_ctx* ctx, const bosss_xdg_desc* desc, bosss_xdg** xdg);
+int bosss_hip_xdg_destroy(bosss_xdg* xdg);
+int bosss_hip_xdg_info(bosss_xdg* xdg, int64_t* ndof, int64_t* nentries);
+/* blocks <- M, affine <- b with R(u) = M u + b (device pointers; ComputeMatrix + ComputeAffine) */
+int bosss_hip_xdg_assemble_dev(bosss_xdg* xdg, double* blocks, double* affine, void* stream);
+/* y <- alpha M x + beta y */
+int bosss_hip_xdg_spmv_dev(bosss_xdg* xdg, const double* blocks, double alpha, const double* x, double beta, double* y, void* stream);
+/* Solve M sol = b by block-Jacobi preconditioned CG (diagonal blocks inverted by the batched
+ * uneven block LU), sol holding the initial guess, to ||b - M sol|| <= tol ||b||.
+ * precision BOSSS_FP64: CG in FP64; BOSSS_FP32: CG in FP32 (matrix and preconditioner
+ * rounded once) inside FP64 iterative refinement.  All scalars stay on the device; each
+ * group of check_every iterations is one HIP graph launch when use_graph != 0.
+ * Returns > 0 (the 1-based cell) if a diagonal block is singular. */
+int bosss_hip_xdg_solve_dev(bosss_xdg* xdg, const double* blocks, const double* b, double* sol, double tol, int32_t maxit,
+                            int32_t precision, int32_t use_graph, int32_t check_every, int32_t* iterations, int32_t* outer,
+                            double* relres, void* stream);
+const char* bosss_hip_xdg_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

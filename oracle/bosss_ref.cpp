@@ -1134,4 +1134,149 @@ int bref_block_solve(int nblk, const int* n, const int64_t* off, const double* L
   return 0;
 }
 
+
+// ------------------------------------------------------------------ XDG Poisson
+// Two-species XDG Poisson operator, agglomeration threshold 0, from precomputed cut-cell
+// quadrature items.  Restates
+//   XLaplace_Bulk (XLaplace.cs:50-200): SIPLaplace volume with Nu = species mu
+//     (ipLaplace.cs:183-189); inner edges with mean weights 1/2 and the penalty
+//     penalty_base * max(1/LS_in, 1/LS_out) * mu_species * penalty_deg;
+//     boundary edges through SIPLaplace.BoundaryEdgeForm (ipLaplace.cs:221-249) with the
+//     base penalty 2 * penalty_base * penalty_deg / LS_in, Dirichlet g_D;
+//   XLaplace_Interface, mode SIP (XLaplace.cs:205-373): weights 1/2, penalty
+//     penalty_base * max(1/LS_A, 1/LS_B) * max(|muA|,|muB|) sign(muA) * penalty_deg;
+//   RHSSource (XdgPoisson3Main.cs:328-358): -rhs * v.
+// LS are the species cell length scales, penalty_deg = max((p+1)^2, (p+1)(p+D)/D).
+// Layout: cell j holds nspec[j] species blocks of N coefficients, species spec0[j] first;
+// blocks (j, slot) are row-major n_j x n_col at boff[j*S + slot] (slot 0 = diagonal).
+int bref_xdg_assemble(void* h, int J, const int* nspec, const int* spec0, const double* invJ, const double* scal,
+                      const double* lscale, double penalty_base, const double* mu, const double* src,
+                      int S, const int* cols, const int64_t* boff,
+                      int nvol, const int* vcell, const int* vspec, const int64_t* voff, const double* vnodes, const double* vw,
+                      int nface, const int* fcells, const int* fslot, const int* fspec, const int64_t* foff,
+                      const double* fnin, const double* fnout, const double* fw, const double* fnormal, const double* gD,
+                      int nifc, const int* icell, const int64_t* ioff, const double* inodes, const double* iw, const double* inormal,
+                      double* blocks, double* affine) {
+  Model* m = (Model*)h;
+  const int D = m->D, N = m->N;
+  const double pdeg = sip_penalty_deg(m->p, D);
+  std::vector<int64_t> dof(J + 1, 0);
+  for (int j = 0; j < J; j++) dof[j + 1] = dof[j] + (int64_t)nspec[j] * N;
+  auto so = [&](int j, int sp) { return (spec0[j] == sp) ? 0 : N; };
+  auto nrow = [&](int j) { return nspec[j] * N; };
+  // physical basis values / gradients of cell j at reference point xi
+  auto basis = [&](int j, const double* xi, double* v, double* g) {
+    for (int n = 0; n < N; n++) {
+      const Poly& P = m->T.polys[D][n];
+      v[n] = scal[j] * poly_eval(P, xi, D);
+      double gr[3];
+      for (int e = 0; e < D; e++) gr[e] = poly_deriv(P, xi, D, e);
+      for (int d = 0; d < D; d++) {
+        double acc = 0;
+        for (int e = 0; e < D; e++) acc += gr[e] * invJ[(size_t)j * D * D + e * D + d];
+        g[n * D + d] = scal[j] * acc;
+      }
+    }
+  };
+  auto blk = [&](int j, int slot, int r, int c) -> double& {
+    int ncol = nrow(cols[j * S + slot]);
+    return blocks[boff[(size_t)j * S + slot] + (int64_t)r * ncol + c];
+  };
+  std::vector<double> va(N), ga(N * D), vb(N), gb(N * D);
+  // volume: -mu grad u . grad v - rhs v
+  for (int it = 0; it < nvol; it++) {
+    int j = vcell[it], sp = vspec[it], o = so(j, sp);
+    for (int64_t q = voff[it]; q < voff[it + 1]; q++) {
+      basis(j, &vnodes[q * D], va.data(), ga.data());
+      double w = vw[q];
+      for (int r = 0; r < N; r++) {
+        for (int c = 0; c < N; c++) {
+          double acc = 0;
+          for (int d = 0; d < D; d++) acc -= ga[c * D + d] * ga[r * D + d] * mu[sp];
+          blk(j, 0, o + r, o + c) += w * acc;
+        }
+        affine[dof[j] + o + r] += w * (-src[sp] * va[r]);
+      }
+    }
+  }
+  // faces
+  for (int it = 0; it < nface; it++) {
+    int ji = fcells[2 * it], jo = fcells[2 * it + 1], sp = fspec[it];
+    const double* n = &fnormal[(size_t)it * D];
+    double muS = mu[sp];
+    for (int64_t q = foff[it]; q < foff[it + 1]; q++) {
+      double w = fw[q];
+      basis(ji, &fnin[q * D], va.data(), ga.data());
+      if (jo >= 0) {
+        basis(jo, &fnout[q * D], vb.data(), gb.data());
+        double eta = penalty_base * std::max(1.0 / lscale[2 * ji + sp], 1.0 / lscale[2 * jo + sp]) * muS * pdeg;
+        int oi = so(ji, sp), oo = so(jo, sp);
+        // test (side, mode r), trial (side, mode c); form bilinear in (uA, uB) x (vA, vB)
+        for (int ts = 0; ts < 2; ts++)
+          for (int us = 0; us < 2; us++)
+            for (int r = 0; r < N; r++)
+              for (int c = 0; c < N; c++) {
+                double vA = ts == 0 ? va[r] : 0.0, vB = ts == 1 ? vb[r] : 0.0;
+                double uA = us == 0 ? va[c] : 0.0, uB = us == 1 ? vb[c] : 0.0;
+                double Acc = 0;
+                for (int d = 0; d < D; d++) {
+                  double guA = us == 0 ? ga[c * D + d] : 0.0, guB = us == 1 ? gb[c * D + d] : 0.0;
+                  double gvA = ts == 0 ? ga[r * D + d] : 0.0, gvB = ts == 1 ? gb[r * D + d] : 0.0;
+                  Acc += (0.5 * muS * guA + 0.5 * muS * guB) * (vA - vB) * n[d];
+                  Acc += (0.5 * muS * gvA + 0.5 * muS * gvB) * (uA - uB) * n[d];
+                }
+                Acc -= (uA - uB) * (vA - vB) * eta;
+                int jr = ts == 0 ? ji : jo, or_ = ts == 0 ? oi : oo, oc = us == 0 ? oi : oo;
+                int slot = (ts == us) ? 0 : (ts == 0 ? fslot[2 * it] : fslot[2 * it + 1]);
+                blk(jr, slot, or_ + r, oc + c) += w * Acc;
+              }
+      } else {
+        double pn = 2.0 * penalty_base * pdeg / lscale[2 * ji + sp];
+        int oi = so(ji, sp);
+        double g = gD ? gD[q] : 0.0;
+        for (int r = 0; r < N; r++) {
+          double gvn = 0;
+          for (int d = 0; d < D; d++) gvn += ga[r * D + d] * n[d];
+          for (int c = 0; c < N; c++) {
+            double gun = 0;
+            for (int d = 0; d < D; d++) gun += ga[c * D + d] * n[d];
+            double Acc = muS * gun * va[r] + muS * gvn * va[c] - muS * va[c] * va[r] * pn;
+            blk(ji, 0, oi + r, oi + c) += w * Acc;
+          }
+          affine[dof[ji] + oi + r] += w * (-muS * gvn * g + muS * g * va[r] * pn);
+        }
+      }
+    }
+  }
+  // interface (species A = 0 negative side, B = 1; normal from A to B)
+  const double muA = mu[0], muB = mu[1];
+  const double pmu = std::max(std::fabs(muA), std::fabs(muB)) * (muA < 0 ? -1.0 : 1.0);
+  for (int it = 0; it < nifc; it++) {
+    int j = icell[it];
+    if (nspec[j] != 2) continue;
+    double eta = penalty_base * std::max(1.0 / lscale[2 * j + 0], 1.0 / lscale[2 * j + 1]) * pmu * pdeg;
+    int oA = so(j, 0), oB = so(j, 1);
+    for (int64_t q = ioff[it]; q < ioff[it + 1]; q++) {
+      double w = iw[q];
+      const double* Nn = &inormal[q * D];
+      basis(j, &inodes[q * D], va.data(), ga.data());
+      for (int ts = 0; ts < 2; ts++)
+        for (int us = 0; us < 2; us++)
+          for (int r = 0; r < N; r++)
+            for (int c = 0; c < N; c++) {
+              double gvn = 0, gun = 0;
+              for (int d = 0; d < D; d++) { gvn += ga[r * D + d] * Nn[d]; gun += ga[c * D + d] * Nn[d]; }
+              double vA = ts == 0 ? va[r] : 0.0, vB = ts == 1 ? va[r] : 0.0;
+              double uA = us == 0 ? va[c] : 0.0, uB = us == 1 ? va[c] : 0.0;
+              double guA = us == 0 ? gun : 0.0, guB = us == 1 ? gun : 0.0;
+              double gvA = ts == 0 ? gvn : 0.0, gvB = ts == 1 ? gvn : 0.0;
+              double Ret = (muA * 0.5 * guA + muB * 0.5 * guB) * (vA - vB) + (muA * 0.5 * gvA + muB * 0.5 * gvB) * (uA - uB);
+              Ret -= eta * (uA - uB) * (vA - vB);
+              blk(j, 0, (ts == 0 ? oA : oB) + r, (us == 0 ? oA : oB) + c) += w * Ret;
+            }
+    }
+  }
+  return 0;
+}
+
 }  // extern "C"

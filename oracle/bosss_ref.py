@@ -183,3 +183,62 @@ def ref_block_solve(sizes, LU, piv, off, poff, rhs):
     x = np.ascontiguousarray(rhs, dtype=np.float64).copy()
     lib.bref_block_solve(len(sizes), _p(n, _ip), _p(off, _lp), _p(LU), _p(piv, _ip), _p(poff, _lp), _p(poff, _lp), _p(x))
     return x
+
+
+def xdg_assemble(rules, p, mu, src, penalty=2.0, gD=None):
+    """Oracle XDG Poisson matrix (block layout of ``rules.block_layout``) and affine vector."""
+    lib = _load()
+    D = rules.D
+    f = lib.bref_xdg_assemble
+    f.restype = ctypes.c_int
+    f.argtypes = ([ctypes.c_void_p, ctypes.c_int, _ip, _ip, _dp, _dp, _dp, ctypes.c_double, _dp, _dp,
+                   ctypes.c_int, _ip, _lp,
+                   ctypes.c_int, _ip, _ip, _lp, _dp, _dp,
+                   ctypes.c_int, _ip, _ip, _ip, _lp, _dp, _dp, _dp, _dp, _dp,
+                   ctypes.c_int, _ip, _lp, _dp, _dp, _dp, _dp, _dp])
+    model = RefModel(D, (1,) * D, p, 2 * p)
+    N = model.N
+    lay = rules.block_layout(N)
+    g = rules.grid
+    blocks = np.zeros(lay["nentries"])
+    affine = np.zeros(lay["ndof"])
+    c = np.ascontiguousarray
+    i32 = lambda a: c(np.asarray(a, np.int32))
+    i64 = lambda a: c(np.asarray(a, np.int64))
+    f64 = lambda a: c(np.asarray(a, np.float64))
+    gd = f64(np.zeros(len(rules.face_w)) if gD is None else gD)
+    keep = [i32(rules.nspec), i32(rules.spec0), f64(g.invJ), f64(g.scaling), f64(rules.lenscale), f64(mu), f64(src),
+            i32(lay["cols"]), i64(lay["boff"]),
+            i32(rules.vol_cell), i32(rules.vol_spec), i64(rules.vol_off), f64(rules.vol_nodes), f64(rules.vol_w),
+            i32(rules.face_cells), i32(lay["fslot"]), i32(rules.face_spec), i64(rules.face_off),
+            f64(rules.face_nodes_in), f64(rules.face_nodes_out), f64(rules.face_w), f64(rules.face_normal), gd,
+            i32(rules.ifc_cell), i64(rules.ifc_off), f64(rules.ifc_nodes), f64(rules.ifc_w), f64(rules.ifc_normal)]
+    k = keep
+    rc = f(model.h, g.J, _p(k[0], _ip), _p(k[1], _ip), _p(k[2]), _p(k[3]), _p(k[4]), float(penalty), _p(k[5]), _p(k[6]),
+           lay["S"], _p(k[7], _ip), _p(k[8], _lp),
+           len(rules.vol_cell), _p(k[9], _ip), _p(k[10], _ip), _p(k[11], _lp), _p(k[12]), _p(k[13]),
+           len(rules.face_spec), _p(k[14], _ip), _p(k[15], _ip), _p(k[16], _ip), _p(k[17], _lp),
+           _p(k[18]), _p(k[19]), _p(k[20]), _p(k[21]), _p(k[22]),
+           len(rules.ifc_cell), _p(k[23], _ip), _p(k[24], _lp), _p(k[25]), _p(k[26]), _p(k[27]),
+           _p(blocks), _p(affine))
+    if rc != 0:
+        raise RuntimeError("bref_xdg_assemble failed")
+    return blocks, affine, lay
+
+
+def xdg_to_csr(rules, lay, blocks):
+    """Scatter the block-ELL matrix into a scipy CSR matrix (tests only)."""
+    import scipy.sparse as sp
+    rows, cols_, vals = [], [], []
+    dof, cl, bo, nr = lay["dof"], lay["cols"], lay["boff"], lay["nrow"]
+    for j in range(rules.J):
+        for s in range(lay["S"]):
+            k = cl[j, s]
+            if k < 0:
+                continue
+            a, b = nr[j], nr[k]
+            B = blocks[bo[j, s]: bo[j, s] + a * b].reshape(a, b)
+            r, cc = np.nonzero(np.ones((a, b), bool))
+            rows.append(dof[j] + r); cols_.append(dof[k] + cc); vals.append(B.reshape(-1))
+    n = lay["ndof"]
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols_))), shape=(n, n))
