@@ -124,6 +124,7 @@ def secondary_workload(args):
     sip2d    SIP Poisson, 64^2 Cartesian, p=2, Dirichlet: residual + assembled block-sparse apply
     advdiff  upwind + SIP (nu 0.01), 64^3 periodic, p=3: matrix-free FP64 and FP32 vs assembled SpMV
     euler32  Euler LF and HLLC, 32^3 periodic, p=2: fused residual + VJP
+    xdg32    XDG Poisson, sphere in 32^3, p=2: cut-cell rules, assembly, apply, block-Jacobi PCG
     """
     import numpy as np
     import torch
@@ -134,7 +135,41 @@ def secondary_workload(args):
     torch.cuda.set_device(dev)
     K, W0 = args.steps, args.warmup
     res = {}
-    if args.workload == "euler32":
+    if args.workload == "xdg32":
+        from bosss_amd.xdg import QuadLevelSet, XdgRules, XdgOperator
+        grid = CartesianGrid((32,) * 3)
+        t0 = time.perf_counter()
+        rules = XdgRules(grid, QuadLevelSet.sphere([0.5] * 3, 0.3), n=3)
+        t_rules = time.perf_counter() - t0
+        mu, src = (-1.0, -10.0), (1.0, 1.0)
+        op = XdgOperator(rules, 2, mu, source=src, penalty=2.0, g_D=None)
+        B = op.new_matrix()
+        a = torch.zeros(op.ndof, dtype=torch.float64, device=dev)
+        ms_asm = _time_ms(lambda: op.assemble_dev(B, a), K, W0)
+        y = torch.zeros_like(a)
+        xr = torch.rand(op.ndof, dtype=torch.float64, device=dev)
+        ms_mv = _time_ms(lambda: op.spmv_dev(B, xr, y), K, W0)
+        b = -a
+        res["rules"] = {"host_seconds": round(t_rules, 3), "cut_cells": int(rules.cut.sum()),
+                        "cells_A_only": int((rules.has[:, 0] & ~rules.cut).sum()), "cells_B_only": int((rules.has[:, 1] & ~rules.cut).sum()),
+                        "volume_nodes": int(len(rules.vol_w)), "face_nodes": int(len(rules.face_w)), "interface_nodes": int(len(rules.ifc_w)),
+                        "min_cut_species_fraction": float(rules.volume_fraction[rules.cut].min())}
+        res["assembly_ms"] = round(ms_asm, 4)
+        res["spmv_ms"] = round(ms_mv, 4)
+        res["matrix_bytes"] = op.nentries * 8
+        for prec, graph in (("fp64", True), ("fp64", False), ("fp32", True)):
+            x = torch.zeros_like(a)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            it, outer, rel = op.solve_dev(B, b, x, tol=1e-10, maxit=50000, precision=prec, use_graph=graph, check_every=16)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            res["pcg_%s_%s" % (prec, "graph" if graph else "eager")] = {
+                "seconds": round(t, 4), "iterations": it, "outer": outer, "relres": rel, "us_per_iteration": round(t / max(it, 1) * 1e6, 2)}
+        dof, desc = op.ndof, ("XDG Poisson, sphere r=0.3 in [0,1]^3, 32^3 cells, p=2, mu_A=-1, mu_B=-10, penalty 2, "
+                              "agglomeration threshold 0: assembly, block-sparse apply, block-Jacobi PCG")
+        op.close()
+    elif args.workload == "euler32":
         grid = CartesianGrid((32,) * 3, periodic=[True] * 3)
         for form in ("euler_lf", "euler_hllc"):
             op = DifferentialOperator(grid, form, 2, gamma=1.4)
@@ -231,7 +266,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--split", action="store_true", help="time separate residual and VJP launches instead of the fused one")
     ap.add_argument("--no-newton", action="store_true", help="skip the implicit-step Newton comparison (FP64 vs mixed)")
-    ap.add_argument("--workload", default="euler64", choices=["euler64", "sip2d", "advdiff", "euler32"],
+    ap.add_argument("--workload", default="euler64", choices=["euler64", "sip2d", "advdiff", "euler32", "xdg32"],
                     help="euler64: the headline line; the others print the secondary configurations")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device copy first (known-size reference for HBM counter readings)")

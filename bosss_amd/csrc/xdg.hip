@@ -7,6 +7,9 @@
 //   XLaplace_Bulk   XLaplace.cs:50-200  (SIPLaplace with Nu = species mu, ipLaplace.cs:183-249)
 //   XLaplace_Interface (mode SIP)  XLaplace.cs:205-373
 //   RHSSource       XdgPoisson3Main.cs:328-358
+// The FP32 solve runs CG on the diagonal-block-equilibrated matrix Linv M Linv^T
+// (D_j = L_j L_j^T), which keeps the FP32 operator positive definite where the
+// unagglomerated cut cells make M itself too ill-conditioned for FP32.
 // Layout: cell j holds nspec[j] species blocks of N coefficients (species spec0[j] first,
 // XDGBasis.cs:92-113); the matrix is block-ELL with uneven blocks: row j, slot s (0 diagonal,
 // 1+f neighbour across local face f) is the row-major n_j x n_col block at boff[j*S+s].
@@ -303,10 +306,6 @@ __global__ void k_fill(int64_t n, T* __restrict__ x, T v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] = v;
 }
 
-__global__ void k_to_float(int64_t n, const double* __restrict__ x, float* __restrict__ y) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = (float)x[i];
-}
-
 // y <- alpha A x + beta y, one wavefront per block row, lane per row of the block
 template <typename Real>
 __global__ void __launch_bounds__(64) k_xdg_spmv(int J, int S, const int32_t* __restrict__ nrows, const int32_t* __restrict__ cols,
@@ -350,6 +349,104 @@ __global__ void __launch_bounds__(64) k_bdiag_apply(int J, const int32_t* __rest
   double acc = 0.0;
   for (int c = 0; c < n; c++) acc += (double)row[c] * (double)rj[c];
   z[dof[j] + i] = (Real)acc;
+}
+
+// Diagonal-block equilibration for the FP32 solve (the reference's SymPart_DiagBlockEquilib
+// pre-preconditioning, MultigridOperator.Mode): D_j = L_j L_j^T (Cholesky), the FP32 CG runs
+// on Linv M Linv^T, whose diagonal blocks are the identity.
+// L <- inv(chol(D)) in place (lower triangular, upper part zero); info[j] = first bad pivot.
+__global__ void __launch_bounds__(64) k_chol_inv(int J, const int32_t* __restrict__ nrows, const int64_t* __restrict__ doff,
+                                                 double* __restrict__ Dg, int32_t* __restrict__ info) {
+  __shared__ double L[40 * 41], X[40 * 41];
+  const int j = blockIdx.x, lane = threadIdx.x;
+  const int n = nrows[j], ld = 41;
+  if (n > 40) { if (lane == 0) info[j] = -1; return; }
+  double* g = Dg + doff[j];
+  for (int i = lane; i < n * n; i += 64) L[(i / n) * ld + (i % n)] = g[i];
+  __syncthreads();
+  int bad = 0;
+  for (int c = 0; c < n; c++) {
+    double d = L[c * ld + c];
+    for (int k = 0; k < c; k++) d -= L[c * ld + k] * L[c * ld + k];
+    if (!(d > 0.0)) { bad = c + 1; break; }
+    const double dc = sqrt(d);
+    __syncthreads();
+    for (int i = lane; i < n; i += 64) {
+      if (i > c) {
+        double v = L[i * ld + c];
+        for (int k = 0; k < c; k++) v -= L[i * ld + k] * L[c * ld + k];
+        L[i * ld + c] = v / dc;
+      }
+    }
+    __syncthreads();
+    if (lane == 0) L[c * ld + c] = dc;
+    __syncthreads();
+  }
+  if (bad) { if (lane == 0) info[j] = bad; return; }
+  // X = L^-1: lane c owns column c
+  for (int c = lane; c < n; c += 64)
+    for (int i = 0; i < n; i++) {
+      double v = (i == c) ? 1.0 : 0.0;
+      if (i < c) { X[i * ld + c] = 0.0; continue; }
+      for (int k = c; k < i; k++) v -= L[i * ld + k] * X[k * ld + c];
+      X[i * ld + c] = v / L[i * ld + i];
+    }
+  __syncthreads();
+  for (int i = lane; i < n * n; i += 64) g[i] = X[(i / n) * ld + (i % n)];
+  if (lane == 0) info[j] = 0;
+}
+
+// A32(j, s) = (float)(Linv_j A(j, s) Linv_k^T), one workgroup per block
+__global__ void __launch_bounds__(64) k_scale_blocks(int J, int S, const int32_t* __restrict__ nrows, const int32_t* __restrict__ cols,
+                                                     const int64_t* __restrict__ boff, const int64_t* __restrict__ doff,
+                                                     const double* __restrict__ Linv, const double* __restrict__ A,
+                                                     float* __restrict__ A32) {
+  __shared__ double Lj[40 * 40], Lk[40 * 40], B[40 * 40], T[40 * 40];
+  const int j = blockIdx.x / S, s = blockIdx.x % S, lane = threadIdx.x;
+  const int k = cols[j * S + s];
+  if (k < 0) return;
+  const int n = nrows[j], m = nrows[k];
+  const int64_t o = boff[(size_t)j * S + s];
+  for (int i = lane; i < n * n; i += 64) Lj[i] = Linv[doff[j] + i];
+  for (int i = lane; i < m * m; i += 64) Lk[i] = Linv[doff[k] + i];
+  for (int i = lane; i < n * m; i += 64) B[i] = A[o + i];
+  __syncthreads();
+  for (int i = lane; i < n * m; i += 64) {          // T = B Lk^T
+    const int r = i / m, c = i % m;
+    double v = 0.0;
+    for (int l = 0; l <= c; l++) v += B[r * m + l] * Lk[c * m + l];
+    T[i] = v;
+  }
+  __syncthreads();
+  for (int i = lane; i < n * m; i += 64) {          // Lj T
+    const int r = i / m, c = i % m;
+    double v = 0.0;
+    for (int l = 0; l <= r; l++) v += Lj[r * n + l] * T[l * m + c];
+    A32[o + i] = (float)v;
+  }
+}
+
+// out_j = scale * (Linv_j or Linv_j^T) in_j  (+ out_j if accumulate)
+template <typename In, typename Out>
+__global__ void __launch_bounds__(64) k_lower_apply(int J, const int32_t* __restrict__ nrows, const int64_t* __restrict__ doff,
+                                                    const int64_t* __restrict__ dof, const double* __restrict__ Linv, int transpose,
+                                                    double scale, const In* __restrict__ in, Out* __restrict__ out, int accumulate) {
+  const int j = blockIdx.x, i = threadIdx.x;
+  const int n = nrows[j];
+  if (i >= n) return;
+  const double* L = Linv + doff[j];
+  const In* v = in + dof[j];
+  double acc = 0.0;
+  if (!transpose) { for (int c = 0; c <= i; c++) acc += L[i * n + c] * (double)v[c]; }
+  else { for (int c = i; c < n; c++) acc += L[c * n + i] * (double)v[c]; }
+  Out* o = out + dof[j] + i;
+  *o = (Out)((accumulate ? (double)*o : 0.0) + scale * acc);
+}
+
+template <typename Real>
+__global__ void k_identity_blocks(int J, const int32_t* __restrict__ nrows, const int64_t* __restrict__ doff, Real* __restrict__ Dg) {
+  const int j = blockIdx.x, n = nrows[j];
+  for (int i = threadIdx.x; i < n * n; i += blockDim.x) Dg[doff[j] + i] = Real((i / n) == (i % n) ? 1 : 0);
 }
 
 // ------------------------------------------------------------ reductions
@@ -448,7 +545,7 @@ struct bosss_xdg {
   double *v_nodes, *v_w, *f_nin, *f_nout, *f_w, *f_normal, *f_gD, *i_nodes, *i_w, *i_normal;
   // solver workspace (allocated on first solve)
   bool ws = false;
-  double *Dinv = nullptr, *scal = nullptr, *partial = nullptr;
+  double *Dinv = nullptr, *Lchol = nullptr, *scal = nullptr, *partial = nullptr;
   int32_t* info = nullptr;
   double *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *q64 = nullptr;
   float *A32 = nullptr, *Dinv32 = nullptr, *b32 = nullptr, *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
@@ -535,7 +632,7 @@ void pcg_start(bosss_xdg* X, const Real* A, const Real* Dinv, const Real* b, Rea
 // Run PCG until sqrt(rr/bb) <= tol or maxit; K iterations per check (one graph launch).
 template <typename Real>
 int pcg_solve(bosss_xdg* X, int gslot, const Real* A, const Real* Dinv, const Real* b, Real* x, Real* r, Real* z, Real* p, Real* q,
-              double tol, int maxit, int K, int use_graph, int* iters, double* relres) {
+              double tol, int maxit, int K, int use_graph, bool stop_on_stall, int* iters, double* relres) {
   hipStream_t s = X->stream;
   pcg_start<Real>(X, A, Dinv, b, x, r, z, p, s);
   double h[S_NSLOT];
@@ -563,7 +660,14 @@ int pcg_solve(bosss_xdg* X, int gslot, const Real* A, const Real* Dinv, const Re
   }
   // The recurrence residual drifts from b - A x in finite precision: on convergence the true
   // residual is recomputed (pcg_start) and CG restarted from the current iterate if it misses.
-  for (int restart = 0; restart < 8; restart++) {
+  // An inner (FP32) solve that stops improving -- the attainable accuracy of the working
+  // precision -- ends early: no 10 % gain on the best residual over `stall` consecutive checks
+  // (>= 512 iterations; the CG residual norm is not monotone).
+  const int stall = std::max(8, 512 / K);
+  bool stalled = false;
+  for (int restart = 0; restart < 8 && !stalled; restart++) {
+    double best = rel;
+    int since = 0;
     while (rel > tol && it < maxit) {
       if (use_graph) {
         XHIP(hipGraphLaunch(X->graph[gslot], s));
@@ -575,6 +679,8 @@ int pcg_solve(bosss_xdg* X, int gslot, const Real* A, const Real* Dinv, const Re
       XHIP(hipStreamSynchronize(s));
       rel = std::sqrt(std::max(h[S_RR], 0.0) / bb);
       if (!std::isfinite(rel)) return xfail(BOSSS_ERR_NONFINITE, "PCG residual is not finite after %d iterations", it);
+      if (rel < 0.9 * best) { best = rel; since = 0; }
+      else if (stop_on_stall && ++since >= stall) { stalled = true; break; }
     }
     if (it == 0) break;
     pcg_start<Real>(X, A, Dinv, b, x, r, z, p, s);        // true residual r = b - A x
@@ -758,11 +864,24 @@ int bosss_hip_xdg_solve_dev(bosss_xdg* x, const double* blocks, const double* b,
   }
   if (precision == BOSSS_FP64) {
     *outer = 1;
-    return pcg_solve<double>(x, 0, blocks, x->Dinv, b, sol, x->r64, x->z64, x->p64, x->q64, tol, maxit, check_every, use_graph, iterations, relres);
+    return pcg_solve<double>(x, 0, blocks, x->Dinv, b, sol, x->r64, x->z64, x->p64, x->q64, tol, maxit, check_every, use_graph, false,
+                             iterations, relres);
   }
-  // FP32 PCG inside FP64 iterative refinement: r = b - A x (FP64), solve A d = r/|r| in FP32, x += |r| d
-  hipLaunchKernelGGL(k_to_float, dim3(grid_for(x->nentries)), dim3(256), 0, s, x->nentries, blocks, x->A32);
-  hipLaunchKernelGGL(k_to_float, dim3(grid_for(x->ndiag)), dim3(256), 0, s, x->ndiag, x->Dinv, x->Dinv32);
+  // FP32 CG on the equilibrated matrix Linv M Linv^T inside FP64 iterative refinement:
+  // r = b - M x (FP64), solve (Linv M Linv^T) y = Linv r / |r| in FP32, x += |r| Linv^T y.
+  if (!x->Lchol) XCHK(alloc(x, &x->Lchol, x->ndiag));
+  hipLaunchKernelGGL(k_gather_diag, dim3(x->a.J), dim3(256), 0, s, x->a.J, x->a.S, x->d_nrows, x->d_boff, x->d_doff, blocks, x->Lchol);
+  hipLaunchKernelGGL(k_chol_inv, dim3(x->a.J), dim3(64), 0, s, x->a.J, x->d_nrows, x->d_doff, x->Lchol, x->info);
+  {
+    std::vector<int32_t> info(x->a.J);
+    XHIP(hipMemcpyAsync(info.data(), x->info, info.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    XHIP(hipStreamSynchronize(s));
+    for (int j = 0; j < x->a.J; j++)
+      if (info[j] != 0) return xfail(BOSSS_ERR_ARG, "diagonal block %d is not symmetric positive definite (pivot %d)", j, info[j]);
+  }
+  hipLaunchKernelGGL(k_scale_blocks, dim3(x->a.J * x->a.S), dim3(64), 0, s, x->a.J, x->a.S, x->d_nrows, x->d_cols, x->d_boff, x->d_doff,
+                     x->Lchol, blocks, x->A32);
+  hipLaunchKernelGGL(k_identity_blocks<float>, dim3(x->a.J), dim3(256), 0, s, x->a.J, x->d_nrows, x->d_doff, x->Dinv32);
   double h[S_NSLOT];
   launch_dot<double>(x, n, b, b, 0, S_BB, -1, s);
   XHIP(hipMemcpyAsync(h, x->scal, sizeof h, hipMemcpyDeviceToHost, s));
@@ -770,7 +889,7 @@ int bosss_hip_xdg_solve_dev(bosss_xdg* x, const double* blocks, const double* b,
   const double bb = h[S_BB];
   int total = 0, k = 0;
   double rel = 0.0;
-  const double inner_tol = 1e-5;
+  const double inner_tol = 1e-4;
   for (k = 0; k < 40; k++) {
     hipLaunchKernelGGL((k_scale_copy<double, double>), dim3(grid_for(n)), dim3(256), 0, s, n, 1.0, b, x->r64, 0);
     launch_spmv<double>(x, blocks, -1.0, sol, 1.0, x->r64, s);
@@ -781,15 +900,17 @@ int bosss_hip_xdg_solve_dev(bosss_xdg* x, const double* blocks, const double* b,
     if (!std::isfinite(rel)) return xfail(BOSSS_ERR_NONFINITE, "refinement residual is not finite");
     if (rel <= tol || total >= maxit) break;
     const double rn = std::sqrt(h[S_RR]);
-    hipLaunchKernelGGL((k_scale_copy<double, float>), dim3(grid_for(n)), dim3(256), 0, s, n, 1.0 / rn, x->r64, x->b32, 0);
+    hipLaunchKernelGGL((k_lower_apply<double, float>), dim3(x->a.J), dim3(64), 0, s, x->a.J, x->d_nrows, x->d_doff, x->d_dof, x->Lchol, 0,
+                       1.0 / rn, x->r64, x->b32, 0);
     hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n)), dim3(256), 0, s, n, x->x32, 0.0f);
     int it = 0;
     double r_in = 0.0;
     rc = pcg_solve<float>(x, 1, x->A32, x->Dinv32, x->b32, x->x32, x->r32, x->z32, x->p32, x->q32, inner_tol,
-                          std::max(1, maxit - total), check_every, use_graph, &it, &r_in);
+                          std::max(1, maxit - total), check_every, use_graph, true, &it, &r_in);
     if (rc != BOSSS_OK) return rc;
     total += it;
-    hipLaunchKernelGGL((k_scale_copy<float, double>), dim3(grid_for(n)), dim3(256), 0, s, n, rn, x->x32, sol, 1);
+    hipLaunchKernelGGL((k_lower_apply<float, double>), dim3(x->a.J), dim3(64), 0, s, x->a.J, x->d_nrows, x->d_doff, x->d_dof, x->Lchol, 1,
+                       rn, x->x32, sol, 1);
   }
   *iterations = total;
   *outer = k;
