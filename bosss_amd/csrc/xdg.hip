@@ -306,15 +306,16 @@ __global__ void k_fill(int64_t n, T* __restrict__ x, T v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] = v;
 }
 
-// y <- alpha A x + beta y, one wavefront per block row, lane per row of the block
+// y <- alpha A x + beta y, one thread per matrix row (rowcell: the cell of each row)
 template <typename Real>
-__global__ void __launch_bounds__(64) k_xdg_spmv(int J, int S, const int32_t* __restrict__ nrows, const int32_t* __restrict__ cols,
-                                                 const int64_t* __restrict__ boff, const int64_t* __restrict__ dof,
-                                                 const Real* __restrict__ A, double alpha, const Real* __restrict__ x, double beta,
-                                                 Real* __restrict__ y) {
-  const int j = blockIdx.x, r = threadIdx.x;
-  const int n = nrows[j];
-  if (r >= n) return;
+__global__ void __launch_bounds__(256) k_xdg_spmv(int64_t ndof, int S, const int32_t* __restrict__ rowcell, const int32_t* __restrict__ nrows,
+                                                  const int32_t* __restrict__ cols, const int64_t* __restrict__ boff,
+                                                  const int64_t* __restrict__ dof, const Real* __restrict__ A, double alpha,
+                                                  const Real* __restrict__ x, double beta, Real* __restrict__ y) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ndof) return;
+  const int j = rowcell[t];
+  const int r = (int)(t - dof[j]);
   double acc = 0.0;
   for (int s = 0; s < S; s++) {
     const int k = cols[j * S + s];
@@ -324,8 +325,7 @@ __global__ void __launch_bounds__(64) k_xdg_spmv(int J, int S, const int32_t* __
     const Real* xk = x + dof[k];
     for (int c = 0; c < m; c++) acc += (double)row[c] * (double)xk[c];
   }
-  Real* yr = y + dof[j] + r;
-  *yr = (Real)(alpha * acc + (beta != 0.0 ? beta * (double)*yr : 0.0));
+  y[t] = (Real)(alpha * acc + (beta != 0.0 ? beta * (double)y[t] : 0.0));
 }
 
 // gather the diagonal blocks into a compact batch (offsets doff) for the block LU
@@ -336,14 +336,16 @@ __global__ void k_gather_diag(int J, int S, const int32_t* __restrict__ nrows, c
   for (int i = threadIdx.x; i < n * n; i += blockDim.x) Dg[doff[j] + i] = A[boff[(size_t)j * S] + i];
 }
 
-// z_j <- Dinv_j r_j
+// z_j <- Dinv_j r_j, one thread per row
 template <typename Real>
-__global__ void __launch_bounds__(64) k_bdiag_apply(int J, const int32_t* __restrict__ nrows, const int64_t* __restrict__ doff,
-                                                    const int64_t* __restrict__ dof, const Real* __restrict__ Dinv,
-                                                    const Real* __restrict__ r, Real* __restrict__ z) {
-  const int j = blockIdx.x, i = threadIdx.x;
+__global__ void __launch_bounds__(256) k_bdiag_apply(int64_t ndof, const int32_t* __restrict__ rowcell, const int32_t* __restrict__ nrows,
+                                                     const int64_t* __restrict__ doff, const int64_t* __restrict__ dof,
+                                                     const Real* __restrict__ Dinv, const Real* __restrict__ r, Real* __restrict__ z) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ndof) return;
+  const int j = rowcell[t];
   const int n = nrows[j];
-  if (i >= n) return;
+  const int i = (int)(t - dof[j]);
   const Real* row = Dinv + doff[j] + (int64_t)i * n;
   const Real* rj = r + dof[j];
   double acc = 0.0;
@@ -537,7 +539,7 @@ struct bosss_xdg {
   int nvol = 0, nface = 0, nifc = 0;
   int64_t ndof = 0, nentries = 0, ndiag = 0;
   std::vector<void*> allocs;
-  int32_t *d_nspec, *d_spec0, *d_expo, *d_cols, *d_nrows;
+  int32_t *d_nspec, *d_spec0, *d_expo, *d_cols, *d_nrows, *d_rowcell;
   int64_t *d_boff, *d_dof, *d_doff;
   double *d_invJ, *d_scal, *d_ls;
   int32_t *v_cell, *v_spec, *f_cells, *f_slot, *f_spec, *i_cell;
@@ -600,8 +602,8 @@ void launch_dot(bosss_xdg* X, int64_t n, const Real* x, const Real* y, int want2
 
 template <typename Real>
 void launch_spmv(bosss_xdg* X, const Real* A, double alpha, const Real* x, double beta, Real* y, hipStream_t s) {
-  hipLaunchKernelGGL(k_xdg_spmv<Real>, dim3(X->a.J), dim3(64), 0, s, X->a.J, X->a.S, X->d_nrows, X->d_cols, X->d_boff, X->d_dof, A,
-                     alpha, x, beta, y);
+  hipLaunchKernelGGL(k_xdg_spmv<Real>, dim3((unsigned)((X->ndof + 255) / 256)), dim3(256), 0, s, X->ndof, X->a.S, X->d_rowcell,
+                     X->d_nrows, X->d_cols, X->d_boff, X->d_dof, A, alpha, x, beta, y);
 }
 
 // one PCG iteration (Real = double or float), all scalars on the device
@@ -611,7 +613,8 @@ void pcg_iteration(bosss_xdg* X, const Real* A, const Real* Dinv, Real* x, Real*
   launch_spmv<Real>(X, A, 1.0, p, 0.0, q, s);
   launch_dot<Real>(X, n, p, q, 0, S_PQ, -1, s);
   hipLaunchKernelGGL(k_pcg_xr<Real>, dim3(grid_for(n)), dim3(256), 0, s, n, X->scal, p, q, x, r);
-  hipLaunchKernelGGL(k_bdiag_apply<Real>, dim3(X->a.J), dim3(64), 0, s, X->a.J, X->d_nrows, X->d_doff, X->d_dof, Dinv, r, z);
+  hipLaunchKernelGGL(k_bdiag_apply<Real>, dim3((unsigned)((X->ndof + 255) / 256)), dim3(256), 0, s, X->ndof, X->d_rowcell, X->d_nrows,
+                     X->d_doff, X->d_dof, Dinv, r, z);
   launch_dot<Real>(X, n, r, z, 1, S_RZN, S_RR, s);
   hipLaunchKernelGGL(k_pcg_p<Real>, dim3(grid_for(n)), dim3(256), 0, s, n, X->scal, z, p);
   hipLaunchKernelGGL(k_scal_copy, dim3(1), dim3(64), 0, s, X->scal, (int)S_RZ, (int)S_RZN);
@@ -623,7 +626,8 @@ void pcg_start(bosss_xdg* X, const Real* A, const Real* Dinv, const Real* b, Rea
   const int64_t n = X->ndof;
   hipLaunchKernelGGL((k_scale_copy<Real, Real>), dim3(grid_for(n)), dim3(256), 0, s, n, 1.0, b, r, 0);
   launch_spmv<Real>(X, A, -1.0, x, 1.0, r, s);
-  hipLaunchKernelGGL(k_bdiag_apply<Real>, dim3(X->a.J), dim3(64), 0, s, X->a.J, X->d_nrows, X->d_doff, X->d_dof, Dinv, r, z);
+  hipLaunchKernelGGL(k_bdiag_apply<Real>, dim3((unsigned)((X->ndof + 255) / 256)), dim3(256), 0, s, X->ndof, X->d_rowcell, X->d_nrows,
+                     X->d_doff, X->d_dof, Dinv, r, z);
   hipLaunchKernelGGL((k_scale_copy<Real, Real>), dim3(grid_for(n)), dim3(256), 0, s, n, 1.0, z, p, 0);
   launch_dot<Real>(X, n, r, z, 1, S_RZ, S_RR, s);
   launch_dot<Real>(X, n, b, b, 0, S_BB, -1, s);
@@ -760,7 +764,11 @@ int bosss_hip_xdg_create(bosss_ctx* ctx, const bosss_xdg_desc* d, bosss_xdg** ou
 #define UP(dst, src, n) if (rc == BOSSS_OK) rc = upload(x, &x->dst, src, n)
   UP(d_nspec, d->nspec, J); UP(d_spec0, d->spec0, J); UP(d_expo, d->mode_exponents, (size_t)N * D);
   UP(d_cols, d->cols, (size_t)J * S); UP(d_boff, d->boff, (size_t)J * S);
+  std::vector<int32_t> rowcell(dof[J]);
+  for (int j = 0; j < J; j++)
+    for (int64_t t = dof[j]; t < dof[j + 1]; t++) rowcell[t] = j;
   UP(d_nrows, nrows.data(), J); UP(d_dof, dof.data(), J + 1); UP(d_doff, doff.data(), J + 1);
+  UP(d_rowcell, rowcell.data(), rowcell.size());
   UP(d_invJ, d->invJ, (size_t)J * D * D); UP(d_scal, d->scaling, J); UP(d_ls, d->lscale, (size_t)J * 2);
   UP(v_cell, d->vol_cell, d->nvol); UP(v_spec, d->vol_spec, d->nvol); UP(v_off, d->vol_off, d->nvol + 1);
   UP(v_nodes, d->vol_nodes, KV * D); UP(v_w, d->vol_w, KV);
