@@ -14,9 +14,13 @@
 //    coefficients (U, and W for the adjoint), staged with one coalesced copy and
 //    then read as B operands;
 //  * the neighbour's B operands are loaded just in time from global memory;
-//  * for Euler the face cotangent is merged before the pointwise adjoint
-//    (fn = -fo, so only sc_own*W_own.trace - sc_nb*W_nb.trace is needed) and the
-//    pointwise adjoint is the hand-written reverse sweep of euler_adjoint.hpp.
+//  * the Euler JVP and VJP (kt_jvp_euler, kt_vjp_euler at the end of this file) park
+//    the 4-nodes-per-lane MFMA outputs in a per-lane LDS buffer and run the pointwise
+//    flux node by node; the VJP merges the face cotangent before the pointwise adjoint
+//    (fn = -fo, so only sc_own*W_own.trace - sc_nb*W_nb.trace is needed) and uses the
+//    hand-written reverse sweep of euler_adjoint.hpp;
+//  * workgroups are renumbered so that each XCD (own L2) walks one contiguous range
+//    of cells (xcd_block).
 #pragma once
 #include <hip/hip_runtime.h>
 
