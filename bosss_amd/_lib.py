@@ -104,6 +104,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIBPATH):
             raise ImportError("libbosss_hip.so not found at %s; run __graft_entry__.build() first" % LIBPATH)
+        # PyTorch-ROCm ships its own HIP runtime (DT_NEEDED "libamdhip64.so", soname
+        # libamdhip64.so.7).  Loading torch first lets this library bind to that same runtime
+        # by soname; loading this library first would put a second HIP runtime into the
+        # process, after which torch finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIBPATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

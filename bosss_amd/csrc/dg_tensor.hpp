@@ -97,9 +97,16 @@ __device__ __forceinline__ Real tval(const Real* tab, const int (&q)[3], int cod
   return x * y;
 }
 
+// Mode code of mode n, clamped to the last mode for the padding rows n >= N (callers mask
+// those rows by a factor 0): every LDS read is unconditional, so no branch separates a
+// load from its use and the compiler can issue the loads of a whole contraction together.
 template <class TS, typename Real>
 __device__ __forceinline__ int tcode(const Real* tab, int n) {
-  return (n < TS::N) ? (int)tab[TS::O_CODE + n] : -1;
+  return (int)tab[TS::O_CODE + (n < TS::N ? n : TS::N - 1)];
+}
+template <class TS, typename Real>
+__device__ __forceinline__ Real mode_mask(int n) {
+  return n < TS::N ? Real(1) : Real(0);
 }
 
 // eval: acc[v] (nodes x cells) = table(node(col), mode) . B,   B operand from `Bsrc(v, kk)`
@@ -111,7 +118,7 @@ __device__ __forceinline__ void t_eval(const Real* tab, const int (&q)[3], int d
 #pragma unroll
   for (int kk = 0; kk < TS::NK; kk++) {
     int code = tcode<TS>(tab, 4 * kk + grp);
-    Real a = (code >= 0) ? tval<TS>(tab, q, code, deriv) : Real(0);
+    Real a = tval<TS>(tab, q, code, deriv) * mode_mask<TS, Real>(4 * kk + grp);
 #pragma unroll
     for (int v = 0; v < V; v++) acc[v] = MM<Real>::mma(a, Bsrc(v, kk), acc[v]);
   }
@@ -124,7 +131,7 @@ __device__ __forceinline__ void t_test(const Real* tab, const int (&q)[3], int d
 #pragma unroll
   for (int mt = 0; mt < TS::MT; mt++) {
     int code = tcode<TS>(tab, mt * 16 + col);
-    Real a = (code >= 0) ? tval<TS>(tab, q, code, deriv) : Real(0);
+    Real a = tval<TS>(tab, q, code, deriv) * mode_mask<TS, Real>(mt * 16 + col);
 #pragma unroll
     for (int v = 0; v < V; v++) racc[v][mt] = MM<Real>::mma(a, g[v], racc[v][mt]);
   }
@@ -232,8 +239,8 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_forward(TArgs<Real> a) {
 
   const Real* ownU = stU + col * STRIDE;
   const Real* ownX = stX + col * STRIDE;
-  auto Bu = [&](int v, int kk) { int m = 4 * kk + grp; return (m < N) ? ownU[v * N + m] * sc : Real(0); };
-  auto Bx = [&](int v, int kk) { int m = 4 * kk + grp; return (m < N) ? ownX[v * N + m] * sc : Real(0); };
+  auto Bu = [&](int v, int kk) { int m = 4 * kk + grp; return ownU[v * N + (m < N ? m : N - 1)] * ((m < N) ? sc : Real(0)); };
+  auto Bx = [&](int v, int kk) { int m = 4 * kk + grp; return ownX[v * N + (m < N ? m : N - 1)] * ((m < N) ? sc : Real(0)); };
 
   // ------------------------------------------------------------------ volume
   for (int t = 0; t < TS::TV; t++) {
@@ -295,11 +302,12 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_forward(TArgs<Real> a) {
   const Real pbd = Real(a.fp.penalty_base * a.fp.penalty_deg);
   const Real cj_own = Real(1.0 / a.lenscale[cj]);
   for (int f = 0; f < NF; f++) {
-    const int nb = valid ? a.cf.nb[(size_t)cj * NF + f] : -1;
+    const int nb_raw = a.cf.nb[(size_t)cj * NF + f];
+    const int nb = valid ? nb_raw : -1;
     const bool has_nb = nb >= 0;
     const int cn = has_nb ? nb : cj;
     const int fo = a.cf.pair[f];
-    const Real sg = Real(valid ? a.cf.sqrtg[(size_t)cj * NF + f] : 0.0);
+    const Real sg = Real(a.cf.sqrtg[(size_t)cj * NF + f]) * (valid ? Real(1) : Real(0));
     Real nrm[D];
 #pragma unroll
     for (int d = 0; d < D; d++) nrm[d] = Real(a.cf.normal[((size_t)cj * NF + f) * D + d]);
@@ -313,8 +321,8 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_forward(TArgs<Real> a) {
       for (int d = 0; d < D; d++) iJn[e][d] = Form::FACE_GRADU ? Real(a.invJ[(size_t)cn * D * D + e * D + d]) : Real(0);
     const Real* nbU = a.U + (size_t)cn * NT;
     const Real* nbX = TAN ? a.X + (size_t)cn * NT : nullptr;
-    auto Bnu = [&](int v, int kk) { int m = 4 * kk + grp; return (has_nb && m < N) ? nbU[v * N + m] * scn : Real(0); };
-    auto Bnx = [&](int v, int kk) { int m = 4 * kk + grp; return (has_nb && m < N) ? nbX[v * N + m] * scn : Real(0); };
+    auto Bnu = [&](int v, int kk) { int m = 4 * kk + grp; return nbU[v * N + (m < N ? m : N - 1)] * ((has_nb && m < N) ? scn : Real(0)); };
+    auto Bnx = [&](int v, int kk) { int m = 4 * kk + grp; return nbX[v * N + (m < N ? m : N - 1)] * ((has_nb && m < N) ? scn : Real(0)); };
 
     for (int t = 0; t < TS::TE; t++) {
       const int nodeE = t * 16 + col;
@@ -474,8 +482,8 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_adjoint(TArgs<Real> a) {
 
   const Real* ownU = stU + col * STRIDE;
   const Real* ownW = stW + col * STRIDE;
-  auto Bu = [&](int v, int kk) { int m = 4 * kk + grp; return (m < N) ? ownU[v * N + m] * sc : Real(0); };
-  auto Bw = [&](int v, int kk) { int m = 4 * kk + grp; return (m < N) ? ownW[v * N + m] : Real(0); };
+  auto Bu = [&](int v, int kk) { int m = 4 * kk + grp; return ownU[v * N + (m < N ? m : N - 1)] * ((m < N) ? sc : Real(0)); };
+  auto Bw = [&](int v, int kk) { int m = 4 * kk + grp; return ownW[v * N + (m < N ? m : N - 1)] * ((m < N) ? Real(1) : Real(0)); };
 
   // ------------------------------------------------------------------ volume
   for (int t = 0; t < TS::TV; t++) {
@@ -584,11 +592,12 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_adjoint(TArgs<Real> a) {
   const Real pbd = Real(a.fp.penalty_base * a.fp.penalty_deg);
   const Real cj_own = Real(1.0 / a.lenscale[cj]);
   for (int f = 0; f < NF; f++) {
-    const int nb = valid ? a.cf.nb[(size_t)cj * NF + f] : -1;
+    const int nb_raw = a.cf.nb[(size_t)cj * NF + f];
+    const int nb = valid ? nb_raw : -1;
     const bool has_nb = nb >= 0;
     const int cn = has_nb ? nb : cj;
     const int fo = a.cf.pair[f];
-    const Real sg = Real(valid ? a.cf.sqrtg[(size_t)cj * NF + f] : 0.0);
+    const Real sg = Real(a.cf.sqrtg[(size_t)cj * NF + f]) * (valid ? Real(1) : Real(0));
     Real nrm[D];
 #pragma unroll
     for (int d = 0; d < D; d++) nrm[d] = Real(a.cf.normal[((size_t)cj * NF + f) * D + d]);
@@ -602,8 +611,8 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_adjoint(TArgs<Real> a) {
       for (int d = 0; d < D; d++) iJn[e][d] = Real(a.invJ[(size_t)cn * D * D + e * D + d]);
     const Real* nbU = a.U + (size_t)cn * NT;
     const Real* nbW = a.X + (size_t)cn * NT;
-    auto Bnu = [&](int v, int kk) { int m = 4 * kk + grp; return (has_nb && m < N) ? nbU[v * N + m] * scn : Real(0); };
-    auto Bnw = [&](int v, int kk) { int m = 4 * kk + grp; return (has_nb && m < N) ? nbW[v * N + m] : Real(0); };
+    auto Bnu = [&](int v, int kk) { int m = 4 * kk + grp; return nbU[v * N + (m < N ? m : N - 1)] * ((has_nb && m < N) ? scn : Real(0)); };
+    auto Bnw = [&](int v, int kk) { int m = 4 * kk + grp; return nbW[v * N + (m < N ? m : N - 1)] * ((has_nb && m < N) ? Real(1) : Real(0)); };
 
     for (int t = 0; t < TS::TE; t++) {
       const int nodeE = t * 16 + col;
@@ -821,8 +830,8 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
 
   const Real* ownU = a.U + (size_t)cj * NT;
   const Real* ownW = a.X + (size_t)cj * NT;
-  auto Bu = [&](int v, int kk) { int m = 4 * kk + grp; return (valid && m < N) ? ownU[v * N + m] * sc : Real(0); };
-  auto Bw = [&](int v, int kk) { int m = 4 * kk + grp; return (valid && m < N) ? ownW[v * N + m] : Real(0); };
+  auto Bu = [&](int v, int kk) { int m = 4 * kk + grp; return ownU[v * N + (m < N ? m : N - 1)] * ((valid && m < N) ? sc : Real(0)); };
+  auto Bw = [&](int v, int kk) { int m = 4 * kk + grp; return ownW[v * N + (m < N ? m : N - 1)] * ((valid && m < N) ? Real(1) : Real(0)); };
   auto park = [&](int arr, const acc_t (&acc)[V]) {
 #pragma unroll
     for (int v = 0; v < V; v++)
@@ -909,18 +918,19 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
 
   // ------------------------------------------------------------------- faces
   for (int f = 0; f < NF; f++) {
-    const int nb = valid ? a.cf.nb[(size_t)cj * NF + f] : -1;
+    const int nb_raw = a.cf.nb[(size_t)cj * NF + f];
+    const int nb = valid ? nb_raw : -1;
     const bool has_nb = nb >= 0;
     const int cn = has_nb ? nb : cj;
     const int fo = a.cf.pair[f];
-    const Real sg = Real(valid ? a.cf.sqrtg[(size_t)cj * NF + f] : 0.0);
+    const Real sg = Real(a.cf.sqrtg[(size_t)cj * NF + f]) * (valid ? Real(1) : Real(0));
     Real nrm[D];
 #pragma unroll
     for (int d = 0; d < D; d++) nrm[d] = Real(a.cf.normal[((size_t)cj * NF + f) * D + d]);
     const Real scn = Real(a.scaling[cn]);
     const Real* nbU = a.U + (size_t)cn * NT;
     const Real* nbW = a.X + (size_t)cn * NT;
-    auto Bnu = [&](int v, int kk) { int m = 4 * kk + grp; return (has_nb && m < N) ? nbU[v * N + m] * scn : Real(0); };
+    auto Bnu = [&](int v, int kk) { int m = 4 * kk + grp; return nbU[v * N + (m < N ? m : N - 1)] * ((has_nb && m < N) ? scn : Real(0)); };
 
     for (int t = 0; t < TS::TE; t++) {
       const int nodeE = t * 16 + col;
@@ -946,12 +956,14 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
         for (int kk = 0; kk < TS::NK; kk++) {
           int m = 4 * kk + grp;
           int code = tcode<TS>(tab, m);
-          Real ao = (code >= 0) ? tval<TS>(tab, qo, code, -1) : Real(0);
-          Real an = (code >= 0) ? tval<TS>(tab, qn, code, -1) : Real(0);
+          Real mk = mode_mask<TS, Real>(m);
+          Real ao = tval<TS>(tab, qo, code, -1) * mk;
+          Real an = tval<TS>(tab, qn, code, -1) * mk;
 #pragma unroll
           for (int v = 0; v < V; v++) {
-            Real wo_ = (valid && m < N) ? ownW[v * N + m] * sc : Real(0);
-            Real wn_ = (has_nb && m < N) ? nbW[v * N + m] * scn : Real(0);
+            const int mc = m < N ? m : N - 1;
+            Real wo_ = ownW[v * N + mc] * ((valid && m < N) ? sc : Real(0));
+            Real wn_ = nbW[v * N + mc] * ((has_nb && m < N) ? scn : Real(0));
             acc[v] = MM<Real>::mma(ao, wo_, acc[v]);
             acc[v] = MM<Real>::mma(an, -wn_, acc[v]);
           }
@@ -1049,8 +1061,8 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_jvp_euler(TArgs<Real> a) {
 
   const Real* ownU = a.U + (size_t)cj * NT;
   const Real* ownX = a.X + (size_t)cj * NT;
-  auto Bu = [&](int v, int kk) { int m = 4 * kk + grp; return (valid && m < N) ? ownU[v * N + m] * sc : Real(0); };
-  auto Bx = [&](int v, int kk) { int m = 4 * kk + grp; return (valid && m < N) ? ownX[v * N + m] * sc : Real(0); };
+  auto Bu = [&](int v, int kk) { int m = 4 * kk + grp; return ownU[v * N + (m < N ? m : N - 1)] * ((valid && m < N) ? sc : Real(0)); };
+  auto Bx = [&](int v, int kk) { int m = 4 * kk + grp; return ownX[v * N + (m < N ? m : N - 1)] * ((valid && m < N) ? sc : Real(0)); };
   auto park = [&](int arr, const acc_t (&acc)[V]) {
 #pragma unroll
     for (int v = 0; v < V; v++)
@@ -1102,19 +1114,20 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_jvp_euler(TArgs<Real> a) {
 
   // ------------------------------------------------------------------- faces
   for (int f = 0; f < NF; f++) {
-    const int nb = valid ? a.cf.nb[(size_t)cj * NF + f] : -1;
+    const int nb_raw = a.cf.nb[(size_t)cj * NF + f];
+    const int nb = valid ? nb_raw : -1;
     const bool has_nb = nb >= 0;
     const int cn = has_nb ? nb : cj;
     const int fo = a.cf.pair[f];
-    const Real sg = Real(valid ? a.cf.sqrtg[(size_t)cj * NF + f] : 0.0);
+    const Real sg = Real(a.cf.sqrtg[(size_t)cj * NF + f]) * (valid ? Real(1) : Real(0));
     Real nrm[D];
 #pragma unroll
     for (int d = 0; d < D; d++) nrm[d] = Real(a.cf.normal[((size_t)cj * NF + f) * D + d]);
     const Real scn = Real(a.scaling[cn]);
     const Real* nbU = a.U + (size_t)cn * NT;
     const Real* nbX = a.X + (size_t)cn * NT;
-    auto Bnu = [&](int v, int kk) { int m = 4 * kk + grp; return (has_nb && m < N) ? nbU[v * N + m] * scn : Real(0); };
-    auto Bnx = [&](int v, int kk) { int m = 4 * kk + grp; return (has_nb && m < N) ? nbX[v * N + m] * scn : Real(0); };
+    auto Bnu = [&](int v, int kk) { int m = 4 * kk + grp; return nbU[v * N + (m < N ? m : N - 1)] * ((has_nb && m < N) ? scn : Real(0)); };
+    auto Bnx = [&](int v, int kk) { int m = 4 * kk + grp; return nbX[v * N + (m < N ? m : N - 1)] * ((has_nb && m < N) ? scn : Real(0)); };
 
     for (int t = 0; t < TS::TE; t++) {
       const int nodeE = t * 16 + col;
