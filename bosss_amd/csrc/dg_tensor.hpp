@@ -199,7 +199,7 @@ constexpr int TWAVES = 4;
 // Forward: residual (MODE_RES) or JVP (MODE_JVP).
 // ----------------------------------------------------------------------------
 template <class Form, int D, int P, int K1, typename Real, int MODE>
-__global__ void __launch_bounds__(64 * TWAVES) kt_forward(TArgs<Real> a) {
+__global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 2 : 1))) kt_forward(TArgs<Real> a) {
   typedef TShape<D, P, K1> TS;
   typedef typename MM<Real>::acc acc_t;
   constexpr int V = Form::V, N = TS::N, NT = V * N, NF = TS::NF, MT = TS::MT;
@@ -1026,7 +1026,7 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
 // trace stays in registers; the dual-number flux is then evaluated node by node.
 // ----------------------------------------------------------------------------
 template <class Form, int D, int P, int K1, typename Real>
-__global__ void __launch_bounds__(64 * TWAVES) kt_jvp_euler(TArgs<Real> a) {
+__global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 2 : 1))) kt_jvp_euler(TArgs<Real> a) {
   typedef TShape<D, P, K1> TS;
   typedef typename MM<Real>::acc acc_t;
   typedef Dual<1, Real> T;

@@ -179,15 +179,18 @@ struct EulerAdjoint {
     R pStar = pos ? pStar0 : R(0);
     const R cq = R(0.5) * (g + R(1)) / g;
     bool qi_on = pStar > si.p, qo_on = pStar > so.p;
-    R qI = qi_on ? sqrt((pStar / si.p - R(1)) * cq + R(1)) : R(1);
-    R qO = qo_on ? sqrt((pStar / so.p - R(1)) * cq + R(1)) : R(1);
+    // shared reciprocals: every quotient below reuses one of ipi, ipo, iden, iA, iB, iqI, iqO
+    const R ipi = R(1) / si.p, ipo = R(1) / so.p;
+    R qI = qi_on ? sqrt((pStar * ipi - R(1)) * cq + R(1)) : R(1);
+    R qO = qo_on ? sqrt((pStar * ipo - R(1)) * cq + R(1)) : R(1);
     R wIn = si.un - si.a * qI;
     R wOut = so.un + so.a * qO;
     R cIn = si.rho * (wIn - si.un);
     R cOut = so.rho * (wOut - so.un);
     R num = cOut * so.un - cIn * si.un + (si.p - so.p) * ims;
     R den = cOut - cIn;
-    R sm = num / den;
+    const R iden = R(1) / den;
+    R sm = num * iden;
     int branch;
     if (wIn > R(0)) branch = 0;
     else if (R(0) < sm) branch = 1;
@@ -207,13 +210,13 @@ struct EulerAdjoint {
     EulerPrimBar<D, R>& sb = kin ? bi : bo;
     const R* UK = kin ? Ui : Uo;
     const R wk = kin ? wIn : wOut;
-    R A = s.rho * (wk - s.un);
+    R A = s.rho * (wk - s.un);          // also the z of HLLCFlux's energy factor
     R B = wk - sm;
-    R base = A / B;
+    const R iA = R(1) / A, iB = R(1) / B;
+    R base = A * iB;
     R dsu = sm - s.un;
-    R z = s.rho * (wk - s.un);
-    R factor = sm * ms + s.p / z;
-    R Er = s.E / s.rho;
+    R factor = sm * ms + s.p * iA;
+    R Er = s.E * s.irho;
     R us[V];
     us[0] = base;
 #pragma unroll
@@ -246,13 +249,13 @@ struct EulerAdjoint {
     {
       R tb = usb[D + 1] * base;
       baseb += usb[D + 1] * (Er + factor * dsu);
-      sb.E += tb / s.rho;
-      sb.rho += -tb * s.E / (s.rho * s.rho);
+      sb.E += tb * s.irho;
+      sb.rho += -tb * Er * s.irho;
       R factorb = tb * dsu;
       dsub += tb * factor;
       smb += factorb * ms;
-      sb.p += factorb / z;
-      R zb = -factorb * s.p / (z * z);
+      sb.p += factorb * iA;
+      R zb = -factorb * s.p * iA * iA;
       sb.rho += zb * (wk - s.un);
       wkb += zb * s.rho;
       sb.un += -zb * s.rho;
@@ -260,8 +263,8 @@ struct EulerAdjoint {
     smb += dsub;
     sb.un += -dsub;
     {
-      R Ab = baseb / B;
-      R Bb = -baseb * A / (B * B);
+      R Ab = baseb * iB;
+      R Bb = -baseb * base * iB;
       sb.rho += Ab * (wk - s.un);
       wkb += Ab * s.rho;
       sb.un += -Ab * s.rho;
@@ -270,8 +273,8 @@ struct EulerAdjoint {
     }
     R wInb = kin ? wkb : R(0), wOutb = kin ? R(0) : wkb;
     // sm = num / den
-    R numb = smb / den;
-    R denb = -smb * sm / den;
+    R numb = smb * iden;
+    R denb = -smb * sm * iden;
     R cOutb = numb * so.un + denb;
     R cInb = -numb * si.un - denb;
     bo.un += numb * cOut;
@@ -295,13 +298,13 @@ struct EulerAdjoint {
     R pStarb = R(0);
     if (qi_on) {
       R yb = qIb * R(0.5) / qI;
-      pStarb += yb * cq / si.p;
-      bi.p += -yb * cq * pStar / (si.p * si.p);
+      pStarb += yb * cq * ipi;
+      bi.p += -yb * cq * pStar * ipi * ipi;
     }
     if (qo_on) {
       R yb = qOb * R(0.5) / qO;
-      pStarb += yb * cq / so.p;
-      bo.p += -yb * cq * pStar / (so.p * so.p);
+      pStarb += yb * cq * ipo;
+      bo.p += -yb * cq * pStar * ipo * ipo;
     }
     R p0b = pos ? pStarb : R(0);
     R meanPb = p0b;
