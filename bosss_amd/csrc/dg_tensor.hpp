@@ -124,16 +124,60 @@ __device__ __forceinline__ void t_eval(const Real* tab, const int (&q)[3], int d
   }
 }
 
-// test: racc[v][mt] += table(node(grp,r), mode(mt*16 + col))^T g[v]
+// Test-contraction accumulators of V variables.  Full 16-mode tiles use
+// v_mfma_f64_16x16x4f64 (4 rows per lane); when N = 16 m + 4 in FP64 (p = 3 in 3D: N = 20)
+// the last four modes use v_mfma_f64_4x4x4f64 (4 blocks of 4x4x4) instead of a 16-row tile
+// that would be three quarters padding.  Its lane layout (measured on the device): lane l
+// supplies A = table(node k = l/16, mode 16 m + l%4) and B = G(node l/16, cell l%16) -- the
+// same B operand as the 16x16x4 tile -- and receives mode 16 m + l/16 of cell l%16.
+template <class TS, typename Real, int V>
+struct TAcc {
+  typedef typename MM<Real>::acc acc_t;
+  static constexpr bool REM = std::is_same<Real, double>::value && TS::N > 16 && (TS::N % 16) == 4;
+  static constexpr int MF = REM ? TS::N / 16 : TS::MT;   // 16-row tiles
+  acc_t t[V][MF];
+  Real rem[REM ? V : 1];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+#pragma unroll
+      for (int mt = 0; mt < MF; mt++) t[v][mt] = acc_zero<Real>();
+      rem[REM ? v : 0] = Real(0);
+    }
+  }
+  // f(v, n, value) for every coefficient row n < N this lane holds
+  template <class F>
+  __device__ __forceinline__ void store(int grp, F f) const {
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+#pragma unroll
+      for (int mt = 0; mt < MF; mt++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          int n = mt * 16 + MM<Real>::row(grp, r);
+          if (n < TS::N) f(v, n, t[v][mt][r]);
+        }
+      if (REM) f(v, MF * 16 + grp, rem[REM ? v : 0]);
+    }
+  }
+};
+
 template <class TS, typename Real, int V>
 __device__ __forceinline__ void t_test(const Real* tab, const int (&q)[3], int deriv, int col, const Real (&g)[V],
-                                       typename MM<Real>::acc (&racc)[V][TS::MT]) {
+                                       TAcc<TS, Real, V>& acc) {
+  typedef TAcc<TS, Real, V> A_;
 #pragma unroll
-  for (int mt = 0; mt < TS::MT; mt++) {
+  for (int mt = 0; mt < A_::MF; mt++) {
     int code = tcode<TS>(tab, mt * 16 + col);
     Real a = tval<TS>(tab, q, code, deriv) * mode_mask<TS, Real>(mt * 16 + col);
 #pragma unroll
-    for (int v = 0; v < V; v++) racc[v][mt] = MM<Real>::mma(a, g[v], racc[v][mt]);
+    for (int v = 0; v < V; v++) acc.t[v][mt] = MM<Real>::mma(a, g[v], acc.t[v][mt]);
+  }
+  if constexpr (A_::REM) {
+    const int m = A_::MF * 16 + (col & 3);
+    Real a = tval<TS>(tab, q, tcode<TS>(tab, m), deriv);
+#pragma unroll
+    for (int v = 0; v < V; v++) acc.rem[v] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, g[v], acc.rem[v], 0, 0, 0);
   }
 }
 
@@ -231,11 +275,8 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
 #pragma unroll
     for (int d = 0; d < D; d++) iJ[e][d] = Real(a.invJ[(size_t)cj * D * D + e * D + d]);
 
-  acc_t racc[V][MT];
-#pragma unroll
-  for (int v = 0; v < V; v++)
-#pragma unroll
-    for (int mt = 0; mt < MT; mt++) racc[v][mt] = acc_zero<Real>();
+  TAcc<TS, Real, V> racc;
+  racc.zero();
 
   const Real* ownU = stU + col * STRIDE;
   const Real* ownX = stX + col * STRIDE;
@@ -418,22 +459,14 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
   if (valid) {
     Real* Op = (TAN ? a.out : a.R) + (size_t)cell * NT;
     const Real al = Real(a.alpha), be = Real(a.beta);
-#pragma unroll
-    for (int v = 0; v < V; v++)
-#pragma unroll
-      for (int mt = 0; mt < MT; mt++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          int n = mt * 16 + MM<Real>::row(grp, r);
-          if (n < N) {
-            if (TAN) {
-              Op[v * N + n] = racc[v][mt][r];
-            } else {
-              Real old = (be != Real(0)) ? be * Op[v * N + n] : Real(0);
-              Op[v * N + n] = old + al * racc[v][mt][r];
-            }
-          }
-        }
+    racc.store(grp, [&](int v, int n, Real val) {
+      if (TAN) {
+        Op[v * N + n] = val;
+      } else {
+        Real old = (be != Real(0)) ? be * Op[v * N + n] : Real(0);
+        Op[v * N + n] = old + al * val;
+      }
+    });
   }
 }
 
@@ -474,11 +507,9 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_adjoint(TArgs<Real> a) {
 #pragma unroll
     for (int d = 0; d < D; d++) iJ[e][d] = Real(a.invJ[(size_t)cj * D * D + e * D + d]);
 
-  acc_t racc[V][MT], aacc[V][MT];
-#pragma unroll
-  for (int v = 0; v < V; v++)
-#pragma unroll
-    for (int mt = 0; mt < MT; mt++) { racc[v][mt] = acc_zero<Real>(); aacc[v][mt] = acc_zero<Real>(); }
+  TAcc<TS, Real, V> racc, aacc;
+  racc.zero();
+  aacc.zero();
 
   const Real* ownU = stU + col * STRIDE;
   const Real* ownW = stW + col * STRIDE;
@@ -754,21 +785,12 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_adjoint(TArgs<Real> a) {
     Real* Rp = FUSED ? a.R + (size_t)cell * NT : nullptr;
     Real* Op = a.out + (size_t)cell * NT;
     const Real al = Real(a.alpha), be = Real(a.beta);
-#pragma unroll
-    for (int v = 0; v < V; v++)
-#pragma unroll
-      for (int mt = 0; mt < MT; mt++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          int n = mt * 16 + MM<Real>::row(grp, r);
-          if (n < N) {
-            if (FUSED) {
-              Real old = (be != Real(0)) ? be * Rp[v * N + n] : Real(0);
-              Rp[v * N + n] = old + al * racc[v][mt][r];
-            }
-            Op[v * N + n] = aacc[v][mt][r];
-          }
-        }
+    if (FUSED)
+      racc.store(grp, [&](int v, int n, Real val) {
+        Real old = (be != Real(0)) ? be * Rp[v * N + n] : Real(0);
+        Rp[v * N + n] = old + al * val;
+      });
+    aacc.store(grp, [&](int v, int n, Real val) { Op[v * N + n] = val; });
   }
 }
 
@@ -822,11 +844,9 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
 #pragma unroll
     for (int d = 0; d < D; d++) iJ[e][d] = Real(a.invJ[(size_t)cj * D * D + e * D + d]);
 
-  acc_t racc[V][MT], aacc[V][MT];
-#pragma unroll
-  for (int v = 0; v < V; v++)
-#pragma unroll
-    for (int mt = 0; mt < MT; mt++) { racc[v][mt] = acc_zero<Real>(); aacc[v][mt] = acc_zero<Real>(); }
+  TAcc<TS, Real, V> racc, aacc;
+  racc.zero();
+  aacc.zero();
 
   const Real* ownU = a.U + (size_t)cj * NT;
   const Real* ownW = a.X + (size_t)cj * NT;
@@ -1002,21 +1022,12 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
     Real* Rp = FUSED ? a.R + (size_t)cell * NT : nullptr;
     Real* Op = a.out + (size_t)cell * NT;
     const Real al = Real(a.alpha), be = Real(a.beta);
-#pragma unroll
-    for (int v = 0; v < V; v++)
-#pragma unroll
-      for (int mt = 0; mt < MT; mt++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          int n = mt * 16 + MM<Real>::row(grp, r);
-          if (n < N) {
-            if (FUSED) {
-              Real old = (be != Real(0)) ? be * Rp[v * N + n] : Real(0);
-              Rp[v * N + n] = old + al * racc[v][mt][r];
-            }
-            Op[v * N + n] = aacc[v][mt][r];
-          }
-        }
+    if (FUSED)
+      racc.store(grp, [&](int v, int n, Real val) {
+        Real old = (be != Real(0)) ? be * Rp[v * N + n] : Real(0);
+        Rp[v * N + n] = old + al * val;
+      });
+    aacc.store(grp, [&](int v, int n, Real val) { Op[v * N + n] = val; });
   }
 }
 
@@ -1053,11 +1064,8 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
 #pragma unroll
     for (int d = 0; d < D; d++) iJ[e][d] = Real(a.invJ[(size_t)cj * D * D + e * D + d]);
 
-  acc_t racc[V][MT];
-#pragma unroll
-  for (int v = 0; v < V; v++)
-#pragma unroll
-    for (int mt = 0; mt < MT; mt++) racc[v][mt] = acc_zero<Real>();
+  TAcc<TS, Real, V> racc;
+  racc.zero();
 
   const Real* ownU = a.U + (size_t)cj * NT;
   const Real* ownX = a.X + (size_t)cj * NT;
@@ -1179,15 +1187,7 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
 
   if (valid) {
     Real* Op = a.out + (size_t)cell * NT;
-#pragma unroll
-    for (int v = 0; v < V; v++)
-#pragma unroll
-      for (int mt = 0; mt < MT; mt++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          int n = mt * 16 + MM<Real>::row(grp, r);
-          if (n < N) Op[v * N + n] = racc[v][mt][r];
-        }
+    racc.store(grp, [&](int v, int n, Real val) { Op[v * N + n] = val; });
   }
 }
 

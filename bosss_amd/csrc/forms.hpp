@@ -178,10 +178,10 @@ struct FormAdvDiff {
 // StateVector.cs / IdealGas.cs / Euler{Density,Momentum,Energy}Component.cs
 template <int D, class T, class R>
 struct EulerState {
-  T rho, m[D], E, u[D], p, a;
+  T rho, m[D], E, u[D], p, a, irho;
   __host__ __device__ __forceinline__ EulerState(const T* U, const FormParams& P) {
     rho = U[0];
-    T irho = R(1) / rho;
+    irho = R(1) / rho;
 #pragma unroll
     for (int d = 0; d < D; d++) { m[d] = U[1 + d]; u[d] = m[d] * irho; }
     E = U[D + 1];
@@ -282,13 +282,15 @@ struct FormEuler {
       if (branch == 1 || branch == 2) {
         const T& wk = (branch == 1) ? wIn : wOut;
         // HLLC{Density,Momentum,Energy}Flux.GetModifiedVariableValue
-        T base = s.rho * (wk - uk) / (wk - sm);
+        T A = s.rho * (wk - uk);
+        T iA = R(1) / A;
+        T base = A / (wk - sm);
         T dsu = sm - uk;
         fo[0] = fo[0] + wk * (base - s.rho);
 #pragma unroll
         for (int c = 0; c < D; c++) fo[1 + c] = fo[1 + c] + wk * (base * (s.u[c] + dsu * n[c]) - s.m[c]);
-        T factor = sm * ms + s.p / (s.rho * (wk - uk));
-        fo[D + 1] = fo[D + 1] + wk * (base * (s.E / s.rho + factor * dsu) - s.E);
+        T factor = sm * ms + s.p * iA;
+        fo[D + 1] = fo[D + 1] + wk * (base * (s.E * s.irho + factor * dsu) - s.E);
       }
     }
 #pragma unroll
