@@ -300,6 +300,7 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
     }
 #pragma unroll
     for (int r = 0; r < 4; r++) {
+      if (t * 16 + MM<Real>::row(0, r) >= TS::KV) continue;   // all 4 nodes of this step are padding
       const int node = t * 16 + MM<Real>::row(grp, r);
       const bool pad = node >= TS::KV;
       int q[3];
@@ -390,6 +391,7 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
       }
 #pragma unroll
       for (int r = 0; r < 4; r++) {
+        if (t * 16 + MM<Real>::row(0, r) >= TS::KE) continue;   // all 4 nodes of this step are padding
         const int node = t * 16 + MM<Real>::row(grp, r);
         const bool pad = node >= TS::KE;
         int q[3];
@@ -530,6 +532,7 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_adjoint(TArgs<Real> a) {
     }
 #pragma unroll
     for (int r = 0; r < 4; r++) {
+      if (t * 16 + MM<Real>::row(0, r) >= TS::KV) continue;   // all 4 nodes of this step are padding
       const int node = t * 16 + MM<Real>::row(grp, r);
       const bool pad = node >= TS::KV;
       int q[3];
@@ -669,6 +672,7 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_adjoint(TArgs<Real> a) {
       }
 #pragma unroll
       for (int r = 0; r < 4; r++) {
+        if (t * 16 + MM<Real>::row(0, r) >= TS::KE) continue;   // all 4 nodes of this step are padding
         const int node = t * 16 + MM<Real>::row(grp, r);
         const bool pad = node >= TS::KE;
         int q[3];
@@ -871,6 +875,7 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
     }
     if (FUSED) {
       for (int r = 0; r < 4; r++) {
+        if (t * 16 + MM<Real>::row(0, r) >= TS::KV) continue;   // all 4 nodes of this step are padding
         const int node = t * 16 + MM<Real>::row(grp, r);
         const bool pad = node >= TS::KV;
         int q[3];
@@ -906,6 +911,7 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
         park(1, acc);
       }
       for (int r = 0; r < 4; r++) {
+        if (t * 16 + MM<Real>::row(0, r) >= TS::KV) continue;   // all 4 nodes of this step are padding
         const int node = t * 16 + MM<Real>::row(grp, r);
         const bool pad = node >= TS::KV;
         int q[3];
@@ -925,6 +931,7 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
       }
     }
     for (int r = 0; r < 4; r++) {
+      if (t * 16 + MM<Real>::row(0, r) >= TS::KV) continue;   // all 4 nodes of this step are padding
       const int node = t * 16 + MM<Real>::row(grp, r);
       const bool pad = node >= TS::KV;
       int q[3];
@@ -991,6 +998,7 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
         park(2, acc);
       }
       for (int r = 0; r < 4; r++) {
+        if (t * 16 + MM<Real>::row(0, r) >= TS::KE) continue;   // all 4 nodes of this step are padding
         const int node = t * 16 + MM<Real>::row(grp, r);
         const bool pad = node >= TS::KE;
         int q[3];
@@ -1091,6 +1099,7 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
       park(1, acc);
     }
     for (int r = 0; r < 4; r++) {
+      if (t * 16 + MM<Real>::row(0, r) >= TS::KV) continue;   // all 4 nodes of this step are padding
       const int node = t * 16 + MM<Real>::row(grp, r);
       const bool pad = node >= TS::KV;
       int q[3];
@@ -1155,6 +1164,7 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
       t_eval<TS, Real, V>(tab, qn, -1, grp, Bnx, dun);
 #pragma unroll
       for (int r = 0; r < 4; r++) {
+        if (t * 16 + MM<Real>::row(0, r) >= TS::KE) continue;   // all 4 nodes of this step are padding
         const int node = t * 16 + MM<Real>::row(grp, r);
         const bool pad = node >= TS::KE;
         int q[3];
