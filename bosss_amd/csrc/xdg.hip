@@ -328,6 +328,45 @@ __global__ void __launch_bounds__(256) k_xdg_spmv(int64_t ndof, int S, const int
   y[t] = (Real)(alpha * acc + (beta != 0.0 ? beta * (double)y[t] : 0.0));
 }
 
+// Same apply on the solver's column-major copy (element (r, c) of block (j, s) at
+// boff + c * n_j + r): consecutive rows of a cell read consecutive addresses.
+template <typename Real>
+__global__ void __launch_bounds__(256) k_xdg_spmv_cm(int64_t ndof, int S, const int32_t* __restrict__ rowcell,
+                                                     const int32_t* __restrict__ nrows, const int32_t* __restrict__ cols,
+                                                     const int64_t* __restrict__ boff, const int64_t* __restrict__ dof,
+                                                     const Real* __restrict__ A, double alpha, const Real* __restrict__ x,
+                                                     double beta, Real* __restrict__ y) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ndof) return;
+  const int j = rowcell[t];
+  const int n = nrows[j];
+  const int r = (int)(t - dof[j]);
+  double acc = 0.0;
+  for (int s = 0; s < S; s++) {
+    const int k = cols[j * S + s];
+    if (k < 0) continue;
+    const int m = nrows[k];
+    const Real* blk = A + boff[(size_t)j * S + s] + r;
+    const Real* xk = x + dof[k];
+    for (int c = 0; c < m; c++) acc += (double)blk[(int64_t)c * n] * (double)xk[c];
+  }
+  y[t] = (Real)(alpha * acc + (beta != 0.0 ? beta * (double)y[t] : 0.0));
+}
+
+// row-major blocks -> column-major copy, one workgroup per block
+__global__ void k_blocks_to_cm(int S, const int32_t* __restrict__ nrows, const int32_t* __restrict__ cols,
+                               const int64_t* __restrict__ boff, const double* __restrict__ A, double* __restrict__ At) {
+  const int j = blockIdx.x / S, s = blockIdx.x % S;
+  const int k = cols[j * S + s];
+  if (k < 0) return;
+  const int n = nrows[j], m = nrows[k];
+  const int64_t o = boff[(size_t)j * S + s];
+  for (int i = threadIdx.x; i < n * m; i += blockDim.x) {
+    const int r = i / m, c = i % m;
+    At[o + (int64_t)c * n + r] = A[o + i];
+  }
+}
+
 // gather the diagonal blocks into a compact batch (offsets doff) for the block LU
 __global__ void k_gather_diag(int J, int S, const int32_t* __restrict__ nrows, const int64_t* __restrict__ boff,
                               const int64_t* __restrict__ doff, const double* __restrict__ A, double* __restrict__ Dg) {
@@ -398,7 +437,7 @@ __global__ void __launch_bounds__(64) k_chol_inv(int J, const int32_t* __restric
   if (lane == 0) info[j] = 0;
 }
 
-// A32(j, s) = (float)(Linv_j A(j, s) Linv_k^T), one workgroup per block
+// A32(j, s) = (float)(Linv_j A(j, s) Linv_k^T), column-major, one workgroup per block
 __global__ void __launch_bounds__(64) k_scale_blocks(int J, int S, const int32_t* __restrict__ nrows, const int32_t* __restrict__ cols,
                                                      const int64_t* __restrict__ boff, const int64_t* __restrict__ doff,
                                                      const double* __restrict__ Linv, const double* __restrict__ A,
@@ -424,7 +463,7 @@ __global__ void __launch_bounds__(64) k_scale_blocks(int J, int S, const int32_t
     const int r = i / m, c = i % m;
     double v = 0.0;
     for (int l = 0; l <= r; l++) v += Lj[r * n + l] * T[l * m + c];
-    A32[o + i] = (float)v;
+    A32[o + (int64_t)c * n + r] = (float)v;      // column-major solver copy
   }
 }
 
@@ -547,7 +586,7 @@ struct bosss_xdg {
   double *v_nodes, *v_w, *f_nin, *f_nout, *f_w, *f_normal, *f_gD, *i_nodes, *i_w, *i_normal;
   // solver workspace (allocated on first solve)
   bool ws = false;
-  double *Dinv = nullptr, *Lchol = nullptr, *scal = nullptr, *partial = nullptr;
+  double *Dinv = nullptr, *Lchol = nullptr, *At = nullptr, *scal = nullptr, *partial = nullptr;
   int32_t* info = nullptr;
   double *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *q64 = nullptr;
   float *A32 = nullptr, *Dinv32 = nullptr, *b32 = nullptr, *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
@@ -606,11 +645,17 @@ void launch_spmv(bosss_xdg* X, const Real* A, double alpha, const Real* x, doubl
                      X->d_nrows, X->d_cols, X->d_boff, X->d_dof, A, alpha, x, beta, y);
 }
 
-// one PCG iteration (Real = double or float), all scalars on the device
+template <typename Real>
+void launch_spmv_cm(bosss_xdg* X, const Real* A, double alpha, const Real* x, double beta, Real* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_xdg_spmv_cm<Real>, dim3((unsigned)((X->ndof + 255) / 256)), dim3(256), 0, s, X->ndof, X->a.S, X->d_rowcell,
+                     X->d_nrows, X->d_cols, X->d_boff, X->d_dof, A, alpha, x, beta, y);
+}
+
+// one PCG iteration (Real = double or float), all scalars on the device; A column-major
 template <typename Real>
 void pcg_iteration(bosss_xdg* X, const Real* A, const Real* Dinv, Real* x, Real* r, Real* z, Real* p, Real* q, hipStream_t s) {
   const int64_t n = X->ndof;
-  launch_spmv<Real>(X, A, 1.0, p, 0.0, q, s);
+  launch_spmv_cm<Real>(X, A, 1.0, p, 0.0, q, s);
   launch_dot<Real>(X, n, p, q, 0, S_PQ, -1, s);
   hipLaunchKernelGGL(k_pcg_xr<Real>, dim3(grid_for(n)), dim3(256), 0, s, n, X->scal, p, q, x, r);
   hipLaunchKernelGGL(k_bdiag_apply<Real>, dim3((unsigned)((X->ndof + 255) / 256)), dim3(256), 0, s, X->ndof, X->d_rowcell, X->d_nrows,
@@ -625,7 +670,7 @@ template <typename Real>
 void pcg_start(bosss_xdg* X, const Real* A, const Real* Dinv, const Real* b, Real* x, Real* r, Real* z, Real* p, hipStream_t s) {
   const int64_t n = X->ndof;
   hipLaunchKernelGGL((k_scale_copy<Real, Real>), dim3(grid_for(n)), dim3(256), 0, s, n, 1.0, b, r, 0);
-  launch_spmv<Real>(X, A, -1.0, x, 1.0, r, s);
+  launch_spmv_cm<Real>(X, A, -1.0, x, 1.0, r, s);
   hipLaunchKernelGGL(k_bdiag_apply<Real>, dim3((unsigned)((X->ndof + 255) / 256)), dim3(256), 0, s, X->ndof, X->d_rowcell, X->d_nrows,
                      X->d_doff, X->d_dof, Dinv, r, z);
   hipLaunchKernelGGL((k_scale_copy<Real, Real>), dim3(grid_for(n)), dim3(256), 0, s, n, 1.0, z, p, 0);
@@ -852,8 +897,11 @@ int bosss_hip_xdg_solve_dev(bosss_xdg* x, const double* blocks, const double* b,
     XCHK(alloc(x, &x->Dinv, x->ndiag)); XCHK(alloc(x, &x->scal, S_NSLOT)); XCHK(alloc(x, &x->partial, 2 * RED_BLOCKS));
     XCHK(alloc(x, &x->info, x->a.J));
     XCHK(alloc(x, &x->r64, n)); XCHK(alloc(x, &x->z64, n)); XCHK(alloc(x, &x->p64, n)); XCHK(alloc(x, &x->q64, n));
+    XCHK(alloc(x, &x->At, x->nentries));
     x->ws = true;
   }
+  // column-major copy of the matrix for the solver's applies
+  hipLaunchKernelGGL(k_blocks_to_cm, dim3(x->a.J * x->a.S), dim3(256), 0, s, x->a.S, x->d_nrows, x->d_cols, x->d_boff, blocks, x->At);
   if (precision == BOSSS_FP32 && !x->A32) {
     XCHK(alloc(x, &x->A32, x->nentries)); XCHK(alloc(x, &x->Dinv32, x->ndiag));
     XCHK(alloc(x, &x->b32, n)); XCHK(alloc(x, &x->r32, n)); XCHK(alloc(x, &x->z32, n)); XCHK(alloc(x, &x->p32, n)); XCHK(alloc(x, &x->q32, n)); XCHK(alloc(x, &x->x32, n));
@@ -872,7 +920,7 @@ int bosss_hip_xdg_solve_dev(bosss_xdg* x, const double* blocks, const double* b,
   }
   if (precision == BOSSS_FP64) {
     *outer = 1;
-    return pcg_solve<double>(x, 0, blocks, x->Dinv, b, sol, x->r64, x->z64, x->p64, x->q64, tol, maxit, check_every, use_graph, false,
+    return pcg_solve<double>(x, 0, x->At, x->Dinv, b, sol, x->r64, x->z64, x->p64, x->q64, tol, maxit, check_every, use_graph, false,
                              iterations, relres);
   }
   // FP32 CG on the equilibrated matrix Linv M Linv^T inside FP64 iterative refinement:
@@ -900,7 +948,7 @@ int bosss_hip_xdg_solve_dev(bosss_xdg* x, const double* blocks, const double* b,
   const double inner_tol = 1e-4;
   for (k = 0; k < 40; k++) {
     hipLaunchKernelGGL((k_scale_copy<double, double>), dim3(grid_for(n)), dim3(256), 0, s, n, 1.0, b, x->r64, 0);
-    launch_spmv<double>(x, blocks, -1.0, sol, 1.0, x->r64, s);
+    launch_spmv_cm<double>(x, x->At, -1.0, sol, 1.0, x->r64, s);
     launch_dot<double>(x, n, x->r64, x->r64, 0, S_RR, -1, s);
     XHIP(hipMemcpyAsync(h, x->scal, sizeof h, hipMemcpyDeviceToHost, s));
     XHIP(hipStreamSynchronize(s));
