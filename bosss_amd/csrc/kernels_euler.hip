@@ -4,6 +4,11 @@
 #include "registry.hpp"
 
 namespace bosss {
+// The fused residual + VJP kernels of 3D p = 3 are compiled in kernels_euler_fused.hip, with
+// the memory-clause scheduling strategy that batches their coefficient loads.
+extern template __global__ void kt_vjp_euler<FormEuler<3, false>, 3, 3, 5, double, true>(TArgs<double>);
+extern template __global__ void kt_vjp_euler<FormEuler<3, true>, 3, 3, 5, double, true>(TArgs<double>);
+
 void register_euler(std::vector<KernelSet>& v) {
   v.push_back(make_kernel_set<FormEuler<2, false>, 2, 6, 4, false>(2));
   v.push_back(make_kernel_set<FormEuler<2, true>, 2, 6, 4, false>(3));
