@@ -241,15 +241,20 @@ def newton_comparison(grid, op64, Un, form, p):
     op32 = DifferentialOperator(grid, form, p, gamma=1.4, precision="fp32")
     dt = 0.5 / grid.ncells[0] / (2 * p + 1)
     st = ImplicitStep(op64, Un, dt, op32=op32)
-    st.solve_fp64(Un, tol=1e-10)        # warm-up (allocations, first launches)
+    st.solve_fp64(Un, tol=1e-10)        # warm-up of both drivers (allocations, first launches)
+    st.solve_mixed(Un, tol=1e-10)
     res = {}
     for name, fn in (("fp64", st.solve_fp64), ("mixed", st.solve_mixed)):
-        torch.cuda.synchronize()
-        U = fn(Un, tol=1e-10)
-        s = dict(st.stats)
-        res[name] = {"seconds": round(s["seconds"], 4), "newton_steps": s["newton"], "jvps": s["jvps"],
-                     "final_residual": s["history"][-1]}
-        del U
+        best = None
+        for _ in range(2):               # best of two solves
+            torch.cuda.synchronize()
+            U = fn(Un, tol=1e-10)
+            s = dict(st.stats)
+            if best is None or s["seconds"] < best["seconds"]:
+                best = s
+            del U
+        res[name] = {"seconds": round(best["seconds"], 4), "newton_steps": best["newton"], "jvps": best["jvps"],
+                     "final_residual": best["history"][-1]}
     res["dt"] = dt
     res["speedup_mixed_vs_fp64"] = round(res["fp64"]["seconds"] / res["mixed"]["seconds"], 3)
     return res
