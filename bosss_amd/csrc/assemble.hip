@@ -73,8 +73,98 @@ void launch_extract(int J, int Nt, int S, const int32_t* cols, const int32_t* co
   size_t n = (size_t)J * Nt;
   hipLaunchKernelGGL(k_extract, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, J, Nt, S, cols, color, k, q, Y, alpha, beta, blocks);
 }
+// Streaming block-ELL apply, one wavefront per cell row, four per workgroup.  The S blocks of a
+// cell row are read as flat row-major arrays: element e = 64 i + lane of a panel of R whole rows
+// (R * Nt <= 64 NL), so every load instruction reads 512 consecutive bytes.  The x entries of the
+// S column cells are staged in LDS first; a lane accumulates sum_s B_s[e] x_s[e % Nt] for each of
+// its NL elements, and the R row sums of the panel are reduced through LDS.  Loads past the panel
+// are clamped to its last element and their products are never read.
+template <int S, int NL, int XMAX>
+__global__ void __launch_bounds__(256) k_bell_spmv_flat(int J, int Nt, const int32_t* __restrict__ cols,
+                                                        const double* __restrict__ blocks, double alpha,
+                                                        const double* __restrict__ x, double beta, double* __restrict__ y) {
+  __shared__ double xs[4][S][XMAX];
+  __shared__ double pr[4][NL * 64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int j0 = blockIdx.x * 4 + w;
+  const bool valid = j0 < J;
+  const int j = valid ? j0 : J - 1;
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    const int k = cols[(size_t)j * S + s];
+    const double* xk = x + (size_t)(k < 0 ? j : k) * Nt;
+#pragma unroll
+    for (int q0 = 0; q0 < XMAX; q0 += 64) {
+      const int q = q0 + lane;
+      if (q < XMAX) {
+        const double t = xk[q < Nt ? q : Nt - 1];
+        xs[w][s][q] = k >= 0 ? t : 0.0;
+      }
+    }
+  }
+  __syncthreads();
+  const int NN = Nt * Nt;
+  const int R = min(Nt, (64 * NL) / Nt);
+  const double* bj = blocks + (size_t)j * S * NN;
+  for (int r0 = 0; r0 < Nt; r0 += R) {
+    const int rows = min(R, Nt - r0), pe = rows * Nt;
+    const double* bp = bj + (size_t)r0 * Nt;
+    double acc[NL];
+    int col[NL];
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+      const int e = 64 * i + lane;
+      acc[i] = 0.0;
+      col[i] = (e < pe ? e : pe - 1) % Nt;
+    }
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+#pragma unroll
+      for (int i = 0; i < NL; i++) {
+        const int e = 64 * i + lane;
+        acc[i] += bp[(size_t)s * NN + (e < pe ? e : pe - 1)] * xs[w][s][col[i]];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NL; i++) pr[w][64 * i + lane] = acc[i];
+    __syncthreads();
+    if (valid && lane < rows) {
+      double a = 0.0;
+      for (int cc = 0; cc < Nt; cc++) a += pr[w][lane * Nt + cc];
+      double* yj = y + (size_t)j * Nt + r0 + lane;
+      *yj = alpha * a + (beta != 0.0 ? beta * *yj : 0.0);
+    }
+    __syncthreads();
+  }
+}
+
+template <int S>
+static bool launch_bell_flat(int J, int Nt, const int32_t* cols, const double* blocks, double alpha, const double* x, double beta,
+                             double* y, hipStream_t st) {
+  const int nl = (Nt * Nt + 63) / 64;
+  const dim3 g((unsigned)((J + 3) / 4)), b(256);
+#define BELL_FLAT(NL, XMAX) hipLaunchKernelGGL((k_bell_spmv_flat<S, NL, XMAX>), g, b, 0, st, J, Nt, cols, blocks, alpha, x, beta, y)
+  if (Nt <= 32) {
+    if (nl <= 1) BELL_FLAT(1, 32);
+    else if (nl <= 2) BELL_FLAT(2, 32);
+    else if (nl <= 4) BELL_FLAT(4, 32);
+    else if (nl <= 7) BELL_FLAT(7, 32);
+    else if (nl <= 8) BELL_FLAT(8, 32);
+    else if (nl <= 12) BELL_FLAT(12, 32);
+    else BELL_FLAT(16, 32);
+  } else if (Nt <= 128) {
+    BELL_FLAT(16, 128);
+  } else {
+    return false;
+  }
+#undef BELL_FLAT
+  return true;
+}
+
 void launch_bell_spmv(int J, int Nt, int S, const int32_t* cols, const double* blocks, double alpha, const double* x, double beta,
                       double* y, hipStream_t s) {
+  if (S == 7 && launch_bell_flat<7>(J, Nt, cols, blocks, alpha, x, beta, y, s)) return;
+  if (S == 5 && launch_bell_flat<5>(J, Nt, cols, blocks, alpha, x, beta, y, s)) return;
   hipLaunchKernelGGL(k_bell_spmv, dim3(J), dim3(64), 0, s, J, Nt, S, cols, blocks, alpha, x, beta, y);
 }
 
