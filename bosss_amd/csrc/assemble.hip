@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(256) k_bell_spmv_flat(int J, int Nt, const int
                                                         const double* __restrict__ x, double beta, double* __restrict__ y) {
   __shared__ double xs[4][S][XMAX];
   __shared__ double pr[4][NL * 64];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;   // wave-uniform: scalar index loads
   const int j0 = blockIdx.x * 4 + w;
   const bool valid = j0 < J;
   const int j = valid ? j0 : J - 1;

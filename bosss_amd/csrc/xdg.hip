@@ -328,29 +328,76 @@ __global__ void __launch_bounds__(256) k_xdg_spmv(int64_t ndof, int S, const int
   y[t] = (Real)(alpha * acc + (beta != 0.0 ? beta * (double)y[t] : 0.0));
 }
 
-// Same apply on the solver's column-major copy (element (r, c) of block (j, s) at
-// boff + c * n_j + r): consecutive rows of a cell read consecutive addresses.
-template <typename Real>
-__global__ void __launch_bounds__(256) k_xdg_spmv_cm(int64_t ndof, int S, const int32_t* __restrict__ rowcell,
-                                                     const int32_t* __restrict__ nrows, const int32_t* __restrict__ cols,
-                                                     const int64_t* __restrict__ boff, const int64_t* __restrict__ dof,
-                                                     const Real* __restrict__ A, double alpha, const Real* __restrict__ x,
-                                                     double beta, Real* __restrict__ y) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ndof) return;
-  const int j = rowcell[t];
+// Sum of v over the 64 lanes of a wavefront (fixed butterfly order: deterministic).
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Apply on the solver's column-major copy (element (r, c) of block (j, s) at boff + c n_j + r).
+// The blocks of cell row j follow each other in slot order, so together they are one n_j x M_j
+// column-major panel, M_j = sum of the column cells' sizes.  One wavefront per cell row, four
+// per workgroup.  Lane l < G n_j (G = 64 / n_j column groups) reads panel entries l + G n_j t,
+// i.e. row l % n_j of panel columns l / n_j + G t: every load instruction reads G n_j consecutive
+// entries.  The column cells' x entries are staged in LDS in panel-column order and the G
+// partial sums of a row are added through LDS in a fixed order.  With DOT, the workgroup's
+// sum of y_i x_i over its rows (p.q of CG, q as stored) goes to partial[2 b] (partial[2 b + 1] = 0).
+constexpr int XPANEL = 7 * 2 * XN;   // panel columns: at most 7 slots of at most 2 N entries
+template <typename Real, bool DOT>
+__global__ void __launch_bounds__(256) k_xdg_spmv_panel(int J, int S, const int32_t* __restrict__ nrows, const int32_t* __restrict__ mcol,
+                                                        const int64_t* __restrict__ xoff, const int64_t* __restrict__ boff,
+                                                        const int64_t* __restrict__ dof, const Real* __restrict__ A, double alpha,
+                                                        const Real* __restrict__ x, double beta, Real* __restrict__ y,
+                                                        double* __restrict__ partial) {
+  __shared__ double xs[4][XPANEL];
+  __shared__ double red[4][64];
+  __shared__ double wsum[4];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;   // wave-uniform: scalar index loads
+  const int j0 = blockIdx.x * 4 + w;
+  const bool valid = j0 < J;
+  const int j = valid ? j0 : J - 1;
   const int n = nrows[j];
-  const int r = (int)(t - dof[j]);
-  double acc = 0.0;
+  int M = 0;
   for (int s = 0; s < S; s++) {
-    const int k = cols[j * S + s];
-    if (k < 0) continue;
-    const int m = nrows[k];
-    const Real* blk = A + boff[(size_t)j * S + s] + r;
-    const Real* xk = x + dof[k];
-    for (int c = 0; c < m; c++) acc += (double)blk[(int64_t)c * n] * (double)xk[c];
+    const int m = mcol[(size_t)j * S + s];
+    if (lane < m) xs[w][M + lane] = (double)x[xoff[(size_t)j * S + s] + lane];
+    M += m;
   }
-  y[t] = (Real)(alpha * acc + (beta != 0.0 ? beta * (double)y[t] : 0.0));
+  __syncthreads();
+  const int G = 64 / n, GN = G * n, L = n * M;
+  const Real* pa = A + boff[(size_t)j * S];
+  double acc = 0.0;
+  if (lane < GN) {
+    int e = lane, c = lane / n;
+    for (; e + 3 * GN < L; e += 4 * GN, c += 4 * G) {
+      const double a0 = (double)pa[e], a1 = (double)pa[e + GN], a2 = (double)pa[e + 2 * GN], a3 = (double)pa[e + 3 * GN];
+      acc += a0 * xs[w][c];
+      acc += a1 * xs[w][c + G];
+      acc += a2 * xs[w][c + 2 * G];
+      acc += a3 * xs[w][c + 3 * G];
+    }
+    for (; e < L; e += GN, c += G) acc += (double)pa[e] * xs[w][c];
+  }
+  red[w][lane] = acc;
+  __syncthreads();
+  double t = 0.0;
+  if (lane < n) {
+    double a = 0.0;
+    for (int g = 0; g < G; g++) a += red[w][g * n + lane];
+    Real* yp = y + dof[j] + lane;
+    const Real yi = (Real)(alpha * a + (beta != 0.0 ? beta * (double)*yp : 0.0));
+    if (valid) *yp = yi;
+    t = valid ? (double)yi * xs[w][lane] : 0.0;   // slot 0 is the diagonal: xs[w][0..n) = own x
+  }
+  if (DOT) {
+    t = wave_sum(t);
+    if (lane == 0) wsum[w] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      partial[2 * blockIdx.x] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+      partial[2 * blockIdx.x + 1] = 0.0;
+    }
+  }
 }
 
 // row-major blocks -> column-major copy, one workgroup per block
@@ -375,21 +422,16 @@ __global__ void k_gather_diag(int J, int S, const int32_t* __restrict__ nrows, c
   for (int i = threadIdx.x; i < n * n; i += blockDim.x) Dg[doff[j] + i] = A[boff[(size_t)j * S] + i];
 }
 
-// z_j <- Dinv_j r_j, one thread per row
-template <typename Real>
-__global__ void __launch_bounds__(256) k_bdiag_apply(int64_t ndof, const int32_t* __restrict__ rowcell, const int32_t* __restrict__ nrows,
-                                                     const int64_t* __restrict__ doff, const int64_t* __restrict__ dof,
-                                                     const Real* __restrict__ Dinv, const Real* __restrict__ r, Real* __restrict__ z) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ndof) return;
-  const int j = rowcell[t];
-  const int n = nrows[j];
-  const int i = (int)(t - dof[j]);
-  const Real* row = Dinv + doff[j] + (int64_t)i * n;
-  const Real* rj = r + dof[j];
-  double acc = 0.0;
-  for (int c = 0; c < n; c++) acc += (double)row[c] * (double)rj[c];
-  z[dof[j] + i] = (Real)acc;
+// In-place transpose of the n_j x n_j blocks at doff[j] (the LU's row-major inverses -> the
+// column-major form the fused CG update reads), one workgroup per block.
+__global__ void __launch_bounds__(64) k_transpose_blocks(int J, const int32_t* __restrict__ nrows, const int64_t* __restrict__ doff,
+                                                         double* __restrict__ Dg) {
+  __shared__ double tile[2 * XN * 2 * XN];
+  const int j = blockIdx.x, n = nrows[j];
+  double* b = Dg + doff[j];
+  for (int i = threadIdx.x; i < n * n; i += 64) tile[i] = b[i];
+  __syncthreads();
+  for (int i = threadIdx.x; i < n * n; i += 64) b[i] = tile[(i % n) * n + i / n];
 }
 
 // Diagonal-block equilibration for the FP32 solve (the reference's SymPart_DiagBlockEquilib
@@ -484,11 +526,6 @@ __global__ void __launch_bounds__(64) k_lower_apply(int J, const int32_t* __rest
   *o = (Out)((accumulate ? (double)*o : 0.0) + scale * acc);
 }
 
-template <typename Real>
-__global__ void k_identity_blocks(int J, const int32_t* __restrict__ nrows, const int64_t* __restrict__ doff, Real* __restrict__ Dg) {
-  const int j = blockIdx.x, n = nrows[j];
-  for (int i = threadIdx.x; i < n * n; i += blockDim.x) Dg[doff[j] + i] = Real((i / n) == (i % n) ? 1 : 0);
-}
 
 // ------------------------------------------------------------ reductions
 constexpr int RED_BLOCKS = 256, RED_THREADS = 256;
@@ -521,30 +558,96 @@ __global__ void __launch_bounds__(RED_THREADS) k_dot_partial(int64_t n, const Re
   if (threadIdx.x == 0) { partial[2 * blockIdx.x] = t1; partial[2 * blockIdx.x + 1] = t2; }
 }
 
-// scal[slot1] = sum of partial sums 0, scal[slot2] = sum of partial sums 1 (slot2 < 0: skip)
-__global__ void __launch_bounds__(RED_THREADS) k_dot_final(int nb, const double* __restrict__ partial, double* __restrict__ scal,
-                                                           int slot1, int slot2) {
-  __shared__ double sh[RED_THREADS / 64];
-  double s1 = 0.0, s2 = 0.0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) { s1 += partial[2 * i]; s2 += partial[2 * i + 1]; }
+// scal[slot1] = sum of partial sums 0, scal[slot2] = sum of partial sums 1 (slot2 < 0: skip);
+// with cdst >= 0, scal[cdst] = scal[csrc] first (rz_old <- rz_new at the start of a CG step).
+// Any block size that is a multiple of 64 up to 1024; fixed summation order.
+__global__ void __launch_bounds__(1024) k_dot_final(int nb, const double* __restrict__ partial, double* __restrict__ scal,
+                                                    int slot1, int slot2, int cdst, int csrc) {
+  __shared__ double sh[16];
+  const int T = blockDim.x;
+  double a1[4] = {0.0, 0.0, 0.0, 0.0}, a2[4] = {0.0, 0.0, 0.0, 0.0};
+  int i = threadIdx.x;
+  for (; i + 3 * T < nb; i += 4 * T) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) { a1[u] += partial[2 * (i + u * T)]; a2[u] += partial[2 * (i + u * T) + 1]; }
+  }
+  for (; i < nb; i += T) { a1[0] += partial[2 * i]; a2[0] += partial[2 * i + 1]; }
+  const double s1 = (a1[0] + a1[1]) + (a1[2] + a1[3]), s2 = (a2[0] + a2[1]) + (a2[2] + a2[3]);
   double t1 = block_sum(s1, sh);
   __syncthreads();
   double t2 = block_sum(s2, sh);
-  if (threadIdx.x == 0) { scal[slot1] = t1; if (slot2 >= 0) scal[slot2] = t2; }
+  if (threadIdx.x == 0) {
+    if (cdst >= 0) scal[cdst] = scal[csrc];
+    scal[slot1] = t1;
+    if (slot2 >= 0) scal[slot2] = t2;
+  }
 }
 
 // PCG scalar slots
 enum { S_RZ = 0, S_PQ = 1, S_RZN = 2, S_RR = 3, S_BB = 4, S_NSLOT = 8 };
 
-// x += alpha p, r -= alpha q with alpha = rz / pq (0 if pq == 0)
-template <typename Real>
-__global__ void k_pcg_xr(int64_t n, const double* __restrict__ scal, const Real* __restrict__ p, const Real* __restrict__ q,
-                         Real* __restrict__ x, Real* __restrict__ r) {
-  const double pq = scal[S_PQ];
-  const double al = (pq != 0.0) ? scal[S_RZ] / pq : 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    x[i] = (Real)((double)x[i] + al * (double)p[i]);
-    r[i] = (Real)((double)r[i] - al * (double)q[i]);
+// Fused CG step on one cell row per wavefront (four per workgroup): with STEP, x += alpha p and
+// r -= alpha q (alpha = rz / pq, 0 if pq == 0); then z = Dinv r with the column-major inverse
+// diagonal block (lane l < G n reads entries l + G n t, as the panel apply), or z = r when IDENT
+// (the FP32 solve runs on the equilibrated matrix, whose diagonal blocks are the identity); the
+// workgroup's sums of r.z and r.r go to partial[2 b], partial[2 b + 1].
+template <typename Real, bool IDENT>
+__global__ void __launch_bounds__(256) k_pcg_update(int J, int step, const int32_t* __restrict__ nrows, const int64_t* __restrict__ dof,
+                                                    const int64_t* __restrict__ doff, const double* __restrict__ scal,
+                                                    const Real* __restrict__ Dinv, const Real* __restrict__ p, const Real* __restrict__ q,
+                                                    Real* __restrict__ x, Real* __restrict__ r, Real* __restrict__ z,
+                                                    double* __restrict__ partial) {
+  __shared__ double rs[4][64];
+  __shared__ double red[4][64];
+  __shared__ double wsum[4][2];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;   // wave-uniform: scalar index loads
+  const int j0 = blockIdx.x * 4 + w;
+  const bool valid = j0 < J;
+  const int j = valid ? j0 : J - 1;
+  const int n = nrows[j];
+  const int64_t o = dof[j];
+  double ri = 0.0;
+  if (lane < n) {
+    if (step) {
+      const double pq = scal[S_PQ];
+      const double al = (pq != 0.0) ? scal[S_RZ] / pq : 0.0;
+      const Real xi = (Real)((double)x[o + lane] + al * (double)p[o + lane]);
+      const Real rr = (Real)((double)r[o + lane] - al * (double)q[o + lane]);
+      if (valid) { x[o + lane] = xi; r[o + lane] = rr; }
+      ri = (double)rr;
+    } else {
+      ri = (double)r[o + lane];
+    }
+  }
+  double zi = ri;
+  if (!IDENT) {
+    rs[w][lane] = ri;
+    __syncthreads();
+    const int G = 64 / n, GN = G * n, L = n * n;
+    const Real* pd = Dinv + doff[j];
+    double acc = 0.0;
+    if (lane < GN)
+      for (int e = lane, c = lane / n; e < L; e += GN, c += G) acc += (double)pd[e] * rs[w][c];
+    red[w][lane] = acc;
+    __syncthreads();
+    zi = 0.0;
+    if (lane < n)
+      for (int g = 0; g < G; g++) zi += red[w][g * n + lane];
+  }
+  double t1 = 0.0, t2 = 0.0;
+  if (lane < n && valid) {
+    const Real zr = (Real)zi;
+    z[o + lane] = zr;
+    t1 = ri * (double)zr;
+    t2 = ri * ri;
+  }
+  t1 = wave_sum(t1);
+  t2 = wave_sum(t2);
+  if (lane == 0) { wsum[w][0] = t1; wsum[w][1] = t2; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = ((wsum[0][0] + wsum[1][0]) + wsum[2][0]) + wsum[3][0];
+    partial[2 * blockIdx.x + 1] = ((wsum[0][1] + wsum[1][1]) + wsum[2][1]) + wsum[3][1];
   }
 }
 // p = z + beta p with beta = rz_new / rz (0 if rz == 0)
@@ -554,9 +657,6 @@ __global__ void k_pcg_p(int64_t n, const double* __restrict__ scal, const Real* 
   const double be = (rz != 0.0) ? scal[S_RZN] / rz : 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = (Real)((double)z[i] + be * (double)p[i]);
-}
-__global__ void k_scal_copy(double* scal, int dst, int src) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) scal[dst] = scal[src];
 }
 // y = (Real)(s * x)   (FP64 residual -> scaled FP32 right-hand side, and back)
 template <typename A, typename B>
@@ -578,8 +678,8 @@ struct bosss_xdg {
   int nvol = 0, nface = 0, nifc = 0;
   int64_t ndof = 0, nentries = 0, ndiag = 0;
   std::vector<void*> allocs;
-  int32_t *d_nspec, *d_spec0, *d_expo, *d_cols, *d_nrows, *d_rowcell;
-  int64_t *d_boff, *d_dof, *d_doff;
+  int32_t *d_nspec, *d_spec0, *d_expo, *d_cols, *d_nrows, *d_rowcell, *d_mcol;
+  int64_t *d_boff, *d_dof, *d_doff, *d_xoff;
   double *d_invJ, *d_scal, *d_ls;
   int32_t *v_cell, *v_spec, *f_cells, *f_slot, *f_spec, *i_cell;
   int64_t *v_off, *f_off, *i_off;
@@ -589,7 +689,7 @@ struct bosss_xdg {
   double *Dinv = nullptr, *Lchol = nullptr, *At = nullptr, *scal = nullptr, *partial = nullptr;
   int32_t* info = nullptr;
   double *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *q64 = nullptr;
-  float *A32 = nullptr, *Dinv32 = nullptr, *b32 = nullptr, *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
+  float *A32 = nullptr, *b32 = nullptr, *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   const void* graph_key[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   int graph_k[2] = {0, 0};
@@ -636,7 +736,7 @@ inline unsigned grid_for(int64_t n, int threads = 256) {
 template <typename Real>
 void launch_dot(bosss_xdg* X, int64_t n, const Real* x, const Real* y, int want2, int slot1, int slot2, hipStream_t s) {
   hipLaunchKernelGGL(k_dot_partial<Real>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, n, x, y, want2, X->partial);
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, s, RED_BLOCKS, X->partial, X->scal, slot1, slot2);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, s, RED_BLOCKS, X->partial, X->scal, slot1, slot2, -1, 0);
 }
 
 template <typename Real>
@@ -645,36 +745,56 @@ void launch_spmv(bosss_xdg* X, const Real* A, double alpha, const Real* x, doubl
                      X->d_nrows, X->d_cols, X->d_boff, X->d_dof, A, alpha, x, beta, y);
 }
 
+// workgroups of the cell-row kernels (four cell rows each)
+inline unsigned row_groups(const bosss_xdg* X) { return (unsigned)((X->a.J + 3) / 4); }
+
+// y = alpha A x + beta y on the column-major copy; with dot_slot >= 0 also scal[dot_slot] = x.y
+// (and first scal[cdst] = scal[csrc] when cdst >= 0)
 template <typename Real>
-void launch_spmv_cm(bosss_xdg* X, const Real* A, double alpha, const Real* x, double beta, Real* y, hipStream_t s) {
-  hipLaunchKernelGGL(k_xdg_spmv_cm<Real>, dim3((unsigned)((X->ndof + 255) / 256)), dim3(256), 0, s, X->ndof, X->a.S, X->d_rowcell,
-                     X->d_nrows, X->d_cols, X->d_boff, X->d_dof, A, alpha, x, beta, y);
+void launch_spmv_cm(bosss_xdg* X, const Real* A, double alpha, const Real* x, double beta, Real* y, hipStream_t s, int dot_slot = -1,
+                    int cdst = -1, int csrc = 0) {
+  if (dot_slot < 0) {
+    hipLaunchKernelGGL((k_xdg_spmv_panel<Real, false>), dim3(row_groups(X)), dim3(256), 0, s, X->a.J, X->a.S, X->d_nrows, X->d_mcol,
+                       X->d_xoff, X->d_boff, X->d_dof, A, alpha, x, beta, y, X->partial);
+  } else {
+    hipLaunchKernelGGL((k_xdg_spmv_panel<Real, true>), dim3(row_groups(X)), dim3(256), 0, s, X->a.J, X->a.S, X->d_nrows, X->d_mcol,
+                       X->d_xoff, X->d_boff, X->d_dof, A, alpha, x, beta, y, X->partial);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(1024), 0, s, (int)row_groups(X), X->partial, X->scal, dot_slot, -1, cdst, csrc);
+  }
 }
 
-// one PCG iteration (Real = double or float), all scalars on the device; A column-major
+// Fused x/r update (step != 0), z = Dinv r (Dinv == nullptr: identity blocks), then
+// scal[S_RZN] = r.z, scal[S_RR] = r.r.
+template <typename Real>
+void launch_update(bosss_xdg* X, int step, const Real* Dinv, const Real* p, const Real* q, Real* x, Real* r, Real* z, hipStream_t s) {
+  if (Dinv)
+    hipLaunchKernelGGL((k_pcg_update<Real, false>), dim3(row_groups(X)), dim3(256), 0, s, X->a.J, step, X->d_nrows, X->d_dof, X->d_doff,
+                       X->scal, Dinv, p, q, x, r, z, X->partial);
+  else
+    hipLaunchKernelGGL((k_pcg_update<Real, true>), dim3(row_groups(X)), dim3(256), 0, s, X->a.J, step, X->d_nrows, X->d_dof, X->d_doff,
+                       X->scal, Dinv, p, q, x, r, z, X->partial);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(1024), 0, s, (int)row_groups(X), X->partial, X->scal, (int)S_RZN, (int)S_RR, -1, 0);
+}
+
+// One PCG iteration (Real = double or float), all scalars on the device; A column-major, Dinv
+// the column-major inverse diagonal blocks (nullptr: identity).  Slot S_RZN holds r.z of the
+// previous step; the p.q reduction moves it to S_RZ before alpha = rz / pq is used.
 template <typename Real>
 void pcg_iteration(bosss_xdg* X, const Real* A, const Real* Dinv, Real* x, Real* r, Real* z, Real* p, Real* q, hipStream_t s) {
   const int64_t n = X->ndof;
-  launch_spmv_cm<Real>(X, A, 1.0, p, 0.0, q, s);
-  launch_dot<Real>(X, n, p, q, 0, S_PQ, -1, s);
-  hipLaunchKernelGGL(k_pcg_xr<Real>, dim3(grid_for(n)), dim3(256), 0, s, n, X->scal, p, q, x, r);
-  hipLaunchKernelGGL(k_bdiag_apply<Real>, dim3((unsigned)((X->ndof + 255) / 256)), dim3(256), 0, s, X->ndof, X->d_rowcell, X->d_nrows,
-                     X->d_doff, X->d_dof, Dinv, r, z);
-  launch_dot<Real>(X, n, r, z, 1, S_RZN, S_RR, s);
+  launch_spmv_cm<Real>(X, A, 1.0, p, 0.0, q, s, S_PQ, S_RZ, S_RZN);
+  launch_update<Real>(X, 1, Dinv, p, q, x, r, z, s);
   hipLaunchKernelGGL(k_pcg_p<Real>, dim3(grid_for(n)), dim3(256), 0, s, n, X->scal, z, p);
-  hipLaunchKernelGGL(k_scal_copy, dim3(1), dim3(64), 0, s, X->scal, (int)S_RZ, (int)S_RZN);
 }
 
-// r = b - A x, z = Dinv r, p = z, rz = r.z, rr = r.r, bb = b.b
+// r = b - A x, z = Dinv r, p = z, scal[S_RZN] = r.z, rr = r.r, bb = b.b
 template <typename Real>
 void pcg_start(bosss_xdg* X, const Real* A, const Real* Dinv, const Real* b, Real* x, Real* r, Real* z, Real* p, hipStream_t s) {
   const int64_t n = X->ndof;
   hipLaunchKernelGGL((k_scale_copy<Real, Real>), dim3(grid_for(n)), dim3(256), 0, s, n, 1.0, b, r, 0);
   launch_spmv_cm<Real>(X, A, -1.0, x, 1.0, r, s);
-  hipLaunchKernelGGL(k_bdiag_apply<Real>, dim3((unsigned)((X->ndof + 255) / 256)), dim3(256), 0, s, X->ndof, X->d_rowcell, X->d_nrows,
-                     X->d_doff, X->d_dof, Dinv, r, z);
+  launch_update<Real>(X, 0, Dinv, p, p, x, r, z, s);
   hipLaunchKernelGGL((k_scale_copy<Real, Real>), dim3(grid_for(n)), dim3(256), 0, s, n, 1.0, z, p, 0);
-  launch_dot<Real>(X, n, r, z, 1, S_RZ, S_RR, s);
   launch_dot<Real>(X, n, b, b, 0, S_BB, -1, s);
 }
 
@@ -814,6 +934,12 @@ int bosss_hip_xdg_create(bosss_ctx* ctx, const bosss_xdg_desc* d, bosss_xdg** ou
     for (int64_t t = dof[j]; t < dof[j + 1]; t++) rowcell[t] = j;
   UP(d_nrows, nrows.data(), J); UP(d_dof, dof.data(), J + 1); UP(d_doff, doff.data(), J + 1);
   UP(d_rowcell, rowcell.data(), rowcell.size());
+  // per (cell row, slot): size and offset of the column cell's x entries (0 / 0 for no column)
+  std::vector<int32_t> mcol((size_t)J * S, 0);
+  std::vector<int64_t> xoff((size_t)J * S, 0);
+  for (size_t i = 0; i < (size_t)J * S; i++)
+    if (d->cols[i] >= 0) { mcol[i] = nrows[d->cols[i]]; xoff[i] = dof[d->cols[i]]; }
+  UP(d_mcol, mcol.data(), mcol.size()); UP(d_xoff, xoff.data(), xoff.size());
   UP(d_invJ, d->invJ, (size_t)J * D * D); UP(d_scal, d->scaling, J); UP(d_ls, d->lscale, (size_t)J * 2);
   UP(v_cell, d->vol_cell, d->nvol); UP(v_spec, d->vol_spec, d->nvol); UP(v_off, d->vol_off, d->nvol + 1);
   UP(v_nodes, d->vol_nodes, KV * D); UP(v_w, d->vol_w, KV);
@@ -894,7 +1020,7 @@ int bosss_hip_xdg_solve_dev(bosss_xdg* x, const double* blocks, const double* b,
   }
   const int64_t n = x->ndof;
   if (!x->ws) {
-    XCHK(alloc(x, &x->Dinv, x->ndiag)); XCHK(alloc(x, &x->scal, S_NSLOT)); XCHK(alloc(x, &x->partial, 2 * RED_BLOCKS));
+    XCHK(alloc(x, &x->Dinv, x->ndiag)); XCHK(alloc(x, &x->scal, S_NSLOT)); XCHK(alloc(x, &x->partial, 2 * std::max<int64_t>(RED_BLOCKS, (x->a.J + 3) / 4)));
     XCHK(alloc(x, &x->info, x->a.J));
     XCHK(alloc(x, &x->r64, n)); XCHK(alloc(x, &x->z64, n)); XCHK(alloc(x, &x->p64, n)); XCHK(alloc(x, &x->q64, n));
     XCHK(alloc(x, &x->At, x->nentries));
@@ -903,7 +1029,7 @@ int bosss_hip_xdg_solve_dev(bosss_xdg* x, const double* blocks, const double* b,
   // column-major copy of the matrix for the solver's applies
   hipLaunchKernelGGL(k_blocks_to_cm, dim3(x->a.J * x->a.S), dim3(256), 0, s, x->a.S, x->d_nrows, x->d_cols, x->d_boff, blocks, x->At);
   if (precision == BOSSS_FP32 && !x->A32) {
-    XCHK(alloc(x, &x->A32, x->nentries)); XCHK(alloc(x, &x->Dinv32, x->ndiag));
+    XCHK(alloc(x, &x->A32, x->nentries));
     XCHK(alloc(x, &x->b32, n)); XCHK(alloc(x, &x->r32, n)); XCHK(alloc(x, &x->z32, n)); XCHK(alloc(x, &x->p32, n)); XCHK(alloc(x, &x->q32, n)); XCHK(alloc(x, &x->x32, n));
   }
   // block-Jacobi: inverse diagonal blocks (batched uneven LU + DGETRI)
@@ -919,6 +1045,7 @@ int bosss_hip_xdg_solve_dev(bosss_xdg* x, const double* blocks, const double* b,
       if (info[j] != 0) return info[j] > 0 ? j + 1 : xfail(BOSSS_ERR_ARG, "block %d larger than %d", j, nmax);
   }
   if (precision == BOSSS_FP64) {
+    hipLaunchKernelGGL(k_transpose_blocks, dim3(x->a.J), dim3(64), 0, s, x->a.J, x->d_nrows, x->d_doff, x->Dinv);
     *outer = 1;
     return pcg_solve<double>(x, 0, x->At, x->Dinv, b, sol, x->r64, x->z64, x->p64, x->q64, tol, maxit, check_every, use_graph, false,
                              iterations, relres);
@@ -937,7 +1064,6 @@ int bosss_hip_xdg_solve_dev(bosss_xdg* x, const double* blocks, const double* b,
   }
   hipLaunchKernelGGL(k_scale_blocks, dim3(x->a.J * x->a.S), dim3(64), 0, s, x->a.J, x->a.S, x->d_nrows, x->d_cols, x->d_boff, x->d_doff,
                      x->Lchol, blocks, x->A32);
-  hipLaunchKernelGGL(k_identity_blocks<float>, dim3(x->a.J), dim3(256), 0, s, x->a.J, x->d_nrows, x->d_doff, x->Dinv32);
   double h[S_NSLOT];
   launch_dot<double>(x, n, b, b, 0, S_BB, -1, s);
   XHIP(hipMemcpyAsync(h, x->scal, sizeof h, hipMemcpyDeviceToHost, s));
@@ -961,7 +1087,7 @@ int bosss_hip_xdg_solve_dev(bosss_xdg* x, const double* blocks, const double* b,
     hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n)), dim3(256), 0, s, n, x->x32, 0.0f);
     int it = 0;
     double r_in = 0.0;
-    rc = pcg_solve<float>(x, 1, x->A32, x->Dinv32, x->b32, x->x32, x->r32, x->z32, x->p32, x->q32, inner_tol,
+    rc = pcg_solve<float>(x, 1, x->A32, nullptr, x->b32, x->x32, x->r32, x->z32, x->p32, x->q32, inner_tol,
                           std::max(1, maxit - total), check_every, use_graph, true, &it, &r_in);
     if (rc != BOSSS_OK) return rc;
     total += it;
