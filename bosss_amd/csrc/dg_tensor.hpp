@@ -255,7 +255,7 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Real* tab = reinterpret_cast<Real*>(smem_raw);
   for (int i = threadIdx.x; i < TS::TAB; i += blockDim.x) tab[i] = a.tab[i];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int col = lane & 15, grp = lane >> 4;
   const int cell0 = (xcd_block(a.nxcd) * TWAVES + wave) * 16;
   Real* stU = tab + ((TS::TAB + 1) & ~1) + wave * (NARR * 16 * STRIDE);
@@ -489,7 +489,7 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_adjoint(TArgs<Real> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Real* tab = reinterpret_cast<Real*>(smem_raw);
   for (int i = threadIdx.x; i < TS::TAB; i += blockDim.x) tab[i] = a.tab[i];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int col = lane & 15, grp = lane >> 4;
   const int cell0 = (xcd_block(a.nxcd) * TWAVES + wave) * 16;
   Real* stU = tab + ((TS::TAB + 1) & ~1) + wave * (2 * 16 * STRIDE);
@@ -832,7 +832,7 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
   Real* tab = reinterpret_cast<Real*>(smem_raw);
   for (int i = threadIdx.x; i < TS::TAB; i += blockDim.x) tab[i] = a.tab[i];
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int col = lane & 15, grp = lane >> 4;
   Real* buf = tab + ((TS::TAB + 1) & ~1) + wave * (NSLOT * 64);
   auto slot = [&](int arr, int v, int r) -> Real& { return buf[((arr * V + v) * 4 + r) * 64 + lane]; };
@@ -854,8 +854,38 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
 
   const Real* ownU = a.U + (size_t)cj * NT;
   const Real* ownW = a.X + (size_t)cj * NT;
-  auto Bu = [&](int v, int kk) { int m = 4 * kk + grp; return ownU[v * N + (m < N ? m : N - 1)] * ((valid && m < N) ? sc : Real(0)); };
-  auto Bw = [&](int v, int kk) { int m = 4 * kk + grp; return ownW[v * N + (m < N ? m : N - 1)] * ((valid && m < N) ? Real(1) : Real(0)); };
+  // The fused kernel keeps the own cell's B operands (scaled U, W) in registers for the whole
+  // launch: U is contracted 7 times and W 9 times.  The VJP-only kernel reads them just in time
+  // from L2 (holding them there costs it more in register traffic than it saves).
+  constexpr int PV = FUSED ? V : 1, PK = FUSED ? TS::NK : 1;
+  Real pU[PV][PK], pW[PV][PK];
+  if constexpr (FUSED) {
+#pragma unroll
+    for (int v = 0; v < V; v++)
+#pragma unroll
+      for (int kk = 0; kk < TS::NK; kk++) {
+        const int m = 4 * kk + grp;
+        const bool in = valid && m < N;
+        pU[v][kk] = ownU[v * N + (m < N ? m : N - 1)] * (in ? sc : Real(0));
+        pW[v][kk] = ownW[v * N + (m < N ? m : N - 1)] * (in ? Real(1) : Real(0));
+      }
+  }
+  auto Bu = [&](int v, int kk) -> Real {
+    if constexpr (FUSED) {
+      return pU[v][kk];
+    } else {
+      const int m = 4 * kk + grp;
+      return ownU[v * N + (m < N ? m : N - 1)] * ((valid && m < N) ? sc : Real(0));
+    }
+  };
+  auto Bw = [&](int v, int kk) -> Real {
+    if constexpr (FUSED) {
+      return pW[v][kk];
+    } else {
+      const int m = 4 * kk + grp;
+      return ownW[v * N + (m < N ? m : N - 1)] * ((valid && m < N) ? Real(1) : Real(0));
+    }
+  };
   auto park = [&](int arr, const acc_t (&acc)[V]) {
 #pragma unroll
     for (int v = 0; v < V; v++)
@@ -989,7 +1019,7 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
 #pragma unroll
           for (int v = 0; v < V; v++) {
             const int mc = m < N ? m : N - 1;
-            Real wo_ = ownW[v * N + mc] * ((valid && m < N) ? sc : Real(0));
+            Real wo_ = Bw(v, kk) * sc;
             Real wn_ = nbW[v * N + mc] * ((has_nb && m < N) ? scn : Real(0));
             acc[v] = MM<Real>::mma(ao, wo_, acc[v]);
             acc[v] = MM<Real>::mma(an, -wn_, acc[v]);
@@ -1056,7 +1086,7 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
   Real* tab = reinterpret_cast<Real*>(smem_raw);
   for (int i = threadIdx.x; i < TS::TAB; i += blockDim.x) tab[i] = a.tab[i];
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int col = lane & 15, grp = lane >> 4;
   Real* buf = tab + ((TS::TAB + 1) & ~1) + wave * (NSLOT * 64);
   auto slot = [&](int arr, int v, int r) -> Real& { return buf[((arr * V + v) * 4 + r) * 64 + lane]; };
