@@ -1107,8 +1107,39 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
 
   const Real* ownU = a.U + (size_t)cj * NT;
   const Real* ownX = a.X + (size_t)cj * NT;
-  auto Bu = [&](int v, int kk) { int m = 4 * kk + grp; return ownU[v * N + (m < N ? m : N - 1)] * ((valid && m < N) ? sc : Real(0)); };
-  auto Bx = [&](int v, int kk) { int m = 4 * kk + grp; return ownX[v * N + (m < N ? m : N - 1)] * ((valid && m < N) ? sc : Real(0)); };
+  // FP32 keeps the own cell's B operands (scaled U and tangent) in registers for the whole launch;
+  // FP64 reads them just in time from L2 (its register file is full at one wavefront per SIMD and
+  // holding them there costs more than the reloads).
+  constexpr bool PRE = std::is_same<Real, float>::value;
+  constexpr int PV = PRE ? V : 1, PK = PRE ? TS::NK : 1;
+  Real pU[PV][PK], pX[PV][PK];
+  if constexpr (PRE) {
+#pragma unroll
+    for (int v = 0; v < V; v++)
+#pragma unroll
+      for (int kk = 0; kk < TS::NK; kk++) {
+        const int m = 4 * kk + grp;
+        const bool in = valid && m < N;
+        pU[v][kk] = ownU[v * N + (m < N ? m : N - 1)] * (in ? sc : Real(0));
+        pX[v][kk] = ownX[v * N + (m < N ? m : N - 1)] * (in ? sc : Real(0));
+      }
+  }
+  auto Bu = [&](int v, int kk) -> Real {
+    if constexpr (PRE) {
+      return pU[v][kk];
+    } else {
+      const int m = 4 * kk + grp;
+      return ownU[v * N + (m < N ? m : N - 1)] * ((valid && m < N) ? sc : Real(0));
+    }
+  };
+  auto Bx = [&](int v, int kk) -> Real {
+    if constexpr (PRE) {
+      return pX[v][kk];
+    } else {
+      const int m = 4 * kk + grp;
+      return ownX[v * N + (m < N ? m : N - 1)] * ((valid && m < N) ? sc : Real(0));
+    }
+  };
   auto park = [&](int arr, const acc_t (&acc)[V]) {
 #pragma unroll
     for (int v = 0; v < V; v++)
