@@ -47,17 +47,22 @@ struct EulerPrimBar {
 #pragma unroll
     for (int d = 0; d < D; d++) { m[d] = R(0); u[d] = R(0); }
   }
-  // accumulate into Ubar[0..D+1]
+  // accumulate into Ubar[0..D+1].  SOUND = false: the sound speed has no adjoint (the
+  // volume flux does not use it), so neither a nor its derivative is formed.
+  template <bool SOUND = true>
   __host__ __device__ __forceinline__ void pullback(const EulerPrim<D, R>& s, const R* n, const FormParams& P, R* Ubar) const {
     const R g1 = R(P.gamma - 1.0);
     const R ms = R(P.gamma * P.mach * P.mach);
     R ub[D];
 #pragma unroll
     for (int d = 0; d < D; d++) ub[d] = u[d] + un * n[d];
-    // a = sqrt(q)/Ma, q = p irho
-    R qb = a * R(0.5) / (s.a * R(P.mach * P.mach));
-    R pb = p + qb * s.irho;
-    R irb = qb * s.p;
+    R pb = p, irb = R(0);
+    if constexpr (SOUND) {
+      // a = sqrt(q)/Ma, q = p irho
+      R qb = a * R(0.5) / (s.a * R(P.mach * P.mach));
+      pb += qb * s.irho;
+      irb += qb * s.p;
+    }
     // p = g1 (E - msq irho ms/2)
     R Eb = E + g1 * pb;
     R msqb = -g1 * R(0.5) * ms * s.irho * pb;
@@ -127,7 +132,7 @@ struct EulerAdjoint {
       sb.E += ge * s.u[d];
       sb.p += ge * s.u[d];
     }
-    sb.pullback(s, zero_n, P, Ubar);
+    sb.template pullback<false>(s, zero_n, P, Ubar);
   }
 
   // Inner face, own side i, neighbour o, normal n out of i.  fo = Fhat(Ui, Uo, n).
