@@ -947,15 +947,16 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
         int q[3];
         vol_axes<D, K1>(pad ? 0 : node, q);
         const Real wq = pad ? Real(0) : vol_weight<TS>(tab, q) * detJ * sc;
-        Real Uq[V], Fb[V][D], ub[V];
+        Real Uq[V], gw[V], ub[V], kd[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) kd[d] = iJ[e][d] * wq;
 #pragma unroll
         for (int v = 0; v < V; v++) {
           Uq[v] = slot(0, v, r);
           ub[v] = slot(2, v, r);
-#pragma unroll
-          for (int d = 0; d < D; d++) Fb[v][d] = iJ[e][d] * slot(1, v, r) * wq;
+          gw[v] = slot(1, v, r);
         }
-        EulerAdjoint<D, Real>::volume_vjp(a.fp, Uq, Fb, ub);
+        EulerAdjoint<D, Real>::volume_vjp_dir(a.fp, Uq, kd, gw, ub);
 #pragma unroll
         for (int v = 0; v < V; v++) slot(2, v, r) = ub[v];
       }

@@ -135,6 +135,24 @@ struct EulerAdjoint {
     sb.template pullback<false>(s, zero_n, P, Ubar);
   }
 
+  // Volume term of one reference direction: Fbar[c][d] = gW[c] k[d] (rank one: the W gradient
+  // along the direction times a row of the inverse Jacobian, times the weight).  Then
+  // sum_{c,d} Fbar[c][d] dF[c][d]/dU = -d/dU sum_c gW[c] (flux(U) k)_c, and flux(U) k is the
+  // normal flux flux_n with n = k, so the adjoint is flux_n_bar along k: the same result as
+  // volume_vjp for this Fbar without the D x D loop.
+  __host__ __device__ static void volume_vjp_dir(const FormParams& P, const R* U, const R* k, const R* gW, R* Ubar) {
+    EulerPrim<D, R> s;
+    s.set(U, k, P);
+    EulerPrimBar<D, R> sb;
+    sb.zero();
+    const R ims = R(1.0 / (P.gamma * P.mach * P.mach));
+    R fb[V];
+#pragma unroll
+    for (int c = 0; c < V; c++) fb[c] = -gW[c];
+    flux_n_bar(s, k, ims, fb, sb);
+    sb.template pullback<false>(s, k, P, Ubar);
+  }
+
   // Inner face, own side i, neighbour o, normal n out of i.  fo = Fhat(Ui, Uo, n).
   // Writes fo (forward value) and accumulates Ubar_i += (dfo/dUi)^T gbar.
   template <bool HLLC>

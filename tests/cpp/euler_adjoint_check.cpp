@@ -50,6 +50,21 @@ int main() {
       for (int c = 0; c < V; c++) for (int d = 0; d < D; d++) ref += Fb[c][d] * F[c][d].d[q];
       worst_vol = std::fmax(worst_vol, std::fabs(ref - Ub[q]) / (std::fabs(ref) + 1e-3));
     }
+    // ---- volume, one reference direction: Fbar = gW (x) k through volume_vjp_dir
+    {
+      double gW[V], k[D], Fk[V][D], Ub1[V] = {0, 0, 0, 0, 0}, Ub2[V] = {0, 0, 0, 0, 0};
+      for (int c = 0; c < V; c++) gW[c] = wd(g);
+      for (int d = 0; d < D; d++) k[d] = wd(g);
+      for (int c = 0; c < V; c++) for (int d = 0; d < D; d++) Fk[c][d] = gW[c] * k[d];
+      EulerAdjoint<D, double>::volume_vjp_dir(P, Ui, k, gW, Ub1);
+      EulerAdjoint<D, double>::volume_vjp(P, Ui, Fk, Ub2);
+      for (int q = 0; q < V; q++) {
+        double ref = 0;
+        for (int c = 0; c < V; c++) for (int d = 0; d < D; d++) ref += Fk[c][d] * F[c][d].d[q];
+        worst_vol = std::fmax(worst_vol, std::fabs(ref - Ub1[q]) / (std::fabs(ref) + 1e-3));
+        worst_vol = std::fmax(worst_vol, std::fabs(Ub2[q] - Ub1[q]) / (std::fabs(Ub2[q]) + 1e-3));
+      }
+    }
     // ---- faces
     double gb[V];
     for (int c = 0; c < V; c++) gb[c] = wd(g);
