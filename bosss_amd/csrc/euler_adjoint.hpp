@@ -15,16 +15,35 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "forms.hpp"
 
 namespace bosss {
+
+// 1/x for normal, finite x to about one ulp: the hardware reciprocal estimate (v_rcp_f64,
+// about 23 correct bits) refined by two Newton-Raphson steps r <- r + r (1 - x r), each of
+// which doubles the number of correct bits.
+template <typename R>
+__host__ __device__ __forceinline__ R recip(R x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (std::is_same<R, double>::value) {
+    R r = __builtin_amdgcn_rcp(x);
+    R e = fma(-x, r, R(1));
+    r = fma(r, e, r);
+    e = fma(-x, r, R(1));
+    return fma(r, e, r);
+  }
+#endif
+  return R(1) / x;
+}
 
 template <int D, typename R>
 struct EulerPrim {
   R rho, m[D], E, irho, u[D], msq, p, a, un;
   __host__ __device__ __forceinline__ void set(const R* U, const R* n, const FormParams& P) {
     rho = U[0];
-    irho = R(1) / rho;
+    irho = recip(rho);
     msq = R(0);
 #pragma unroll
     for (int d = 0; d < D; d++) { m[d] = U[1 + d]; u[d] = m[d] * irho; msq += m[d] * m[d]; }
@@ -59,7 +78,7 @@ struct EulerPrimBar {
     R pb = p, irb = R(0);
     if constexpr (SOUND) {
       // a = sqrt(q)/Ma, q = p irho
-      R qb = a * R(0.5) / (s.a * R(P.mach * P.mach));
+      R qb = a * R(0.5 / (P.mach * P.mach)) * recip(s.a);
       pb += qb * s.irho;
       irb += qb * s.p;
     }
@@ -203,7 +222,7 @@ struct EulerAdjoint {
     const R cq = R(0.5) * (g + R(1)) / g;
     bool qi_on = pStar > si.p, qo_on = pStar > so.p;
     // shared reciprocals: every quotient below reuses one of ipi, ipo, iden, iA, iB, iqI, iqO
-    const R ipi = R(1) / si.p, ipo = R(1) / so.p;
+    const R ipi = recip(si.p), ipo = recip(so.p);
     R qI = qi_on ? sqrt((pStar * ipi - R(1)) * cq + R(1)) : R(1);
     R qO = qo_on ? sqrt((pStar * ipo - R(1)) * cq + R(1)) : R(1);
     R wIn = si.un - si.a * qI;
@@ -212,7 +231,7 @@ struct EulerAdjoint {
     R cOut = so.rho * (wOut - so.un);
     R num = cOut * so.un - cIn * si.un + (si.p - so.p) * ims;
     R den = cOut - cIn;
-    const R iden = R(1) / den;
+    const R iden = recip(den);
     R sm = num * iden;
     int branch;
     if (wIn > R(0)) branch = 0;
@@ -235,7 +254,7 @@ struct EulerAdjoint {
     const R wk = kin ? wIn : wOut;
     R A = s.rho * (wk - s.un);          // also the z of HLLCFlux's energy factor
     R B = wk - sm;
-    const R iA = R(1) / A, iB = R(1) / B;
+    const R iA = recip(A), iB = recip(B);
     R base = A * iB;
     R dsu = sm - s.un;
     R factor = sm * ms + s.p * iA;
@@ -320,12 +339,12 @@ struct EulerAdjoint {
     R qOb = wOutb * so.a;
     R pStarb = R(0);
     if (qi_on) {
-      R yb = qIb * R(0.5) / qI;
+      R yb = qIb * R(0.5) * recip(qI);
       pStarb += yb * cq * ipi;
       bi.p += -yb * cq * pStar * ipi * ipi;
     }
     if (qo_on) {
-      R yb = qOb * R(0.5) / qO;
+      R yb = qOb * R(0.5) * recip(qO);
       pStarb += yb * cq * ipo;
       bo.p += -yb * cq * pStar * ipo * ipo;
     }
