@@ -21,22 +21,6 @@
 
 namespace bosss {
 
-// 1/x for normal, finite x to about one ulp: the hardware reciprocal estimate (v_rcp_f64,
-// about 23 correct bits) refined by two Newton-Raphson steps r <- r + r (1 - x r), each of
-// which doubles the number of correct bits.
-template <typename R>
-__host__ __device__ __forceinline__ R recip(R x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  if constexpr (std::is_same<R, double>::value) {
-    R r = __builtin_amdgcn_rcp(x);
-    R e = fma(-x, r, R(1));
-    r = fma(r, e, r);
-    e = fma(-x, r, R(1));
-    return fma(r, e, r);
-  }
-#endif
-  return R(1) / x;
-}
 
 template <int D, typename R>
 struct EulerPrim {

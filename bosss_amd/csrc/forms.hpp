@@ -22,6 +22,30 @@ struct FormParams {
   double penalty_deg;  // max((p+1)^2, (p+1)(p+D)/D), SIPLaplace.CoefficientUpdate (ipLaplace.cs:100-113)
 };
 
+// 1/x for normal, finite x.  On the device: the hardware reciprocal estimate refined by
+// Newton-Raphson steps r <- r + r (1 - x r), each doubling the correct bits (FP64:
+// v_rcp_f64 gives about 23 bits, two steps reach the last ulp; FP32: v_rcp_f32 is within an
+// ulp, one step).  On the host: the quotient.
+__host__ __device__ __forceinline__ double recip(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+#else
+  return 1.0 / x;
+#endif
+}
+__host__ __device__ __forceinline__ float recip(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  float r = __builtin_amdgcn_rcpf(x);
+  return fmaf(r, fmaf(-x, r, 1.0f), r);
+#else
+  return 1.0f / x;
+#endif
+}
+
 // ----------------------------------------------------------------- dual numbers
 template <int M, typename R>
 struct Dual {
@@ -44,7 +68,7 @@ BOSSS_DUAL_BINOP(-, a.v - b.v, a.d[i] - b.d[i])
 BOSSS_DUAL_BINOP(*, a.v * b.v, a.d[i] * b.v + a.v * b.d[i])
 #undef BOSSS_DUAL_BINOP
 template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> operator/(const Dual<M, R>& a, const Dual<M, R>& b) {
-  Dual<M, R> r; R ib = R(1) / b.v; r.v = a.v * ib;
+  Dual<M, R> r; R ib = recip(b.v); r.v = a.v * ib;
 #pragma unroll
   for (int i = 0; i < M; i++) r.d[i] = (a.d[i] - r.v * b.d[i]) * ib;
   return r;
@@ -66,10 +90,11 @@ template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> oper
 template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> operator+(R s, const Dual<M, R>& a) { Dual<M, R> r = a; r.v += s; return r; }
 template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> operator-(const Dual<M, R>& a, R s) { Dual<M, R> r = a; r.v -= s; return r; }
 template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> operator-(R s, const Dual<M, R>& a) { Dual<M, R> r = -a; r.v += s; return r; }
-template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> operator/(const Dual<M, R>& a, R s) { return a * (R(1) / s); }
+template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> operator/(const Dual<M, R>& a, R s) { return a * recip(s); }
+template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> recip(const Dual<M, R>& b) { return dconst<M, R>(R(1)) / b; }
 template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> operator/(R s, const Dual<M, R>& b) { return dconst<M, R>(s) / b; }
 template <int M, typename R> __host__ __device__ __forceinline__ Dual<M, R> dsqrt(const Dual<M, R>& a) {
-  Dual<M, R> r; r.v = sqrt(a.v); R f = R(0.5) / r.v;
+  Dual<M, R> r; r.v = sqrt(a.v); R f = R(0.5) * recip(r.v);
 #pragma unroll
   for (int i = 0; i < M; i++) r.d[i] = a.d[i] * f;
   return r;
@@ -181,7 +206,7 @@ struct EulerState {
   T rho, m[D], E, u[D], p, a, irho;
   __host__ __device__ __forceinline__ EulerState(const T* U, const FormParams& P) {
     rho = U[0];
-    irho = R(1) / rho;
+    irho = recip(rho);
 #pragma unroll
     for (int d = 0; d < D; d++) { m[d] = U[1 + d]; u[d] = m[d] * irho; }
     E = U[D + 1];
@@ -263,14 +288,14 @@ struct FormEuler {
       pStar = vmax(tconst<T>(R(0)), pStar);
       T qIn = tconst<T>(R(1)), qOut = tconst<T>(R(1));
       const R cq = R(0.5) * (g + R(1)) / g;
-      if (value(pStar) > value(si.p)) qIn = dsqrt((pStar / si.p - R(1)) * cq + R(1));
-      if (value(pStar) > value(so.p)) qOut = dsqrt((pStar / so.p - R(1)) * cq + R(1));
+      if (value(pStar) > value(si.p)) qIn = dsqrt((pStar * recip(si.p) - R(1)) * cq + R(1));
+      if (value(pStar) > value(so.p)) qOut = dsqrt((pStar * recip(so.p) - R(1)) * cq + R(1));
       T wIn = uni - si.a * qIn;
       T wOut = uno + so.a * qOut;
       // HLLCFlux.InnerEdgeFlux (HLLCFlux.cs:48-86, Toro 10.70/10.71)
       T cIn = si.rho * (wIn - uni);
       T cOut = so.rho * (wOut - uno);
-      T sm = (cOut * uno - cIn * uni + (si.p - so.p) * ims) / (cOut - cIn);
+      T sm = (cOut * uno - cIn * uni + (si.p - so.p) * ims) * recip(cOut - cIn);
       int branch;
       if (value(wIn) > R(0)) branch = 0;
       else if (R(0) < value(sm)) branch = 1;
@@ -283,8 +308,8 @@ struct FormEuler {
         const T& wk = (branch == 1) ? wIn : wOut;
         // HLLC{Density,Momentum,Energy}Flux.GetModifiedVariableValue
         T A = s.rho * (wk - uk);
-        T iA = R(1) / A;
-        T base = A / (wk - sm);
+        T iA = recip(A);
+        T base = A * recip(wk - sm);
         T dsu = sm - uk;
         fo[0] = fo[0] + wk * (base - s.rho);
 #pragma unroll
