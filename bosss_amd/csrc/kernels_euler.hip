@@ -4,8 +4,8 @@
 #include "registry.hpp"
 
 namespace bosss {
-// The fused residual + VJP kernels of 3D p = 3 are compiled in kernels_euler_fused.hip, with
-// the memory-clause scheduling strategy that batches their coefficient loads.
+// The fused residual + VJP kernels of 3D p = 3 are compiled in their own translation unit,
+// kernels_euler_fused.hip (built in parallel with this one).
 extern template __global__ void kt_vjp_euler<FormEuler<3, false>, 3, 3, 5, double, true>(TArgs<double>);
 extern template __global__ void kt_vjp_euler<FormEuler<3, true>, 3, 3, 5, double, true>(TArgs<double>);
 
