@@ -128,7 +128,9 @@ class ImplicitStep:
             out.add_(v, alpha=inv_dt)
         return apply
 
-    def solve_fp64(self, U0, tol=1e-10, inner_tol=1e-4, max_newton=20, restart=40, maxit=400):
+    def solve_fp64(self, U0, tol=1e-10, inner_tol=1e-4, max_newton=20, restart=40, maxit=400, precond=None):
+        """Newton-GMRES in FP64.  precond(v, z): optional right preconditioner, z ~ (I/dt + J)^-1 v
+        (e.g. the cell-block Jacobi inverse of the assembled diagonal blocks, frozen)."""
         import torch
         U = U0.clone()
         r = torch.empty_like(U)
@@ -140,15 +142,16 @@ class ImplicitStep:
             hist.append(rn)
             if rn <= tol:
                 break
-            d, its, _ = gmres(self._jac(self.op64, U), -r, inner_tol, restart, maxit)
+            d, its, _ = gmres(self._jac(self.op64, U), -r, inner_tol, restart, maxit, precond=precond)
             n_jvp += its
             U += d
         _sync(U)
         self.stats = dict(history=hist, jvps=n_jvp, seconds=time.perf_counter() - t0, newton=len(hist) - 1)
         return U
 
-    def solve_mixed(self, U0, tol=1e-10, inner_tol=1e-4, max_outer=20, restart=40, maxit=400):
-        """FP64 defect correction around an FP32 Newton-GMRES correction."""
+    def solve_mixed(self, U0, tol=1e-10, inner_tol=1e-4, max_outer=20, restart=40, maxit=400, precond=None):
+        """FP64 defect correction around an FP32 Newton-GMRES correction.  precond(v, z) acts on
+        the FP32 Krylov vectors."""
         import torch
         if self.op32 is None:
             raise ValueError("solve_mixed needs an FP32 operator")
@@ -164,7 +167,7 @@ class ImplicitStep:
                 break
             U32 = U.float()
             b32 = (-r / rn).float()                        # scaled so FP32 never underflows
-            d32, its, _ = gmres(self._jac(self.op32, U32), b32, inner_tol, restart, maxit)
+            d32, its, _ = gmres(self._jac(self.op32, U32), b32, inner_tol, restart, maxit, precond=precond)
             n_jvp += its
             U += d32.double() * rn
         _sync(U)

@@ -55,6 +55,44 @@ def test_newton_fp64_and_mixed_reach_1e10():
     assert np.linalg.norm(F) <= 2e-10
 
 
+def test_newton_with_cell_block_jacobi_preconditioner():
+    """Right preconditioning by the inverted diagonal cell blocks of I/dt + J(U_n) (assembled
+    by the coloured exact JVPs, inverted by the batched LU): same converged state, fewer
+    Krylov iterations; FP64 and the FP32-inner defect correction."""
+    import torch
+    from bosss_amd.blocks import BlockJacobi
+    from bosss_amd.newton import ImplicitStep
+    say("case newton block-Jacobi preconditioner")
+    g, op64, op32, Un, dt = setup(n=4)
+    st = ImplicitStep(op64, Un, dt, op32=op32)
+    U0 = st.solve_fp64(Un, tol=1e-10)
+    s0 = dict(st.stats)
+    ev = op64.GetMatrixBuilder()
+    B = ev.new_matrix()
+    op64.assemble_dev(Un, B)
+    Nt, S = op64.nvars * op64.N, ev.S
+    B.view(g.J, S, Nt, Nt)[:, 0] += torch.eye(Nt, dtype=B.dtype, device=B.device) / dt
+    bj = BlockJacobi(op64, B)
+    bj.check()
+    U1 = st.solve_fp64(Un, tol=1e-10, precond=bj.apply_inverse_diagonal)
+    s1 = dict(st.stats)
+
+    def precond32(v, z):
+        z64 = torch.empty(v.shape, dtype=torch.float64, device=v.device)
+        bj.apply_inverse_diagonal(v.double(), z64)
+        z.copy_(z64)
+
+    Um = st.solve_mixed(Un, tol=1e-10, precond=precond32)
+    sm = dict(st.stats)
+    say("  plain      :", s0["jvps"], "JVPs", s0["history"][-1])
+    say("  block-Jac  :", s1["jvps"], "JVPs", s1["history"][-1])
+    say("  mixed + bJ :", sm["jvps"], "JVPs", sm["history"][-1])
+    assert s1["history"][-1] <= 1e-10 and sm["history"][-1] <= 1e-10
+    assert s1["jvps"] < s0["jvps"]
+    assert float(torch.linalg.vector_norm(U1 - U0)) <= 1e-9 * dt * 10
+    assert float(torch.linalg.vector_norm(Um - U0)) <= 1e-9 * dt * 10
+
+
 def test_adjoint_gradient_matches_finite_differences():
     import torch
     from bosss_amd.newton import ImplicitStep, pressure_functional
