@@ -58,6 +58,14 @@ def test_assembled_blocks_and_spmv(form, D, n, p, per):
     err = np.abs(y.cpu().numpy() - ref_y).max() / np.abs(ref_y).max()
     say("  colours %d, spmv rel err %.2e" % (ev.ncolors, err))
     assert err < 1e-12
+    # y <- alpha A x + beta y (BlockMsrMatrix.SpMV semantics), y non-zero on entry
+    y0 = np.random.default_rng(5).uniform(-1, 1, op.ndof)
+    y2 = torch.from_numpy(y0).cuda()
+    op.spmv_dev(B, xt, y2, alpha=-2.0, beta=0.5)
+    want = -2.0 * ref_y + 0.5 * y0
+    err2 = np.abs(y2.cpu().numpy() - want).max() / np.abs(want).max()
+    say("  alpha/beta spmv rel err %.2e" % err2)
+    assert err2 < 1e-12
 
 
 def test_block_jacobi_and_pcg_on_sip():
