@@ -71,32 +71,57 @@ struct XArgs {
 };
 
 // normalised Legendre values/derivatives L_k(x) = sqrt((2k+1)/2) P_k(x), k <= P
-__device__ inline void legendre(double x, int P, double* L, double* dL) {
+__device__ inline void legendre(double x, double (&L)[4], double (&dL)[4]) {
+  // all degrees up to 3 (the kernels' maximum) so every index below is a compile-time constant
   double p0 = 1.0, p1 = x, d0 = 0.0, d1 = 1.0;
   L[0] = p0; dL[0] = d0;
-  if (P >= 1) { L[1] = p1; dL[1] = d1; }
-  for (int k = 1; k < P; k++) {
+  L[1] = p1; dL[1] = d1;
+#pragma unroll
+  for (int k = 1; k < 3; k++) {
     double p2 = ((2 * k + 1) * x * p1 - k * p0) / (k + 1);
     double d2 = d0 + (2 * k + 1) * p1;
     p0 = p1; p1 = p2; d0 = d1; d1 = d2;
     L[k + 1] = p2; dL[k + 1] = d2;
   }
-  for (int k = 0; k <= P; k++) { double s = sqrt((2 * k + 1) * 0.5); L[k] *= s; dL[k] *= s; }
+#pragma unroll
+  for (int k = 0; k < 4; k++) { double s = sqrt((2 * k + 1) * 0.5); L[k] *= s; dL[k] *= s; }
+}
+
+// table[e] for e in 0..3 by selects (a run-time index into a private array would put the
+// array in scratch memory)
+__device__ inline double pick4(const double (&t)[4], int e) { return e == 0 ? t[0] : (e == 1 ? t[1] : (e == 2 ? t[2] : t[3])); }
+
+// Per-dimension factors of mode n at the staged point: Lv[d] = L_{e_d}(xi_d), dLv[d] its
+// derivative, with Lv = 1 for the dimensions beyond D.
+__device__ inline void mode_factors(const XArgs& a, int n, const double (&L)[XD][4], const double (&dL)[XD][4], double (&Lv)[XD],
+                                    double (&dLv)[XD]) {
+  const int32_t* e = a.expo + n * a.D;
+#pragma unroll
+  for (int d = 0; d < XD; d++) {
+    const int ed = d < a.D ? e[d] : 0;
+    Lv[d] = d < a.D ? pick4(L[d], ed) : 1.0;
+    dLv[d] = pick4(dL[d], ed);
+  }
 }
 
 // physical basis values v[n] and gradients g[n][d] of cell j at reference point xi
 __device__ inline void phys_basis(const XArgs& a, int j, const double* xi, double* v, double (*g)[XD]) {
   double L[XD][4], dL[XD][4];
-  for (int d = 0; d < a.D; d++) legendre(xi[d], a.P, L[d], dL[d]);
+#pragma unroll
+  for (int d = 0; d < XD; d++) legendre(d < a.D ? xi[d] : 0.0, L[d], dL[d]);
   const double sc = a.scaling[j];
   const double* iJ = a.invJ + (size_t)j * a.D * a.D;
   for (int n = 0; n < a.N; n++) {
-    const int32_t* e = a.expo + n * a.D;
+    double Lv[XD], dLv[XD];
+    mode_factors(a, n, L, dL, Lv, dLv);
     double val = 1.0, gr[XD];
-    for (int d = 0; d < a.D; d++) val *= L[d][e[d]];
-    for (int k = 0; k < a.D; k++) {
-      double t = dL[k][e[k]];
-      for (int d = 0; d < a.D; d++) if (d != k) t *= L[d][e[d]];
+#pragma unroll
+    for (int d = 0; d < XD; d++) val *= Lv[d];
+#pragma unroll
+    for (int k = 0; k < XD; k++) {
+      double t = dLv[k];
+#pragma unroll
+      for (int d = 0; d < XD; d++) if (d != k) t *= Lv[d];
       gr[k] = t;
     }
     v[n] = sc * val;
@@ -108,6 +133,40 @@ __device__ inline void phys_basis(const XArgs& a, int j, const double* xi, doubl
   }
 }
 
+// Values and normal derivatives grad(phi_n) . nrm of the physical basis at one node (what the
+// face and interface forms contract; the full gradient is not needed there).
+__device__ inline void phys_basis_n(const XArgs& a, int j, const double* xi, const double* nrm, double* v, double* gn) {
+  double L[XD][4], dL[XD][4];
+#pragma unroll
+  for (int d = 0; d < XD; d++) legendre(d < a.D ? xi[d] : 0.0, L[d], dL[d]);
+  const double sc = a.scaling[j];
+  const double* iJ = a.invJ + (size_t)j * a.D * a.D;
+  double in_[XD];   // invJ . nrm: d(phi)/dn = sum_k d(phi)/dxi_k (invJ . nrm)_k
+#pragma unroll
+  for (int k = 0; k < XD; k++) {
+    double t = 0.0;
+    if (k < a.D)
+      for (int d = 0; d < a.D; d++) t += iJ[k * a.D + d] * nrm[d];
+    in_[k] = t;
+  }
+  for (int n = 0; n < a.N; n++) {
+    double Lv[XD], dLv[XD];
+    mode_factors(a, n, L, dL, Lv, dLv);
+    double val = 1.0, s = 0.0;
+#pragma unroll
+    for (int d = 0; d < XD; d++) val *= Lv[d];
+#pragma unroll
+    for (int k = 0; k < XD; k++) {
+      double t = dLv[k];
+#pragma unroll
+      for (int d = 0; d < XD; d++) if (d != k) t *= Lv[d];
+      s += t * in_[k];
+    }
+    v[n] = sc * val;
+    gn[n] = sc * s;
+  }
+}
+
 __device__ inline int spec_off(const XArgs& a, int j, int sp) { return a.spec0[j] == sp ? 0 : a.N; }
 __device__ inline int nrow(const XArgs& a, int j) { return a.nspec[j] * a.N; }
 __device__ inline double* block_at(const XArgs& a, double* blocks, int j, int slot, int r, int c) {
@@ -115,10 +174,15 @@ __device__ inline double* block_at(const XArgs& a, double* blocks, int j, int sl
   return blocks + a.boff[(size_t)j * a.S + slot] + (int64_t)r * nrow(a, col) + c;
 }
 
-// staged basis of up to 64 nodes: values [64][XN], gradients [64][XN][XD]
+// staged basis of up to XCHUNK nodes: values [XCHUNK][XN], gradients [XCHUNK][XN][XD]
 struct Stage {
   double v[XCHUNK][XN];
   double g[XCHUNK][XN][XD];
+};
+// staged values and normal derivatives of up to XCHUNK nodes (faces, interface)
+struct NStage {
+  double v[XCHUNK][XN];
+  double gn[XCHUNK][XN];
 };
 
 __global__ void __launch_bounds__(64) k_xdg_volume(XArgs a, const int32_t* __restrict__ cell, const int32_t* __restrict__ spec,
@@ -140,26 +204,28 @@ __global__ void __launch_bounds__(64) k_xdg_volume(XArgs a, const int32_t* __res
       sw[lane] = w[q0 + lane];
     }
     __syncthreads();
+#pragma unroll
     for (int i = 0; i < XPAIRS; i++) {
       const int pr = lane + 64 * i;
-      if (pr >= NP) break;
-      const int r = pr / N, c = pr % N;
-      double s = 0.0;
-      for (int q = 0; q < nq; q++) {
-        double dot = 0.0;
-        for (int d = 0; d < a.D; d++) dot += st.g[q][c][d] * st.g[q][r][d];
-        s -= sw[q] * mu * dot;
+      if (pr < NP) {
+        const int r = pr / N, c = pr % N;
+        double s = 0.0;
+        for (int q = 0; q < nq; q++) {
+          double dot = 0.0;
+          for (int d = 0; d < a.D; d++) dot += st.g[q][c][d] * st.g[q][r][d];
+          s -= sw[q] * mu * dot;
+        }
+        acc[i] += s;
       }
-      acc[i] += s;
     }
     if (lane < N)
       for (int q = 0; q < nq; q++) aff -= sw[q] * a.src[sp] * st.v[q][lane];
     __syncthreads();
   }
+#pragma unroll
   for (int i = 0; i < XPAIRS; i++) {
     const int pr = lane + 64 * i;
-    if (pr >= NP) break;
-    *block_at(a, blocks, j, 0, o + pr / N, o + pr % N) += acc[i];
+    if (pr < NP) *block_at(a, blocks, j, 0, o + pr / N, o + pr % N) += acc[i];
   }
   if (lane < N) affine[a.dof[j] + o + lane] += aff;
 }
@@ -170,7 +236,7 @@ __global__ void __launch_bounds__(64) k_xdg_face(XArgs a, const int32_t* __restr
                                                  const double* __restrict__ w, const double* __restrict__ normal,
                                                  const double* __restrict__ gD, double* __restrict__ blocks,
                                                  double* __restrict__ affine) {
-  __shared__ Stage si, so;
+  __shared__ NStage si, so;
   __shared__ double sw[XCHUNK], sg[XCHUNK];
   const int it = blockIdx.x, lane = threadIdx.x;
   const int ji = cells[2 * it], jo = cells[2 * it + 1], sp = spec[it];
@@ -182,67 +248,65 @@ __global__ void __launch_bounds__(64) k_xdg_face(XArgs a, const int32_t* __restr
   const double eta = inner ? a.penalty_base * fmax(1.0 / a.lscale[2 * ji + sp], 1.0 / a.lscale[2 * jo + sp]) * mu * a.pdeg : 0.0;
   const double pn = 2.0 * a.penalty_base * a.pdeg / a.lscale[2 * ji + sp];
   double acc[XPAIRS][4];
-  for (int i = 0; i < XPAIRS; i++) for (int k = 0; k < 4; k++) acc[i][k] = 0.0;
+#pragma unroll
+  for (int i = 0; i < XPAIRS; i++)
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[i][k] = 0.0;
   double aff = 0.0;
   for (int64_t q0 = off[it]; q0 < off[it + 1]; q0 += XCHUNK) {
     const int nq = (int)std::min<int64_t>(XCHUNK, off[it + 1] - q0);
     if (lane < nq) {
-      phys_basis(a, ji, nin + (q0 + lane) * D, si.v[lane], si.g[lane]);
-      if (inner) phys_basis(a, jo, nout + (q0 + lane) * D, so.v[lane], so.g[lane]);
+      phys_basis_n(a, ji, nin + (q0 + lane) * D, nrm, si.v[lane], si.gn[lane]);
+      if (inner) phys_basis_n(a, jo, nout + (q0 + lane) * D, nrm, so.v[lane], so.gn[lane]);
       sw[lane] = w[q0 + lane];
       sg[lane] = gD ? gD[q0 + lane] : 0.0;
     }
     __syncthreads();
+#pragma unroll
     for (int i = 0; i < XPAIRS; i++) {
       const int pr = lane + 64 * i;
-      if (pr >= NP) break;
-      const int r = pr / N, c = pr % N;
-      for (int q = 0; q < nq; q++) {
-        double gri = 0, gci = 0, gro = 0, gco = 0;
-        for (int d = 0; d < D; d++) {
-          gri += si.g[q][r][d] * nrm[d];
-          gci += si.g[q][c][d] * nrm[d];
-          if (inner) { gro += so.g[q][r][d] * nrm[d]; gco += so.g[q][c][d] * nrm[d]; }
-        }
-        const double ww = sw[q];
-        if (inner) {
-          // (test side ts, trial side us): vA - vB = s_t phi_r, uA - uB = s_u phi_c
-          const double vr[2] = {si.v[q][r], so.v[q][r]}, vc[2] = {si.v[q][c], so.v[q][c]};
-          const double gr[2] = {gri, gro}, gc[2] = {gci, gco};
+      if (pr < NP) {
+        const int r = pr / N, c = pr % N;
+        for (int q = 0; q < nq; q++) {
+          const double ww = sw[q];
+          const double gri = si.gn[q][r], gci = si.gn[q][c];
+          if (inner) {
+            // (test side ts, trial side us): vA - vB = s_t phi_r, uA - uB = s_u phi_c
+            const double vr[2] = {si.v[q][r], so.v[q][r]}, vc[2] = {si.v[q][c], so.v[q][c]};
+            const double gr[2] = {gri, so.gn[q][r]}, gc[2] = {gci, so.gn[q][c]};
 #pragma unroll
-          for (int ts = 0; ts < 2; ts++)
+            for (int ts = 0; ts < 2; ts++)
 #pragma unroll
-            for (int us = 0; us < 2; us++) {
-              const double st_ = ts ? -1.0 : 1.0, su = us ? -1.0 : 1.0;
-              double f = 0.5 * mu * gc[us] * st_ * vr[ts] + 0.5 * mu * gr[ts] * su * vc[us] - eta * su * st_ * vc[us] * vr[ts];
-              acc[i][ts * 2 + us] += ww * f;
-            }
-        } else {
-          const double vr = si.v[q][r], vc = si.v[q][c];
-          acc[i][0] += ww * (mu * gci * vr + mu * gri * vc - mu * vc * vr * pn);
+              for (int us = 0; us < 2; us++) {
+                const double st_ = ts ? -1.0 : 1.0, su = us ? -1.0 : 1.0;
+                double f = 0.5 * mu * gc[us] * st_ * vr[ts] + 0.5 * mu * gr[ts] * su * vc[us] - eta * su * st_ * vc[us] * vr[ts];
+                acc[i][ts * 2 + us] += ww * f;
+              }
+          } else {
+            const double vr = si.v[q][r], vc = si.v[q][c];
+            acc[i][0] += ww * (mu * gci * vr + mu * gri * vc - mu * vc * vr * pn);
+          }
         }
       }
     }
     if (!inner && lane < N)
-      for (int q = 0; q < nq; q++) {
-        double gvn = 0;
-        for (int d = 0; d < D; d++) gvn += si.g[q][lane][d] * nrm[d];
-        aff += sw[q] * (-mu * gvn * sg[q] + mu * sg[q] * si.v[q][lane] * pn);
-      }
+      for (int q = 0; q < nq; q++) aff += sw[q] * (-mu * si.gn[q][lane] * sg[q] + mu * sg[q] * si.v[q][lane] * pn);
     __syncthreads();
   }
   const int oi = spec_off(a, ji, sp), oo = inner ? spec_off(a, jo, sp) : 0;
+#pragma unroll
   for (int i = 0; i < XPAIRS; i++) {
     const int pr = lane + 64 * i;
-    if (pr >= NP) break;
-    const int r = pr / N, c = pr % N;
-    if (inner) {
-      unsafeAtomicAdd(block_at(a, blocks, ji, 0, oi + r, oi + c), acc[i][0]);
-      unsafeAtomicAdd(block_at(a, blocks, ji, fslot[2 * it], oi + r, oo + c), acc[i][1]);
-      unsafeAtomicAdd(block_at(a, blocks, jo, fslot[2 * it + 1], oo + r, oi + c), acc[i][2]);
-      unsafeAtomicAdd(block_at(a, blocks, jo, 0, oo + r, oo + c), acc[i][3]);
-    } else {
-      unsafeAtomicAdd(block_at(a, blocks, ji, 0, oi + r, oi + c), acc[i][0]);
+    if (pr < NP) {
+      const int r = pr / N, c = pr % N;
+      if (inner) {
+        unsafeAtomicAdd(block_at(a, blocks, ji, 0, oi + r, oi + c), acc[i][0]);
+        unsafeAtomicAdd(block_at(a, blocks, ji, fslot[2 * it], oi + r, oo + c), acc[i][1]);
+        unsafeAtomicAdd(block_at(a, blocks, jo, fslot[2 * it + 1], oo + r, oi + c), acc[i][2]);
+        unsafeAtomicAdd(block_at(a, blocks, jo, 0, oo + r, oo + c), acc[i][3]);
+      } else {
+        unsafeAtomicAdd(block_at(a, blocks, ji, 0, oi + r, oi + c), acc[i][0]);
+      }
     }
   }
   if (!inner && lane < N) unsafeAtomicAdd(affine + a.dof[ji] + oi + lane, aff);
@@ -251,8 +315,8 @@ __global__ void __launch_bounds__(64) k_xdg_face(XArgs a, const int32_t* __restr
 __global__ void __launch_bounds__(64) k_xdg_interface(XArgs a, const int32_t* __restrict__ cell, const int64_t* __restrict__ off,
                                                       const double* __restrict__ nodes, const double* __restrict__ w,
                                                       const double* __restrict__ normal, double* __restrict__ blocks) {
-  __shared__ Stage st;
-  __shared__ double sw[XCHUNK], sn[XCHUNK][XD];
+  __shared__ NStage st;
+  __shared__ double sw[XCHUNK];
   const int it = blockIdx.x, lane = threadIdx.x;
   const int j = cell[it];
   if (a.nspec[j] != 2) return;
@@ -262,42 +326,49 @@ __global__ void __launch_bounds__(64) k_xdg_interface(XArgs a, const int32_t* __
   const double eta = a.penalty_base * fmax(1.0 / a.lscale[2 * j], 1.0 / a.lscale[2 * j + 1]) * pmu * a.pdeg;
   const double muS[2] = {muA, muB};
   double acc[XPAIRS][4];
-  for (int i = 0; i < XPAIRS; i++) for (int k = 0; k < 4; k++) acc[i][k] = 0.0;
+#pragma unroll
+  for (int i = 0; i < XPAIRS; i++)
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[i][k] = 0.0;
   for (int64_t q0 = off[it]; q0 < off[it + 1]; q0 += XCHUNK) {
     const int nq = (int)std::min<int64_t>(XCHUNK, off[it + 1] - q0);
     if (lane < nq) {
-      phys_basis(a, j, nodes + (q0 + lane) * D, st.v[lane], st.g[lane]);
+      phys_basis_n(a, j, nodes + (q0 + lane) * D, normal + (q0 + lane) * D, st.v[lane], st.gn[lane]);
       sw[lane] = w[q0 + lane];
-      for (int d = 0; d < D; d++) sn[lane][d] = normal[(q0 + lane) * D + d];
     }
     __syncthreads();
+#pragma unroll
     for (int i = 0; i < XPAIRS; i++) {
       const int pr = lane + 64 * i;
-      if (pr >= NP) break;
-      const int r = pr / N, c = pr % N;
-      for (int q = 0; q < nq; q++) {
-        double gr = 0, gc = 0;
-        for (int d = 0; d < D; d++) { gr += st.g[q][r][d] * sn[q][d]; gc += st.g[q][c][d] * sn[q][d]; }
-        const double vr = st.v[q][r], vc = st.v[q][c], ww = sw[q];
+      if (pr < NP) {
+        const int r = pr / N, c = pr % N;
+        for (int q = 0; q < nq; q++) {
+          const double gr = st.gn[q][r], gc = st.gn[q][c];
+          const double vr = st.v[q][r], vc = st.v[q][c], ww = sw[q];
 #pragma unroll
-        for (int ts = 0; ts < 2; ts++)
+          for (int ts = 0; ts < 2; ts++)
 #pragma unroll
-          for (int us = 0; us < 2; us++) {
-            const double st_ = ts ? -1.0 : 1.0, su = us ? -1.0 : 1.0;
-            double f = muS[us] * 0.5 * gc * st_ * vr + muS[ts] * 0.5 * gr * su * vc - eta * su * st_ * vc * vr;
-            acc[i][ts * 2 + us] += ww * f;
-          }
+            for (int us = 0; us < 2; us++) {
+              const double st_ = ts ? -1.0 : 1.0, su = us ? -1.0 : 1.0;
+              double f = muS[us] * 0.5 * gc * st_ * vr + muS[ts] * 0.5 * gr * su * vc - eta * su * st_ * vc * vr;
+              acc[i][ts * 2 + us] += ww * f;
+            }
+        }
       }
     }
     __syncthreads();
   }
   const int o[2] = {spec_off(a, j, 0), spec_off(a, j, 1)};
+#pragma unroll
   for (int i = 0; i < XPAIRS; i++) {
     const int pr = lane + 64 * i;
-    if (pr >= NP) break;
-    const int r = pr / N, c = pr % N;
-    for (int ts = 0; ts < 2; ts++)
-      for (int us = 0; us < 2; us++) *block_at(a, blocks, j, 0, o[ts] + r, o[us] + c) += acc[i][ts * 2 + us];
+    if (pr < NP) {
+      const int r = pr / N, c = pr % N;
+#pragma unroll
+      for (int ts = 0; ts < 2; ts++)
+#pragma unroll
+        for (int us = 0; us < 2; us++) *block_at(a, blocks, j, 0, o[ts] + r, o[us] + c) += acc[i][ts * 2 + us];
+    }
   }
 }
 
