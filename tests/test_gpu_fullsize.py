@@ -1,4 +1,5 @@
-"""The 64^3 p = 3 Euler HLLC workload (26 214 400 DOF) at full size.
+"""The measured workloads at full size: Euler HLLC p = 3 on 64^3 (26 214 400 DOF), upwind
+advection + SIP p = 3 on 64^3, XDG Poisson on the 32^3 sphere.
 
 The oracle finishes this size in a few seconds on the host's cores, so the fused residual +
 VJP kernel and the FP64 JVP kernel are compared with it entry by entry (1e-12 per field, as
@@ -132,3 +133,81 @@ def test_free_stream_is_preserved_at_full_size(case):
     worst = float(Rc.abs().max()) / float(R.abs().max())
     say("  max |R(constant)| / max |R(U)| = %.2e" % worst)
     assert worst <= 1e-11
+
+
+def test_advdiff_p3_64cube_fp64_fp32_and_assembled_at_full_size():
+    """Upwind advection + SIP (nu = 0.01), 64^3 periodic, p = 3 (5.2 M DOF): the matrix-free
+    residual in FP64 against the oracle (1e-12) and in FP32 (error stated, bound 1e-5), and
+    the assembled block-ELL apply M U + b against the matrix-free residual."""
+    import torch
+    from bosss_amd.grid import CartesianGrid
+    from bosss_amd.operator import DifferentialOperator, default_quad_order
+    from bosss_amd.workloads import uniform_coefficients
+    from oracle.bosss_ref import RefModel, params_vector, OP_ADVDIFF
+    say("case fullsize advdiff p=3 64^3")
+    n, p, nu, vel = N_CELLS, 3, -0.01, (1.0, 0.5, 0.25)
+    g = CartesianGrid((n,) * 3, periodic=[True] * 3)
+    op = DifferentialOperator(g, "advdiff", p, penalty=1.3, nu=nu, velocity=vel)
+    op32 = DifferentialOperator(g, "advdiff", p, penalty=1.3, nu=nu, velocity=vel, precision="fp32")
+    U = uniform_coefficients(op.ndof, 3)
+    Ut = torch.from_numpy(U).cuda()
+    R = torch.zeros_like(Ut)
+    op.evaluate_dev(Ut, R)
+    R32 = torch.zeros(op.ndof, dtype=torch.float32, device=Ut.device)
+    op32.evaluate_dev(Ut.float(), R32)
+    torch.cuda.synchronize()
+    ref = RefModel(3, (n,) * 3, p, default_quad_order("advdiff", p), periodic=[1, 1, 1])
+    Rref = ref.evaluate(OP_ADVDIFF, params_vector(1.3, nu, vel, 1.4), U)
+    pf, nw = rel_err(R.cpu().numpy(), Rref, 1, op.N)
+    say("  FP64 per-field %.2e normwise %.2e" % (pf, nw))
+    assert pf < TOL and nw < TOL
+    pf32, nw32 = rel_err(R32.double().cpu().numpy(), Rref, 1, op.N)
+    say("  FP32 per-field %.2e normwise %.2e" % (pf32, nw32))
+    assert pf32 < 1e-5
+    # the operator is linear: R(U) = M U + b with the assembled matrix (b = R(0))
+    ev = op.GetMatrixBuilder()
+    B = ev.new_matrix()
+    op.assemble_dev(torch.zeros_like(Ut), B)
+    y = torch.zeros_like(Ut)
+    op.evaluate_dev(torch.zeros_like(Ut), y)
+    op.spmv_dev(B, Ut, y, alpha=1.0, beta=1.0)
+    torch.cuda.synchronize()
+    d = float((y - R).abs().max()) / float(R.abs().max())
+    say("  |M U + b - R(U)| / |R| = %.2e" % d)
+    assert d < 1e-12
+    del B
+
+
+def test_xdg_sphere_32cube_assembly_and_solves_at_full_size():
+    """XDG Poisson on the 32^3 sphere (p = 2, 345 k DOF): GPU assembly against the oracle's
+    restated assembly (1e-12), and both block-Jacobi PCG solvers to ||b - M x|| <= 1e-10 ||b||
+    with the residual recomputed from the oracle's matrix."""
+    import torch
+    from bosss_amd.grid import CartesianGrid
+    from bosss_amd.xdg import QuadLevelSet, XdgRules, XdgOperator
+    from oracle.bosss_ref import xdg_assemble, xdg_to_csr
+    say("case fullsize xdg sphere 32^3")
+    g = CartesianGrid((32,) * 3)
+    rules = XdgRules(g, QuadLevelSet.sphere([0.5] * 3, 0.3), n=3)
+    mu, src = (-1.0, -10.0), (1.0, 1.0)
+    op = XdgOperator(rules, 2, mu, source=src, penalty=2.0, g_D=None)
+    B = op.new_matrix()
+    a = torch.zeros(op.ndof, dtype=torch.float64, device="cuda:0")
+    op.assemble_dev(B, a)
+    torch.cuda.synchronize()
+    Bref, aref, lay = xdg_assemble(rules, 2, mu, src, penalty=2.0, gD=rules.boundary_values(None))
+    eb = np.abs(B.cpu().numpy() - Bref).max() / np.abs(Bref).max()
+    ea = np.abs(a.cpu().numpy() - aref).max() / np.abs(aref).max()
+    say("  blocks %.2e affine %.2e" % (eb, ea))
+    assert eb < 1e-12 and ea < 1e-12
+    M = xdg_to_csr(rules, lay, Bref)
+    b = -a
+    bh = b.cpu().numpy()
+    for prec in ("fp64", "fp32"):
+        x = torch.zeros_like(a)
+        it, outer, rel = op.solve_dev(B, b, x, tol=1e-10, maxit=50000, precision=prec)
+        torch.cuda.synchronize()
+        true_rel = np.linalg.norm(bh - M @ x.cpu().numpy()) / np.linalg.norm(bh)
+        say("  %s: %d iterations, %d outer, reported %.2e, oracle-matrix residual %.2e" % (prec, it, outer, rel, true_rel))
+        assert rel <= 1e-10 and true_rel <= 2e-10
+    op.close()
