@@ -1,5 +1,6 @@
 """The measured workloads at full size: Euler HLLC p = 3 on 64^3 (26 214 400 DOF), upwind
-advection + SIP p = 3 on 64^3, XDG Poisson on the 32^3 sphere.
+advection + SIP p = 3 on 64^3, XDG Poisson on the 32^3 sphere, SIP p = 2 on 64^2 and Euler
+LF / HLLC p = 2 on 32^3.
 
 The oracle finishes this size in a few seconds on the host's cores, so the fused residual +
 VJP kernel and the FP64 JVP kernel are compared with it entry by entry (1e-12 per field, as
@@ -211,3 +212,30 @@ def test_xdg_sphere_32cube_assembly_and_solves_at_full_size():
         say("  %s: %d iterations, %d outer, reported %.2e, oracle-matrix residual %.2e" % (prec, it, outer, rel, true_rel))
         assert rel <= 1e-10 and true_rel <= 2e-10
     op.close()
+
+
+@pytest.mark.parametrize("form,D,n,p,per", [("sip", 2, 64, 2, False), ("euler_lf", 3, 32, 2, True), ("euler_hllc", 3, 32, 2, True)])
+def test_small_configurations_at_full_size(form, D, n, p, per):
+    """SIP Poisson on 64^2 (p = 2, Dirichlet) and Euler LF / HLLC on 32^3 (p = 2, periodic):
+    the fused residual + VJP kernel against the oracle (1e-12 per field)."""
+    import torch
+    from bosss_amd.grid import CartesianGrid
+    from bosss_amd.operator import DifferentialOperator, default_quad_order
+    from bosss_amd.workloads import euler_state, uniform_coefficients
+    from oracle.bosss_ref import RefModel, params_vector, OP_SIP, OP_EULER_LF, OP_EULER_HLLC
+    say("case fullsize", form, D, n, p, per)
+    kind = {"sip": OP_SIP, "euler_lf": OP_EULER_LF, "euler_hllc": OP_EULER_HLLC}[form]
+    g = CartesianGrid((n,) * D, periodic=[per] * D)
+    op = DifferentialOperator(g, form, p, penalty=1.3, nu=1.0, gamma=1.4)
+    ref = RefModel(D, (n,) * D, p, default_quad_order(form, p), periodic=[int(per)] * D)
+    prm = params_vector(1.3, 1.0, (1.0, 0.5, 0.25)[:D], 1.4)
+    U = euler_state(g, op.N, seed=4) if form.startswith("euler") else uniform_coefficients(op.ndof, 4)
+    W = uniform_coefficients(op.ndof, 6)
+    Ut, Wt = torch.from_numpy(U).cuda(), torch.from_numpy(W).cuda()
+    R = torch.zeros_like(Ut); G = torch.zeros_like(Ut)
+    op.evaluate_vjp_dev(Ut, R, Wt, G, alpha=1.0, beta=0.0)
+    torch.cuda.synchronize()
+    for name, got, want in (("residual", R, ref.evaluate(kind, prm, U)), ("VJP", G, ref.vjp(kind, prm, U, W))):
+        pf, nw = rel_err(got.cpu().numpy(), want, op.nvars, op.N)
+        say("  %s per-field %.2e normwise %.2e" % (name, pf, nw))
+        assert pf < TOL and nw < TOL
