@@ -935,11 +935,10 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
 #pragma unroll
       for (int r = 0; r < 4; r++) slot(2, v, r) = Real(0);
     for (int e = 0; e < D; e++) {
-      {
-        acc_t acc[V];
-        t_eval<TS, Real, V>(tab, qe, e, grp, Bw, acc);
-        park(1, acc);
-      }
+      // the W gradient along e is used once per node: read straight from the MFMA result
+      acc_t gW[V];
+      t_eval<TS, Real, V>(tab, qe, e, grp, Bw, gW);
+#pragma unroll
       for (int r = 0; r < 4; r++) {
         if (t * 16 + MM<Real>::row(0, r) >= TS::KV) continue;   // all 4 nodes of this step are padding
         const int node = t * 16 + MM<Real>::row(grp, r);
@@ -954,7 +953,7 @@ __global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
         for (int v = 0; v < V; v++) {
           Uq[v] = slot(0, v, r);
           ub[v] = slot(2, v, r);
-          gw[v] = slot(1, v, r);
+          gw[v] = gW[v][r];
         }
         EulerAdjoint<D, Real>::volume_vjp_dir(a.fp, Uq, kd, gw, ub);
 #pragma unroll
