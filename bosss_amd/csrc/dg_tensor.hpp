@@ -1152,13 +1152,11 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
     int qe[3];
     const int nodeE = t * 16 + col;
     vol_axes<D, K1>(nodeE < TS::KV ? nodeE : 0, qe);
-    {
-      acc_t acc[V];
-      t_eval<TS, Real, V>(tab, qe, -1, grp, Bu, acc);
-      park(0, acc);
-      t_eval<TS, Real, V>(tab, qe, -1, grp, Bx, acc);
-      park(1, acc);
-    }
+    // state and tangent at the nodes are used once each: read straight from the MFMA results
+    acc_t au[V], adu[V];
+    t_eval<TS, Real, V>(tab, qe, -1, grp, Bu, au);
+    t_eval<TS, Real, V>(tab, qe, -1, grp, Bx, adu);
+#pragma unroll
     for (int r = 0; r < 4; r++) {
       if (t * 16 + MM<Real>::row(0, r) >= TS::KV) continue;   // all 4 nodes of this step are padding
       const int node = t * 16 + MM<Real>::row(grp, r);
@@ -1169,8 +1167,8 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
       T Uq[V], GU[V][D], F[V][D];
 #pragma unroll
       for (int v = 0; v < V; v++) {
-        Uq[v].v = slot(0, v, r);
-        Uq[v].d[0] = slot(1, v, r);
+        Uq[v].v = au[v][r];
+        Uq[v].d[0] = adu[v][r];
 #pragma unroll
         for (int d = 0; d < D; d++) GU[v][d] = Uq[v];
       }
