@@ -232,6 +232,32 @@ def secondary_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def fp32_comparison(grid, form, p, Ut, Wt, R64, G64, steps, warmup):
+    """The same fused residual + VJP step in FP32 (FP32 MFMA and FP32 flux, inputs rounded to
+    FP32 on the device): its time and its error against the FP64 result, per field."""
+    import torch
+    from bosss_amd.operator import DifferentialOperator
+    op32 = DifferentialOperator(grid, form, p, penalty=1.3, nu=-0.01, velocity=(1.0, 0.5, 0.25), gamma=1.4, precision="fp32")
+    U32, W32 = Ut.float(), Wt.float()
+    R32, G32 = torch.zeros_like(U32), torch.zeros_like(U32)
+    ms = _time_ms(lambda: op32.evaluate_vjp_dev(U32, R32, W32, G32), steps, warmup)
+    V, N = op32.nvars, op32.N
+
+    def err(a32, a64):
+        a, b = a32.double().view(-1, V, N), a64.view(-1, V, N)
+        return max(float((a[:, f] - b[:, f]).abs().max() / b[:, f].abs().max()) for f in range(V))
+
+    def nerr(a32, a64):
+        return float((a32.double() - a64).norm() / a64.norm())
+
+    # The HLLC Jacobian jumps where the contact speed changes sign: at the few face nodes within
+    # rounding distance of such a switch the FP32 VJP follows the other branch, which sets its
+    # max-norm error (the FP64 kernel fed the FP32-rounded state differs from FP64 the same way).
+    return {"ms": round(ms, 4), "gdof_s": round(op32.ndof / (ms * 1e-3) / 1e9, 4),
+            "max_rel_err_residual_vs_fp64": err(R32, R64), "normwise_rel_err_residual_vs_fp64": nerr(R32, R64),
+            "max_rel_err_vjp_vs_fp64": err(G32, G64), "normwise_rel_err_vjp_vs_fp64": nerr(G32, G64)}
+
+
 def newton_comparison(grid, op64, Un, form, p):
     """One implicit Euler step solved to ||F|| <= 1e-10: FP64 Newton-GMRES vs FP64 defect
     correction around FP32 JVPs (same driver, bosss_amd.newton)."""
@@ -357,6 +383,10 @@ def main():
     print("bench: %s kernels, %s, %.3f ms/step, %.3f ms kernel" % (
         "generic-table" if os.environ.get("BOSSS_HIP_GENERIC_TABLES") == "1" else "tensor-table",
         "split" if args.split else "fused", ms_per_step, kernel_ms), file=sys.stderr, flush=True)
+    if form.startswith("euler") and not args.split and not args.no_newton:
+        f32 = fp32_comparison(grid, form, p, Ut, Wt, Rt, Gt, args.steps, args.warmup)
+        f32["speedup_vs_fp64"] = round(kernel_ms / f32["ms"], 3)
+        out["fp32_fused"] = f32
     if form.startswith("euler") and not args.no_newton:
         out["newton_implicit_step"] = newton_comparison(grid, op, Ut, form, p)
     if not args.no_cpu_baseline:
