@@ -102,6 +102,25 @@ def copy_calibration(dev, nbytes=1 << 30):
     del a, b
 
 
+def measured_ceilings(op, dev, nbytes=1 << 30):
+    """The device's ceilings measured in this process: dense FP64 / FP32 MFMA and FP64 FMA
+    issue rates (bosss_hip_measure_peaks), and the HBM rate of a device-to-device copy of
+    nbytes (read + write bytes over the best of three copies)."""
+    import ctypes
+    import torch
+    from bosss_amd._lib import lib
+    f64, f32, fma = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    rc = lib().bosss_hip_measure_peaks(op.ctx.h, ctypes.byref(f64), ctypes.byref(f32), ctypes.byref(fma))
+    if rc != 0:
+        raise RuntimeError("bosss_hip_measure_peaks: %s" % lib().bosss_hip_peaks_last_error().decode())
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    ms = min(_time_ms(lambda: b.copy_(a), 1, 1) for _ in range(3))
+    del a, b
+    return {"fp64_mfma_tflops": round(f64.value, 2), "fp32_mfma_tflops": round(f32.value, 2),
+            "fp64_fma_tflops": round(fma.value, 2), "hbm_copy_GBps": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
+
+
 def _time_ms(fn, steps, warmup):
     """Mean duration of fn() over `steps` launches (HIP events on torch's current stream)."""
     import torch
@@ -383,6 +402,8 @@ def main():
     print("bench: %s kernels, %s, %.3f ms/step, %.3f ms kernel" % (
         "generic-table" if os.environ.get("BOSSS_HIP_GENERIC_TABLES") == "1" else "tensor-table",
         "split" if args.split else "fused", ms_per_step, kernel_ms), file=sys.stderr, flush=True)
+    if not args.no_newton:
+        out["roofline"]["measured_ceilings"] = measured_ceilings(op, dev)
     if form.startswith("euler") and not args.split and not args.no_newton:
         f32 = fp32_comparison(grid, form, p, Ut, Wt, Rt, Gt, args.steps, args.warmup)
         f32["speedup_vs_fp64"] = round(kernel_ms / f32["ms"], 3)

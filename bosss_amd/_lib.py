@@ -87,6 +87,9 @@ SIGNATURES = {
                                                ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                                _dp, _vp]),
     "bosss_hip_xdg_last_error": (ctypes.c_char_p, []),
+    "bosss_hip_measure_peaks": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                               ctypes.POINTER(ctypes.c_double)]),
+    "bosss_hip_peaks_last_error": (ctypes.c_char_p, []),
 }
 
 _lib = None

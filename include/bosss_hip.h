@@ -208,6 +208,14 @@ int bosss_hip_xdg_solve_dev(bosss_xdg* xdg, const double* blocks, const double* 
                             double* relres, void* stream);
 const char* bosss_hip_xdg_last_error(void);
 
+/* ---- measured arithmetic ceilings (no counterpart in the reference: the rates the roofline
+ *      fractions are quoted against, measured on the context's device) ----
+ * Dense FP64 MFMA (v_mfma_f64_16x16x4f64), FP32 MFMA (v_mfma_f32_16x16x4f32) and FP64 FMA
+ * issue rates in TFLOP/s, best of three launches of independent accumulation chains over
+ * the whole device (a few milliseconds each). */
+int bosss_hip_measure_peaks(bosss_ctx* ctx, double* tflops_fp64_mfma, double* tflops_fp32_mfma, double* tflops_fp64_fma);
+const char* bosss_hip_peaks_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif
