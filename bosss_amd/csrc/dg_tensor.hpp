@@ -15,10 +15,15 @@
 //    then read as B operands;
 //  * the neighbour's B operands are loaded just in time from global memory;
 //  * the Euler JVP and VJP (kt_jvp_euler, kt_vjp_euler at the end of this file) park
-//    the 4-nodes-per-lane MFMA outputs in a per-lane LDS buffer and run the pointwise
-//    flux node by node; the VJP merges the face cotangent before the pointwise adjoint
-//    (fn = -fo, so only sc_own*W_own.trace - sc_nb*W_nb.trace is needed) and uses the
-//    hand-written reverse sweep of euler_adjoint.hpp;
+//    the 4-nodes-per-lane MFMA outputs that are read more than once, or across register-
+//    heavy flux code, in a per-lane LDS buffer and run the pointwise flux node by node;
+//    values used once (the volume W gradients of the VJP, the volume state and tangent
+//    of the JVP) are read straight from the MFMA results.  They read their own cell's
+//    operands from global memory (L2) rather than LDS; the fused residual + VJP and the
+//    FP32 JVP keep them in registers for the whole launch.  The VJP merges the face
+//    cotangent before the pointwise adjoint (fn = -fo, so only
+//    sc_own*W_own.trace - sc_nb*W_nb.trace is needed) and uses the hand-written reverse
+//    sweeps of euler_adjoint.hpp;
 //  * workgroups are renumbered so that each XCD (own L2) walks one contiguous range
 //    of cells (xcd_block).
 #pragma once
