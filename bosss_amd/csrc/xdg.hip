@@ -972,6 +972,20 @@ int bosss_hip_xdg_create(bosss_ctx* ctx, const bosss_xdg_desc* d, bosss_xdg** ou
         ent = std::max<int64_t>(ent, d->boff[(size_t)j * S + s] + (int64_t)nrows[j] * nrows[k]);
       }
     }
+  // the solver streams each cell row's blocks as one panel: they must follow each other in
+  // slot order (the layout block_layout builds; rows themselves may be anywhere)
+  for (int j = 0; j < J; j++) {
+    int64_t next = d->boff[(size_t)j * S];
+    for (int s = 0; s < S; s++) {
+      const int k = d->cols[j * S + s];
+      if (k < 0) continue;
+      if (d->boff[(size_t)j * S + s] != next) {
+        delete x;
+        return xfail(BOSSS_ERR_ARG, "cell row %d: blocks must follow each other in slot order (slot %d)", j, s);
+      }
+      next += (int64_t)nrows[j] * nrows[k];
+    }
+  }
   x->ndof = dof[J]; x->nentries = ent; x->ndiag = doff[J];
   // bounds checks of the quadrature items against the layout
   auto bad_off = [](const int64_t* o, int n) { if (n && o[0] != 0) return true; for (int i = 0; i < n; i++) if (o[i + 1] < o[i]) return true; return false; };

@@ -175,7 +175,8 @@ typedef struct {
   double penalty_base, mu[2], source[2];
   int32_t S;
   const int32_t* cols;             /* [J][S] */
-  const int64_t* boff;             /* [J][S] */
+  const int64_t* boff;             /* [J][S] offset of the row-major n_j x n_col block; the
+                                      blocks of a cell row follow each other in slot order */
   int32_t nvol;                    /* volume items: (cell, species) */
   const int32_t *vol_cell, *vol_spec;
   const int64_t* vol_off;          /* [nvol+1] node offsets */

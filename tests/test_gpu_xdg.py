@@ -118,3 +118,24 @@ def test_xdg_graph_replay_matches_eager():
     assert sols[0][0] == sols[1][0] == sols[2][0]
     assert np.array_equal(sols[0][1], sols[1][1]) and np.array_equal(sols[1][1], sols[2][1])
     op.close()
+
+
+def test_xdg_create_rejects_scattered_block_offsets(monkeypatch):
+    """The solver streams each cell row's blocks as one panel, so bosss_hip_xdg_create refuses
+    block offsets whose slots do not follow each other (before any device allocation)."""
+    from bosss_amd._lib import BosssError
+    say("case xdg scattered block offsets")
+    R, mu, src, g_D, _ = build("sphere", 6, 2)
+    real = R.block_layout
+
+    def scattered(N):
+        lay = real(N)
+        j = int(np.nonzero((lay["cols"] >= 0).all(1))[0][0])   # a cell row with every slot present
+        b = lay["boff"].copy()
+        b[j, 1], b[j, 2] = b[j, 2], b[j, 1]
+        lay["boff"] = b
+        return lay
+
+    monkeypatch.setattr(R, "block_layout", scattered)
+    with pytest.raises(BosssError, match="slot order"):
+        XdgOperator(R, 2, mu, source=src, penalty=2.0, g_D=g_D)

@@ -139,3 +139,4 @@ def test_oracle_species_sum_equals_single_phase_sip():
         worst = max(worst, np.abs(s - Rs[j * N:(j + 1) * N]).max())
     assert R.cut[interior].sum() > 20
     assert worst < 1e-11 * np.abs(Rs).max()
+
