@@ -129,3 +129,15 @@ def check(rc):
         msg = lib().bosss_hip_last_error().decode()
         raise BosssError(rc, msg)
     return rc
+
+
+def device_array(t, dtype, numel, device, name, at_least=False):
+    """Refuse a tensor a kernel would read or write out of bounds: it must be a contiguous
+    CUDA tensor of `dtype` on device index `device` with exactly (or, with at_least, at
+    least) `numel` entries.  The C entry points receive bare pointers and cannot check."""
+    ok_size = t.numel() >= numel if at_least else t.numel() == numel
+    if not (t.is_cuda and t.dtype == dtype and t.is_contiguous() and ok_size and t.device.index == device):
+        raise ValueError("%s: expected a contiguous %s tensor on cuda:%d with %s%d entries, got %s %s on %s%s"
+                         % (name, dtype, device, "at least " if at_least else "", numel, t.dtype, tuple(t.shape), t.device,
+                            "" if t.is_contiguous() else " (not contiguous)"))
+    return t

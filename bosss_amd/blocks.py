@@ -11,7 +11,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import lib
+from ._lib import device_array, lib
 
 
 class BlockBatch:
@@ -39,24 +39,43 @@ class BlockBatch:
     def _p(t):
         return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
 
+    def _check(self, matrix=None, ipiv=None, info=None, vecs=()):
+        """Sizes the kernels index with: packed blocks (total), pivots (ndof), info (nblk), vectors (ndof)."""
+        import torch
+        d = self.dev.index if self.dev.index is not None else torch.cuda.current_device()
+        if matrix is not None:
+            device_array(matrix, torch.float64, self.total, d, "blocks")
+        if ipiv is not None:
+            device_array(ipiv, torch.int32, self.ndof, d, "ipiv")
+        if info is not None:
+            device_array(info, torch.int32, self.nblk, d, "info")
+        for t in vecs:
+            device_array(t, torch.float64, self.ndof, d, "vector")
+
     def _stream(self):
         import torch
         return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
     def factor(self, A, ipiv, info, inverse=False):
-        """In-place LU (or inverse) of all blocks of A (float64 tensor, packed row-major)."""
+        """In-place LU (or inverse) of all blocks of A (float64 tensor, packed row-major); ipiv and
+        info may be None (no pivots kept, no singularity report)."""
+        self._check(A, ipiv, info)
         rc = lib().bosss_hip_block_lu_dev(None, self.nblk, self.nmax, self._p(self.n_t), self._p(self.off_t), self._p(A),
                                            self._p(ipiv), self._p(self.xoff_t), self._p(info), 1 if inverse else 0, self._stream())
         if rc != 0:
             raise RuntimeError("block LU failed (%d): %s" % (rc, lib().bosss_hip_block_last_error().decode()))
 
     def solve(self, LU, ipiv, x):
+        if ipiv is None:
+            raise ValueError("ipiv is required")
+        self._check(LU, ipiv, None, (x,))
         rc = lib().bosss_hip_block_solve_dev(None, self.nblk, self._p(self.n_t), self._p(self.off_t), self._p(LU), self._p(ipiv),
                                               self._p(self.xoff_t), self._p(self.xoff_t), self._p(x), self._stream())
         if rc != 0:
             raise RuntimeError("block solve failed (%d): %s" % (rc, lib().bosss_hip_block_last_error().decode()))
 
     def gemv(self, Ainv, x, y, alpha=1.0, beta=0.0):
+        self._check(Ainv, None, None, (x, y))
         rc = lib().bosss_hip_block_gemv_dev(None, self.nblk, self._p(self.n_t), self._p(self.off_t), self._p(Ainv), self._p(self.xoff_t),
                                              alpha, self._p(x), beta, self._p(y), self._stream())
         if rc != 0:

@@ -172,6 +172,7 @@ int bosss_hip_block_lu_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t
                            int32_t* ipiv, const int64_t* pivoff, int32_t* info, int32_t want_inverse, void* stream) {
   (void)ctx;
   if (nblk <= 0) return BOSSS_OK;
+  if (!n || !off || !A || (ipiv && !pivoff)) { g_lu_err = "NULL argument"; return BOSSS_ERR_ARG; }
   if (nmax <= 0 || nmax > LU_MAXN) { g_lu_err = "block size out of range (1..128)"; return BOSSS_ERR_ARG; }
   size_t lds = lu_lds_bytes(nmax, want_inverse);
   if (lds > 160 * 1024) { g_lu_err = "block too large for an in-LDS inverse (n <= 100)"; return BOSSS_ERR_UNSUPPORTED; }
@@ -189,6 +190,7 @@ int bosss_hip_block_solve_dev(bosss_ctx* ctx, int nblk, const int32_t* n, const 
                               const int64_t* pivoff, const int64_t* xoff, double* x, void* stream) {
   (void)ctx;
   if (nblk <= 0) return BOSSS_OK;
+  if (!n || !off || !LU || !ipiv || !pivoff || !xoff || !x) { g_lu_err = "NULL argument"; return BOSSS_ERR_ARG; }
   hipLaunchKernelGGL(k_block_solve<double>, dim3((nblk + 63) / 64), dim3(64), 0, (hipStream_t)stream, nblk, n, off, LU, ipiv,
                      pivoff, xoff, x);
   hipError_t e = hipGetLastError();
@@ -200,6 +202,7 @@ int bosss_hip_block_gemv_dev(bosss_ctx* ctx, int nblk, const int32_t* n, const i
                              double alpha, const double* x, double beta, double* y, void* stream) {
   (void)ctx;
   if (nblk <= 0) return BOSSS_OK;
+  if (!n || !off || !Ainv || !xoff || !x || !y) { g_lu_err = "NULL argument"; return BOSSS_ERR_ARG; }
   hipLaunchKernelGGL(k_block_gemv<double>, dim3(nblk), dim3(64), 0, (hipStream_t)stream, nblk, n, off, Ainv, xoff, alpha, x, beta, y);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }

@@ -17,7 +17,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import check, lib
+from ._lib import check, device_array, lib
 from .grid import CartesianGrid
 from .refelement import Tables
 
@@ -144,11 +144,25 @@ class DifferentialOperator:
         return ctypes.c_void_p(t.data_ptr())
 
     def _check_dev(self, *ts):
+        """Vectors of the device variants: ndof entries of the operator's precision."""
         import torch
         want = torch.float64 if self.precision == "fp64" else torch.float32
         for t in ts:
-            if not (t.is_cuda and t.dtype == want and t.is_contiguous() and t.numel() == self.ndof):
-                raise ValueError("expected a contiguous %s CUDA tensor with %d entries" % (want, self.ndof))
+            device_array(t, want, self.ndof, self.ctx.device, "vector")
+
+    def _check_matrix_args(self, blocks, *vecs):
+        """The operator matrix and the vectors it is applied to are FP64 whatever the precision."""
+        import torch
+        device_array(blocks, torch.float64, self._entries(), self.ctx.device, "blocks", at_least=True)
+        for t in vecs:
+            device_array(t, torch.float64, self.ndof, self.ctx.device, "vector")
+
+    def _entries(self):
+        if getattr(self, "_nentries", None) is None:
+            n = ctypes.c_int64()
+            check(lib().bosss_hip_op_matrix_layout(self.h, None, ctypes.byref(n), None, None))
+            self._nentries = n.value
+        return self._nentries
 
     def _stream(self, stream):
         """HIP stream handle: torch's current stream by default (0 is HIP's null stream)."""
@@ -186,11 +200,11 @@ class DifferentialOperator:
 
     def assemble_dev(self, U, blocks, alpha=1.0, beta=0.0, stream=None):
         """blocks <- beta*blocks + alpha*dR/dU(U) (the operator matrix for linear forms)."""
-        self._check_dev(U)
+        self._check_matrix_args(blocks, U)
         check(lib().bosss_hip_op_assemble_dev(self.h, self._ptr(U), alpha, beta, self._ptr(blocks), self._stream(stream)))
 
     def spmv_dev(self, blocks, x, y, alpha=1.0, beta=0.0, stream=None):
-        self._check_dev(x, y)
+        self._check_matrix_args(blocks, x, y)
         check(lib().bosss_hip_bell_spmv_dev(self.h, self._ptr(blocks), alpha, self._ptr(x), beta, self._ptr(y), self._stream(stream)))
 
     def GetMatrixBuilder(self):

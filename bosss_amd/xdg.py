@@ -547,15 +547,23 @@ class XdgOperator:
         import torch
         return torch.zeros(self.nentries, dtype=torch.float64, device="cuda:%d" % self.ctx.device)
 
+    def _check_args(self, blocks, *vecs):
+        """Matrix (nentries or more FP64 entries) and vectors (ndof FP64 entries) on this device."""
+        import torch
+        from ._lib import device_array
+        device_array(blocks, torch.float64, self.nentries, self.ctx.device, "blocks", at_least=True)
+        for t in vecs:
+            device_array(t, torch.float64, self.ndof, self.ctx.device, "vector")
+
     def assemble_dev(self, blocks, affine, stream=None):
         """blocks <- M, affine <- b with R(u) = M u + b."""
         from ._lib import lib
-        assert blocks.numel() >= self.nentries and affine.numel() == self.ndof
+        self._check_args(blocks, affine)
         self._check(lib().bosss_hip_xdg_assemble_dev(self.h, blocks.data_ptr(), affine.data_ptr(), self._stream(stream)))
 
     def spmv_dev(self, blocks, x, y, alpha=1.0, beta=0.0, stream=None):
         from ._lib import lib
-        assert x.numel() == self.ndof and y.numel() == self.ndof
+        self._check_args(blocks, x, y)
         self._check(lib().bosss_hip_xdg_spmv_dev(self.h, blocks.data_ptr(), alpha, x.data_ptr(), beta, y.data_ptr(),
                                                  self._stream(stream)))
 
@@ -565,6 +573,7 @@ class XdgOperator:
         from ._lib import lib, BOSSS_FP64, BOSSS_FP32
         it, outer, rel = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
         prec = BOSSS_FP64 if precision == "fp64" else BOSSS_FP32
+        self._check_args(blocks, b, x)
         self._check(lib().bosss_hip_xdg_solve_dev(self.h, blocks.data_ptr(), b.data_ptr(), x.data_ptr(), tol, maxit, prec,
                                                   int(bool(use_graph)), check_every, ctypes.byref(it), ctypes.byref(outer),
                                                   ctypes.byref(rel), self._stream(stream)))

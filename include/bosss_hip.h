@@ -110,7 +110,11 @@ int bosss_hip_vjp(bosss_op* op, const double* U, const double* W, double* JtW);
 
 /* ---- device-resident variants: pointers are device memory, work is queued on
  *      `stream` (a hipStream_t; NULL is HIP's null stream, bosss_hip_ctx_stream()
- *      gives the context's own stream) and returns without synchronising. ---- */
+ *      gives the context's own stream) and returns without synchronising.
+ *      Each vector holds ndof entries of the operator's precision: double for
+ *      BOSSS_FP64, float for BOSSS_FP32 (passed through the same double* parameter,
+ *      which is then read as float*).  The host-pointer entry points above always
+ *      take double and round to the operator's precision on the device. ---- */
 int bosss_hip_evaluate_dev(bosss_op* op, double alpha, double beta, const double* U, double* R, void* stream);
 int bosss_hip_jvp_dev(bosss_op* op, const double* U, const double* dU, double* JdU, void* stream);
 int bosss_hip_vjp_dev(bosss_op* op, const double* U, const double* W, double* JtW, void* stream);
@@ -135,7 +139,8 @@ int bosss_hip_bell_spmv_dev(bosss_op* op, const double* blocks, double alpha, co
  *      IMatrixExtensions.cs:2128-2230, and Schwarz's LAPACK block solver, Schwarz.cs:827-915).
  *      Blocks are dense row-major at element offsets off[b], sizes n[b] <= nmax <= 128.
  *      ipiv (1-based, LAPACK convention) at pivoff[b]; info[b] as DGETRF's INFO
- *      (0, or j+1 for the first zero pivot; -1 if n[b] exceeds nmax).
+ *      (0, or j+1 for the first zero pivot; -1 if n[b] exceeds nmax).  In
+ *      bosss_hip_block_lu_dev, ipiv (with pivoff) and info may be NULL.
  *      want_inverse != 0 overwrites each block with its inverse (DGETRI), n <= 100.
  *      All arrays are device memory; one launch covers the whole batch. ---- */
 int bosss_hip_block_lu_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t* n, const int64_t* off, double* A,

@@ -38,3 +38,47 @@ def test_measured_peaks_are_near_the_published_ceilings():
     for got, pub in zip((x.value for x in v), (78.6, 157.3, 78.6)):
         print("  measured %.1f TFLOP/s of published %.1f" % (got, pub), flush=True)
         assert 0.5 * pub < got < 1.1 * pub
+
+
+def test_device_wrappers_refuse_mismatched_tensors():
+    """The device wrappers check dtype, size, contiguity and device of every tensor before any
+    launch (the C entry points take bare pointers): short, FP32-for-FP64, strided and host
+    tensors raise ValueError; nothing is launched with them."""
+    import torch
+    from bosss_amd.blocks import BlockBatch
+    from bosss_amd.grid import CartesianGrid
+    from bosss_amd.operator import DifferentialOperator
+    print("case wrapper argument checks", flush=True)
+    g = CartesianGrid((2, 2, 2), periodic=[True] * 3)
+    vel = (1.0, 0.5, 0.25)
+    op = DifferentialOperator(g, "advdiff", 3, velocity=vel)
+    op32 = DifferentialOperator(g, "advdiff", 3, velocity=vel, precision="fp32")   # FP32 kernels: p = 3
+    mb = op.GetMatrixBuilder()
+    A = mb.new_matrix()
+    x = torch.zeros(op.ndof, dtype=torch.float64, device="cuda:0")
+    y = torch.zeros_like(x)
+    bad = [x[:-1], x.float(), torch.zeros(2 * op.ndof, dtype=torch.float64, device="cuda:0")[::2], x.cpu()]
+    for b in bad:
+        with pytest.raises(ValueError):
+            op.spmv_dev(A, b, y)
+        with pytest.raises(ValueError):
+            op.evaluate_dev(b, y)
+    with pytest.raises(ValueError):
+        op.spmv_dev(A[:-1], x, y)
+    with pytest.raises(ValueError):
+        op.assemble_dev(x, A.float())
+    # an FP32 operator's vectors are float32, but its matrix products stay FP64
+    with pytest.raises(ValueError):
+        op32.evaluate_dev(x, y)
+    op32.evaluate_dev(x.float(), y.float())
+    op32.spmv_dev(A, x, y)
+    bb = BlockBatch([3, 4])
+    M = torch.zeros(bb.total, dtype=torch.float64, device="cuda:0")
+    v = torch.zeros(bb.ndof, dtype=torch.float64, device="cuda:0")
+    with pytest.raises(ValueError):
+        bb.gemv(M[:-1], v, v.clone())
+    with pytest.raises(ValueError):
+        bb.factor(M, torch.zeros(bb.ndof, dtype=torch.int64, device="cuda:0"), None)
+    with pytest.raises(ValueError):
+        bb.solve(M, None, v)
+    torch.cuda.synchronize()
