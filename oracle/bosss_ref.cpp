@@ -221,6 +221,8 @@ struct Model {
   Tables T;
   int D = 0, p = 0, N = 0, order = 0;
   int n[3] = {1, 1, 1}; double lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1}; bool per[3] = {false, false, false};
+  // explicit node coordinates per direction (n[d] + 1 each) when given; else uniform on [lo, hi]
+  std::vector<double> xn[3];
   int J = 0, E = 0;
   // cell metrics (GridData.CellData)
   std::vector<double> detJ, invJ, scal, lscale, center;
@@ -268,28 +270,43 @@ bool build_model(Model& m) {
         for (int d = 0; d < D; d++) m.dtr[(((size_t)f * Ke + k) * N + nn) * D + d] = poly_deriv(P, xi, D, d);
       }
     }
-  // grid
+  // grid: Cartesian2DGrid / Cartesian3DGrid over per-direction node arrays (Grid3D.cs:85-235);
+  // every cell is an axis-aligned box, so each cell's metrics are diagonal and constant
   m.J = 1; for (int d = 0; d < D; d++) m.J *= m.n[d];
-  double h[3], vol = 1;
-  for (int d = 0; d < D; d++) { h[d] = (m.hi[d] - m.lo[d]) / m.n[d]; vol *= h[d]; }
-  double dj = 1; for (int d = 0; d < D; d++) dj *= h[d] / 2;
-  m.detJ.assign(m.J, dj);
-  m.scal.assign(m.J, 1.0 / std::sqrt(dj));
+  auto width = [&](int d, int i) { return m.xn[d].empty() ? (m.hi[d] - m.lo[d]) / m.n[d] : m.xn[d][i + 1] - m.xn[d][i]; };
+  auto left = [&](int d, int i) { return m.xn[d].empty() ? m.lo[d] + i * width(d, i) : m.xn[d][i]; };
+  std::vector<double> hc((size_t)m.J * D), vol(m.J), surf(m.J, 0.0);
+  m.detJ.assign(m.J, 0.0);
+  m.scal.assign(m.J, 0.0);
   m.invJ.assign((size_t)m.J * D * D, 0.0);
   m.center.assign((size_t)m.J * D, 0.0);
   std::vector<double> bndArea(m.J, 0.0);
-  double surf = 0, farea[3];
-  for (int d = 0; d < D; d++) { farea[d] = 1; for (int e = 0; e < D; e++) if (e != d) farea[d] *= h[e]; surf += 2 * farea[d]; }
   int ijk[3] = {0, 0, 0};
+  // area of the faces of cell j normal to direction d
+  auto farea = [&](int j, int d) { double a = 1; for (int e = 0; e < D; e++) if (e != d) a *= hc[(size_t)j * D + e]; return a; };
   for (int j = 0; j < m.J; j++) {
     int rem = j;
     for (int d = D - 1; d >= 0; d--) { ijk[d] = rem % m.n[d]; rem /= m.n[d]; }
-    for (int d = 0; d < D; d++) { m.invJ[(size_t)j * D * D + d * D + d] = 2.0 / h[d]; m.center[j * D + d] = m.lo[d] + (ijk[d] + 0.5) * h[d]; }
+    double dj = 1, v = 1;
+    for (int d = 0; d < D; d++) {
+      const double h = width(d, ijk[d]);
+      hc[(size_t)j * D + d] = h;
+      dj *= h / 2;
+      v *= h;
+      m.invJ[(size_t)j * D * D + d * D + d] = 2.0 / h;
+      m.center[j * D + d] = left(d, ijk[d]) + 0.5 * h;
+    }
+    m.detJ[j] = dj;
+    m.scal[j] = 1.0 / std::sqrt(dj);
+    vol[j] = v;
   }
+  for (int j = 0; j < m.J; j++)
+    for (int d = 0; d < D; d++) surf[j] += 2 * farea(j, d);
   // edges: per direction, interior (and periodic) faces first in cell order, then boundary faces
   m.cin.clear(); m.cout_.clear(); m.fin.clear(); m.fout.clear(); m.normal.clear(); m.sqrtG.clear();
+  // sqrt(Gram) of a face = its area / 2^(D-1) (both neighbours share the tangential extents)
+  const double gscale = std::pow(2.0, D - 1);
   for (int d = 0; d < D; d++) {
-    double sg = farea[d] / std::pow(2.0, D - 1);
     for (int j = 0; j < m.J; j++) {
       int rem = j;
       for (int dd = D - 1; dd >= 0; dd--) { ijk[dd] = rem % m.n[dd]; rem /= m.n[dd]; }
@@ -299,12 +316,11 @@ bool build_model(Model& m) {
       int jo = cell_index(m, nb);
       m.cin.push_back(j); m.cout_.push_back(jo); m.fin.push_back(face_of(d, 1)); m.fout.push_back(face_of(d, 0));
       for (int e = 0; e < D; e++) m.normal.push_back(e == d ? 1.0 : 0.0);
-      m.sqrtG.push_back(sg);
+      m.sqrtG.push_back(farea(j, d) / gscale);
     }
   }
   for (int d = 0; d < D; d++) {
     if (m.per[d]) continue;
-    double sg = farea[d] / std::pow(2.0, D - 1);
     for (int side = 0; side < 2; side++)
       for (int j = 0; j < m.J; j++) {
         int rem = j;
@@ -312,14 +328,14 @@ bool build_model(Model& m) {
         if (ijk[d] != (side ? m.n[d] - 1 : 0)) continue;
         m.cin.push_back(j); m.cout_.push_back(-1); m.fin.push_back(face_of(d, side)); m.fout.push_back(-1);
         for (int e = 0; e < D; e++) m.normal.push_back(e == d ? (side ? 1.0 : -1.0) : 0.0);
-        m.sqrtG.push_back(sg);
-        bndArea[j] += farea[d];
+        m.sqrtG.push_back(farea(j, d) / gscale);
+        bndArea[j] += farea(j, d);
       }
   }
   m.E = (int)m.cin.size();
   // CellLengthScale = 1/cj, cj = (SurfaceArea + bndArea) * 0.5 / Volume  (GridData_CellData.cs:681-684)
   m.lscale.resize(m.J);
-  for (int j = 0; j < m.J; j++) m.lscale[j] = 1.0 / ((surf + bndArea[j]) * 0.5 / vol);
+  for (int j = 0; j < m.J; j++) m.lscale[j] = 1.0 / ((surf[j] + bndArea[j]) * 0.5 / vol[j]);
   return true;
 }
 
@@ -990,6 +1006,28 @@ void* bref_create(const char* tables_dir, int D, const int* ncells, const double
   if (D < 2 || D > 3) { g_err = "D must be 2 or 3"; delete m; return nullptr; }
   m->D = D; m->p = p; m->order = quad_order;
   for (int d = 0; d < D; d++) { m->n[d] = ncells[d]; m->lo[d] = lo[d]; m->hi[d] = hi[d]; m->per[d] = periodic[d] != 0; }
+  if (!build_model(*m)) { g_err = "unsupported degree/order"; delete m; return nullptr; }
+  return m;
+}
+// As bref_create over explicit node coordinates: nodes holds ncells[d] + 1 increasing
+// coordinates per direction, direction after direction (Cartesian3DGrid(xNodes, yNodes, zNodes)).
+void* bref_create_nodes(const char* tables_dir, int D, const int* ncells, const double* nodes, const int* periodic, int p,
+                        int quad_order) {
+  if (D < 2 || D > 3) { g_err = "D must be 2 or 3"; return nullptr; }
+  Model* m = new Model();
+  if (!load_tables(tables_dir, m->T)) { g_err = "cannot read reference tables in " + std::string(tables_dir); delete m; return nullptr; }
+  m->D = D; m->p = p; m->order = quad_order;
+  const double* x = nodes;
+  for (int d = 0; d < D; d++) {
+    m->n[d] = ncells[d];
+    m->per[d] = periodic[d] != 0;
+    m->xn[d].assign(x, x + ncells[d] + 1);
+    for (int i = 0; i < ncells[d]; i++)
+      if (!(x[i + 1] > x[i])) { g_err = "node coordinates must increase"; delete m; return nullptr; }
+    m->lo[d] = x[0];
+    m->hi[d] = x[ncells[d]];
+    x += ncells[d] + 1;
+  }
   if (!build_model(*m)) { g_err = "unsupported degree/order"; delete m; return nullptr; }
   return m;
 }

@@ -34,6 +34,8 @@ def _load():
         lib = ctypes.CDLL(LIB)
         lib.bref_create.restype = ctypes.c_void_p
         lib.bref_create.argtypes = [ctypes.c_char_p, ctypes.c_int, _ip, _dp, _dp, _ip, ctypes.c_int, ctypes.c_int]
+        lib.bref_create_nodes.restype = ctypes.c_void_p
+        lib.bref_create_nodes.argtypes = [ctypes.c_char_p, ctypes.c_int, _ip, _dp, _ip, ctypes.c_int, ctypes.c_int]
         lib.bref_destroy.argtypes = [ctypes.c_void_p]
         lib.bref_info.argtypes = [ctypes.c_void_p, _ip]
         lib.bref_last_error.restype = ctypes.c_char_p
@@ -66,14 +68,21 @@ def params_vector(penalty=1.3, nu=1.0, vel=(0.0, 0.0, 0.0), gamma=1.4, mach=None
 class RefModel:
     """Cartesian DG discretisation evaluated by the CPU restatement."""
 
-    def __init__(self, D, ncells, p, quad_order, lo=None, hi=None, periodic=None):
+    def __init__(self, D, ncells, p, quad_order, lo=None, hi=None, periodic=None, nodes=None):
+        """Uniform grid of ncells on the box [lo, hi], or, with `nodes` (D increasing coordinate
+        arrays, as Cartesian3DGrid takes them), the tensor grid over those nodes."""
         lib = _load()
         self.D, self.p, self.order = D, p, quad_order
-        nc = np.array(ncells, dtype=np.int32)
-        lo = np.zeros(D) if lo is None else np.asarray(lo, dtype=np.float64)
-        hi = np.ones(D) if hi is None else np.asarray(hi, dtype=np.float64)
         per = np.zeros(D, np.int32) if periodic is None else np.asarray(periodic, dtype=np.int32)
-        self.h = lib.bref_create(TABLES.encode(), D, _p(nc, _ip), _p(lo), _p(hi), _p(per, _ip), p, quad_order)
+        if nodes is not None:
+            nc = np.array([len(x) - 1 for x in nodes], dtype=np.int32)
+            flat = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64) for x in nodes]))
+            self.h = lib.bref_create_nodes(TABLES.encode(), D, _p(nc, _ip), _p(flat), _p(per, _ip), p, quad_order)
+        else:
+            nc = np.array(ncells, dtype=np.int32)
+            lo = np.zeros(D) if lo is None else np.asarray(lo, dtype=np.float64)
+            hi = np.ones(D) if hi is None else np.asarray(hi, dtype=np.float64)
+            self.h = lib.bref_create(TABLES.encode(), D, _p(nc, _ip), _p(lo), _p(hi), _p(per, _ip), p, quad_order)
         if not self.h:
             raise RuntimeError(lib.bref_last_error().decode())
         info = np.zeros(5, np.int32)

@@ -53,3 +53,22 @@ def test_host_setup_matches_oracle(D, n, p, order, per):
     for a, b in ((t.phi, rt["phi"]), (t.dphi, rt["dphi"]), (t.trace, rt["trace"]), (t.dtrace, rt["dtrace"]),
                  (t.vw, rt["vw"]), (t.fw, rt["fw"]), (t.vnodes, rt["vnodes"])):
         assert np.abs(a - b).max() < 5e-14 * max(1.0, np.abs(b).max())
+
+
+@pytest.mark.parametrize("D,per", [(2, False), (3, False), (3, True)])
+def test_host_grid_on_graded_nodes_matches_oracle(D, per):
+    """Per-cell metrics of a graded tensor grid (node arrays per direction, a different cell
+    count in each): the product's host setup equals the oracle's."""
+    rng = np.random.default_rng(4)
+    nodes = []
+    for d in range(D):
+        w = rng.uniform(1.0, 3.0, 3 + d)
+        nodes.append(np.concatenate([[0.0], np.cumsum(w) / w.sum()]))
+    g = CartesianGrid(nodes=nodes, periodic=[per] * D)
+    m = RefModel(D, None, 2, 4, periodic=[int(per)] * D, nodes=nodes)
+    rg = m.grid()
+    assert (g.cells == rg["cells"]).all() and (g.trafo == rg["faces"]).all()
+    for a, b in ((g.lenscale, rg["lenscale"]), (g.normals, rg["normals"]), (g.sqrtgram, rg["sqrtgram"]), (g.detJ, rg["detJ"]),
+                 (g.invJ, rg["invJ"]), (g.scaling, rg["scaling"])):
+        assert np.abs(a - b).max() <= 1e-15 * max(1.0, np.abs(b).max())
+    assert len(set(np.round(g.detJ, 14))) > g.J // 2          # the cells really differ
