@@ -480,8 +480,9 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
 // ----------------------------------------------------------------------------
 // Adjoint: out <- J(U)^T W  (FUSED: also R <- beta R + alpha Op(U)).
 // ----------------------------------------------------------------------------
+// FP32 instantiations are compiled for two waves per SIMD (they fit 256 registers unspilled).
 template <class Form, int D, int P, int K1, typename Real, bool FUSED>
-__global__ void __launch_bounds__(64 * TWAVES) kt_adjoint(TArgs<Real> a) {
+__global__ void __launch_bounds__(64 * TWAVES, sizeof(Real) == 4 ? 2 : 1) kt_adjoint(TArgs<Real> a) {
   typedef TShape<D, P, K1> TS;
   typedef typename MM<Real>::acc acc_t;
   constexpr int V = Form::V, N = TS::N, NT = V * N, NF = TS::NF, MT = TS::MT;
@@ -826,8 +827,16 @@ namespace bosss {
 // global memory (L1/L2) instead of being staged.  This keeps the FP64 Euler VJP
 // inside the register file.
 // ----------------------------------------------------------------------------
+// Waves per SIMD the FP32 Euler VJP kernels are compiled for.  In FP32 the per-lane state fits
+// 256 registers and the LDS staging is 61 KB per workgroup, so two workgroups share a CU and
+// one wave's memory waits overlap the other's arithmetic (the fused p = 3 residual + VJP runs
+// 1.4x faster than at one wave).  The non-fused p = 3 variants reach two waves without the
+// hint and spill with it, so they are left to the compiler.
+template <typename Real, int P, bool FUSED>
+constexpr int vjp_euler_min_waves() { return (sizeof(Real) == 4 && (FUSED || P < 3)) ? 2 : 1; }
+
 template <class Form, int D, int P, int K1, typename Real, bool FUSED>
-__global__ void __launch_bounds__(64 * TWAVES) kt_vjp_euler(TArgs<Real> a) {
+__global__ void __launch_bounds__(64 * TWAVES, (vjp_euler_min_waves<Real, P, FUSED>())) kt_vjp_euler(TArgs<Real> a) {
   typedef TShape<D, P, K1> TS;
   typedef typename MM<Real>::acc acc_t;
   constexpr int V = Form::V, N = TS::N, NT = V * N, NF = TS::NF, MT = TS::MT;
