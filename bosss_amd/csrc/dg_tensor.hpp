@@ -244,11 +244,21 @@ template <int D> struct IsHLLC<FormEuler<D, true>> : std::true_type {};
 
 constexpr int TWAVES = 4;
 
+// Waves per SIMD the residual/JVP (kt_forward) and linear-form adjoint (kt_adjoint) kernels
+// are compiled for.  Their LDS use is small, so registers alone set the occupancy: FP32, and
+// FP64 up to p = 2, fit two waves with little or no spilling and gain 13-28 % from the
+// overlap.  The FP64 p = 3 kernels, and the FP64 fused residual + adjoint at p = 2, would
+// spill 260-300 B per lane and run slower (measured), so they stay at one wave.
+template <typename Real, int P>
+constexpr int tensor_min_waves() { return (sizeof(Real) == 4 || P < 3) ? 2 : 1; }
+template <typename Real, int P, bool FUSED>
+constexpr int adjoint_min_waves() { return (sizeof(Real) == 4 || (P < 3 && !FUSED)) ? 2 : 1; }
+
 // ----------------------------------------------------------------------------
 // Forward: residual (MODE_RES) or JVP (MODE_JVP).
 // ----------------------------------------------------------------------------
 template <class Form, int D, int P, int K1, typename Real, int MODE>
-__global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 2 : 1))) kt_forward(TArgs<Real> a) {
+__global__ void __launch_bounds__(64 * TWAVES, (tensor_min_waves<Real, P>())) kt_forward(TArgs<Real> a) {
   typedef TShape<D, P, K1> TS;
   typedef typename MM<Real>::acc acc_t;
   constexpr int V = Form::V, N = TS::N, NT = V * N, NF = TS::NF, MT = TS::MT;
@@ -480,9 +490,8 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
 // ----------------------------------------------------------------------------
 // Adjoint: out <- J(U)^T W  (FUSED: also R <- beta R + alpha Op(U)).
 // ----------------------------------------------------------------------------
-// FP32 instantiations are compiled for two waves per SIMD (they fit 256 registers unspilled).
 template <class Form, int D, int P, int K1, typename Real, bool FUSED>
-__global__ void __launch_bounds__(64 * TWAVES, sizeof(Real) == 4 ? 2 : 1) kt_adjoint(TArgs<Real> a) {
+__global__ void __launch_bounds__(64 * TWAVES, (adjoint_min_waves<Real, P, FUSED>())) kt_adjoint(TArgs<Real> a) {
   typedef TShape<D, P, K1> TS;
   typedef typename MM<Real>::acc acc_t;
   constexpr int V = Form::V, N = TS::N, NT = V * N, NF = TS::NF, MT = TS::MT;
