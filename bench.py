@@ -122,7 +122,11 @@ def measured_ceilings(op, dev, nbytes=1 << 30):
 
 
 def _time_ms(fn, steps, warmup):
-    """Mean duration of fn() over `steps` launches (HIP events on torch's current stream)."""
+    """Median duration of fn() over `steps` launches (a HIP event pair around each launch on
+    torch's current stream).  A host-side stall between an event and its launch leaves the
+    GPU idle inside that pair; the median keeps such a stall (one 38 ms pair has been seen
+    among 0.2 ms launches) out of the figure."""
+    import numpy as np
     import torch
     stream = torch.cuda.current_stream()
     for _ in range(warmup):
@@ -134,7 +138,7 @@ def _time_ms(fn, steps, warmup):
         fn()
         b.record(stream)
     torch.cuda.synchronize()
-    return sum(a.elapsed_time(b) for a, b in ev) / steps
+    return float(np.median([a.elapsed_time(b) for a, b in ev]))
 
 
 def secondary_workload(args):
