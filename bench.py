@@ -62,7 +62,26 @@ def cpu_baseline(form, p, budget_s=15.0, n_max=64):
         "value": dof / (tr + tv) / 1e9, "unit": "GDOF/s", "cores": int(threads), "kind": "port",
         "sample": "%s p=%d, %d^3 periodic cells (%d DOF): residual %.3f s + VJP %.3f s (residual alone %.4f GDOF/s)"
                   % (form, p, n, dof, tr, tv, dof / tr / 1e9),
+        "host": host_cpu_layout(),
     }
+
+
+def host_cpu_layout():
+    """The host's CPU layout as lscpu reports it (model, sockets, cores per socket, threads per
+    core) and the CPUs this process may run on."""
+    import subprocess
+    info = {"cpus_usable": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return info
+    keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+            "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+    for line in out.splitlines():
+        k, _, v = line.partition(":")
+        if k.strip() in keys and keys[k.strip()] not in info:
+            info[keys[k.strip()]] = v.strip()
+    return info
 
 
 def kernel_label(form, p, split):

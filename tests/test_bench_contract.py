@@ -35,3 +35,11 @@ def test_algorithmic_flops_per_apply(form, D, n, p, per, expected):
     from bosss_amd.workloads import contraction_flops
     g = CartesianGrid((n,) * D, periodic=[per] * D)
     assert contraction_flops(form, D, p, g.J, len(g.cells)) == expected
+
+
+def test_cpu_baseline_records_the_host_layout():
+    sys.path.insert(0, ROOT)
+    import bench
+    info = bench.host_cpu_layout()
+    assert info["cpus_usable"] >= 1
+    assert int(info.get("cpus", info["cpus_usable"])) >= info["cpus_usable"]
