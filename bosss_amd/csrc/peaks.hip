@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(PK_THREADS) k_peak_fma64(double* out, double s
   out[(size_t)blockIdx.x * PK_THREADS + threadIdx.x] = s;
 }
 
-std::string g_peak_err;
+thread_local std::string g_peak_err;   // last error of this host thread
 
 int pfail(int code, const char* what, hipError_t e) {
   g_peak_err = std::string(what) + ": " + hipGetErrorString(e);

@@ -40,7 +40,7 @@ constexpr int XD = 3;
 constexpr int XCHUNK = 32;      // nodes staged per pass
 constexpr int XPAIRS = (XN * XN + 63) / 64;
 
-std::string g_xdg_err;
+thread_local std::string g_xdg_err;   // last error of this host thread
 int xfail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;

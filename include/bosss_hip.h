@@ -5,8 +5,9 @@
  * the same mechanism as BLAS.cs:1065-1090 / LAPACK.cs:28-66).  Conventions follow
  * that precedent:
  *   - every function returns int: 0 = ok, < 0 = error (message via
- *     bosss_hip_last_error()), > 0 = LAPACK-style INFO (1-based pivot index of
- *     the first singular block);
+ *     bosss_hip_last_error(), or the block / XDG / peaks variant of the family
+ *     that failed; each holds the last error of the calling host thread),
+ *     > 0 = LAPACK-style INFO (1-based pivot index of the first singular block);
  *   - host pointers are only read/written during the call (managed arrays are
  *     pinned with `fixed` for one call only, DGField_Evaluation.cs:78-110);
  *   - a context is bound to one host thread and one GPU; calls on one context
