@@ -945,6 +945,14 @@ int bosss_hip_xdg_create(bosss_ctx* ctx, const bosss_xdg_desc* d, bosss_xdg** ou
   if (d->D < 2 || d->D > 3) return xfail(BOSSS_ERR_ARG, "D must be 2 or 3");
   if (d->N < 1 || d->N > XN || d->degree < 0 || d->degree > 3) return xfail(BOSSS_ERR_UNSUPPORTED, "XDG kernels cover p <= 3 (N <= %d)", XN);
   if (d->S != 1 + 2 * d->D) return xfail(BOSSS_ERR_ARG, "S must be 1 + 2D");
+  if (d->J <= 0 || d->nvol < 0 || d->nface < 0 || d->nifc < 0) return xfail(BOSSS_ERR_ARG, "bad sizes J=%d nvol=%d nface=%d nifc=%d", d->J, d->nvol, d->nface, d->nifc);
+  if (!d->mode_exponents || !d->nspec || !d->spec0 || !d->invJ || !d->scaling || !d->lscale || !d->cols || !d->boff)
+    return xfail(BOSSS_ERR_ARG, "XDG descriptor has NULL arrays");
+  if ((d->nvol && (!d->vol_cell || !d->vol_spec || !d->vol_off || !d->vol_nodes || !d->vol_w)) ||
+      (d->nface && (!d->face_cells || !d->face_slot || !d->face_spec || !d->face_off || !d->face_nodes_in || !d->face_nodes_out ||
+                    !d->face_w || !d->face_normal)) ||   // face_gD may be NULL: homogeneous Dirichlet data
+      (d->nifc && (!d->ifc_cell || !d->ifc_off || !d->ifc_nodes || !d->ifc_w || !d->ifc_normal)))
+    return xfail(BOSSS_ERR_ARG, "XDG descriptor has NULL quadrature-item arrays");
   for (int n = 0; n < d->N * d->D; n++)
     if (d->mode_exponents[n] < 0 || d->mode_exponents[n] > d->degree) return xfail(BOSSS_ERR_ARG, "mode exponent out of range");
   for (int j = 0; j < d->J; j++)
