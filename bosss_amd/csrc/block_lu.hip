@@ -159,8 +159,23 @@ __global__ void __launch_bounds__(64) k_block_gemv(int nblk, const int32_t* __re
 
 using namespace bosss;
 
+namespace bosss {
+int ctx_device_stream(bosss_ctx* ctx, int* device, hipStream_t* stream);
+}
+
 namespace {
 thread_local std::string g_lu_err;
+
+// With a context, launch on its device (the stream argument must belong to that device);
+// without one, on the calling thread's current device.
+int select_device(bosss_ctx* ctx) {
+  if (!ctx) return BOSSS_OK;
+  int dev = 0;
+  hipStream_t st = nullptr;
+  if (bosss::ctx_device_stream(ctx, &dev, &st) != BOSSS_OK) { g_lu_err = "bad context"; return BOSSS_ERR_ARG; }
+  if (hipSetDevice(dev) != hipSuccess) { g_lu_err = "hipSetDevice failed"; return BOSSS_ERR_HIP; }
+  return BOSSS_OK;
+}
 }
 
 extern "C" {
@@ -170,8 +185,8 @@ const char* bosss_hip_block_last_error(void) { return g_lu_err.c_str(); }
 
 int bosss_hip_block_lu_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t* n, const int64_t* off, double* A,
                            int32_t* ipiv, const int64_t* pivoff, int32_t* info, int32_t want_inverse, void* stream) {
-  (void)ctx;
   if (nblk <= 0) return BOSSS_OK;
+  if (int rc = select_device(ctx)) return rc;
   if (!n || !off || !A || (ipiv && !pivoff)) { g_lu_err = "NULL argument"; return BOSSS_ERR_ARG; }
   if (nmax <= 0 || nmax > LU_MAXN) { g_lu_err = "block size out of range (1..128)"; return BOSSS_ERR_ARG; }
   size_t lds = lu_lds_bytes(nmax, want_inverse);
@@ -188,8 +203,8 @@ int bosss_hip_block_lu_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t
 
 int bosss_hip_block_solve_dev(bosss_ctx* ctx, int nblk, const int32_t* n, const int64_t* off, const double* LU, const int32_t* ipiv,
                               const int64_t* pivoff, const int64_t* xoff, double* x, void* stream) {
-  (void)ctx;
   if (nblk <= 0) return BOSSS_OK;
+  if (int rc = select_device(ctx)) return rc;
   if (!n || !off || !LU || !ipiv || !pivoff || !xoff || !x) { g_lu_err = "NULL argument"; return BOSSS_ERR_ARG; }
   hipLaunchKernelGGL(k_block_solve<double>, dim3((nblk + 63) / 64), dim3(64), 0, (hipStream_t)stream, nblk, n, off, LU, ipiv,
                      pivoff, xoff, x);
@@ -200,8 +215,8 @@ int bosss_hip_block_solve_dev(bosss_ctx* ctx, int nblk, const int32_t* n, const 
 
 int bosss_hip_block_gemv_dev(bosss_ctx* ctx, int nblk, const int32_t* n, const int64_t* off, const double* Ainv, const int64_t* xoff,
                              double alpha, const double* x, double beta, double* y, void* stream) {
-  (void)ctx;
   if (nblk <= 0) return BOSSS_OK;
+  if (int rc = select_device(ctx)) return rc;
   if (!n || !off || !Ainv || !xoff || !x || !y) { g_lu_err = "NULL argument"; return BOSSS_ERR_ARG; }
   hipLaunchKernelGGL(k_block_gemv<double>, dim3(nblk), dim3(64), 0, (hipStream_t)stream, nblk, n, off, Ainv, xoff, alpha, x, beta, y);
   hipError_t e = hipGetLastError();

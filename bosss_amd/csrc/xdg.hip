@@ -1096,7 +1096,7 @@ int bosss_hip_xdg_spmv_dev(bosss_xdg* x, const double* blocks, double alpha, con
 int bosss_hip_xdg_solve_dev(bosss_xdg* x, const double* blocks, const double* b, double* sol, double tol, int32_t maxit,
                             int32_t precision, int32_t use_graph, int32_t check_every, int32_t* iterations, int32_t* outer,
                             double* relres, void* stream) {
-  if (!x || !blocks || !b || !sol || !iterations || !relres) return xfail(BOSSS_ERR_ARG, "NULL argument");
+  if (!x || !blocks || !b || !sol || !iterations || !outer || !relres) return xfail(BOSSS_ERR_ARG, "NULL argument");
   if (precision != BOSSS_FP64 && precision != BOSSS_FP32) return xfail(BOSSS_ERR_ARG, "precision must be BOSSS_FP64 or BOSSS_FP32");
   if (check_every < 1 || check_every > 256) return xfail(BOSSS_ERR_ARG, "check_every must be 1..256");
   if (!(tol > 0)) return xfail(BOSSS_ERR_ARG, "tol must be positive");

@@ -141,7 +141,8 @@ int bosss_hip_bell_spmv_dev(bosss_op* op, const double* blocks, double alpha, co
  *      Blocks are dense row-major at element offsets off[b], sizes n[b] <= nmax <= 128.
  *      ipiv (1-based, LAPACK convention) at pivoff[b]; info[b] as DGETRF's INFO
  *      (0, or j+1 for the first zero pivot; -1 if n[b] exceeds nmax).  In
- *      bosss_hip_block_lu_dev, ipiv (with pivoff) and info may be NULL.
+ *      bosss_hip_block_lu_dev, ipiv (with pivoff) and info may be NULL.  ctx selects the
+ *      device (NULL: the calling thread's current device); stream must belong to it.
  *      want_inverse != 0 overwrites each block with its inverse (DGETRI), n <= 100.
  *      All arrays are device memory; one launch covers the whole batch. ---- */
 int bosss_hip_block_lu_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t* n, const int64_t* off, double* A,
