@@ -30,8 +30,13 @@ def _sync(t):
         torch.cuda.synchronize(t.device)
 
 
-def gmres(apply_A, b, tol, restart=30, maxit=300, precond=None):
-    """Solve A x = b to ||b - A x|| <= tol * ||b||; returns (x, iterations, relative residual)."""
+def gmres(apply_A, b, tol, restart=30, maxit=300, precond=None, true_residual=True):
+    """Solve A x = b to ||b - A x|| <= tol * ||b||; returns (x, iterations, relative residual).
+
+    A cycle that ends unconverged restarts from the true residual b - A x.  One that meets the
+    tolerance by the Hessenberg least-squares estimate is confirmed with the true residual too
+    (one more apply), unless true_residual is False: then the estimate is returned (inexact
+    Newton, whose next nonlinear residual checks the step anyway)."""
     import torch
     n = b.numel()
     x = torch.zeros_like(b)
@@ -90,6 +95,8 @@ def gmres(apply_A, b, tol, restart=30, maxit=300, precond=None):
         yt = torch.from_numpy(y).to(device=b.device, dtype=b.dtype)
         dx = (Zb[:k_done] if precond is not None else Vb[:k_done]).t() @ yt
         x += dx
+        if not true_residual and abs(g[k_done]) / bnorm <= tol:
+            return x, it, abs(g[k_done]) / bnorm
         # true residual for the restart
         apply_A(x, w)
         r = b - w
@@ -142,7 +149,7 @@ class ImplicitStep:
             hist.append(rn)
             if rn <= tol:
                 break
-            d, its, _ = gmres(self._jac(self.op64, U), -r, inner_tol, restart, maxit, precond=precond)
+            d, its, _ = gmres(self._jac(self.op64, U), -r, inner_tol, restart, maxit, precond=precond, true_residual=False)
             n_jvp += its
             U += d
         _sync(U)
@@ -167,7 +174,8 @@ class ImplicitStep:
                 break
             U32 = U.float()
             b32 = (-r / rn).float()                        # scaled so FP32 never underflows
-            d32, its, _ = gmres(self._jac(self.op32, U32), b32, inner_tol, restart, maxit, precond=precond)
+            d32, its, _ = gmres(self._jac(self.op32, U32), b32, inner_tol, restart, maxit, precond=precond,
+                                true_residual=False)
             n_jvp += its
             U += d32.double() * rn
         _sync(U)

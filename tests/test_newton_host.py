@@ -40,6 +40,30 @@ def test_gmres_solves_small_nonsymmetric_system():
     assert rel <= 1e-12 and torch.linalg.vector_norm(A @ x - b) <= 1e-11 * torch.linalg.vector_norm(b)
 
 
+def test_gmres_estimate_mode_saves_one_apply_and_returns_the_same_solution():
+    """true_residual=False (Newton's inner solves): the converged cycle's least-squares
+    estimate is returned instead of an extra apply; the solution is the same."""
+    rng = np.random.default_rng(1)
+    A = torch.from_numpy(np.eye(60) * 5 + rng.uniform(-1, 1, (60, 60)))
+    b = torch.from_numpy(rng.uniform(-1, 1, 60))
+    calls = [0]
+
+    def apply(v, out):
+        calls[0] += 1
+        out.copy_(A @ v)
+
+    x1, it1, rel1 = gmres(apply, b, 1e-8, restart=50, maxit=200)
+    n1, calls[0] = calls[0], 0
+    x2, it2, rel2 = gmres(apply, b, 1e-8, restart=50, maxit=200, true_residual=False)
+    n2 = calls[0]
+    assert it1 == it2 and n2 == n1 - 1 and torch.equal(x1, x2)
+    true = float(torch.linalg.vector_norm(A @ x2 - b) / torch.linalg.vector_norm(b))
+    assert rel2 <= 1e-8 and abs(rel2 - true) <= 1e-3 * true + 1e-15
+    # a cycle that ends unconverged still restarts from the true residual
+    x3, it3, rel3 = gmres(apply, b, 1e-8, restart=5, maxit=400, true_residual=False)
+    assert float(torch.linalg.vector_norm(A @ x3 - b) / torch.linalg.vector_norm(b)) <= 1.01e-8
+
+
 def test_newton_fp64_and_defect_correction_converge():
     n, p = 4, 2
     g = CartesianGrid((n,) * 3, periodic=[True] * 3)
