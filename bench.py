@@ -267,7 +267,8 @@ def secondary_workload(args):
         res["assembled"] = {"assembly_s": round(t_asm, 4), "probe_colours": mb.ncolors, "matrix_bytes": blocks.numel() * 8,
                             "spmv_ms": round(ms_mv, 4), "spmv_gdof_s": round(dof / ms_mv / 1e6, 4),
                             "spmv_gb_s": round(nbytes / ms_mv / 1e6, 1), "spmv_frac_of_8TBs": round(nbytes / ms_mv / 1e6 / 8000, 4),
-                            "max_rel_diff_MU_plus_b_vs_residual": err}
+                            "max_rel_diff_MU_plus_b_vs_residual": err,
+                            "spmv_tile_autotuner": op.spmv_tiles()}
         op.close()
     out = {"metric": "DG operator apply, secondary configuration", "workload": args.workload, "config": desc,
            "dof": dof, "steps": K, "warmup": W0, "dtype": "f64", "data": "synthetic", "results": res}

@@ -16,7 +16,9 @@
 // AccBlock :1388) laid out for one-pass streaming.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <vector>
 
 namespace bosss {
 
@@ -140,22 +142,60 @@ __global__ void __launch_bounds__(256) k_bell_spmv_flat(int J, int Nt, const int
   }
 }
 
-template <int S>
-static bool launch_bell_flat(int J, int Nt, const int32_t* cols, const double* blocks, double alpha, const double* x, double beta,
-                             double* y, hipStream_t st) {
+// Panel sizes (NL: 64-element loads per lane and panel) compiled for the apply.  Every choice
+// computes each row sum in the same order (columns in order, slots in order within a column), so
+// the result does not depend on it; only the number of panels per row and the registers per
+// lane do.  Nt <= 32 stages x in 32-entry rows, Nt <= 128 in 128-entry rows.
+static const int kBellNL32[] = {1, 2, 4, 7, 8, 12, 16};
+static const int kBellNL128[] = {4, 8, 16};
+
+// The fixed rule: the smallest compiled NL whose panel holds a whole block (Nt <= 32), else 16.
+int bell_default_nl(int Nt) {
+  if (Nt > 128) return 0;
+  if (Nt > 32) return 16;
   const int nl = (Nt * Nt + 63) / 64;
+  for (int c : kBellNL32)
+    if (c >= nl) return c;
+  return 16;
+}
+
+// Candidates an autotuner may time: the compiled sizes up to the fixed rule's (larger ones only
+// add padding lanes).
+void bell_candidates(int Nt, std::vector<int>& out) {
+  out.clear();
+  const int def = bell_default_nl(Nt);
+  if (def == 0) return;
+  if (Nt <= 32) {
+    for (int c : kBellNL32)
+      if (c <= def) out.push_back(c);
+  } else {
+    for (int c : kBellNL128) out.push_back(c);
+  }
+}
+
+template <int S>
+static bool launch_bell_flat(int J, int Nt, int nl, const int32_t* cols, const double* blocks, double alpha, const double* x,
+                             double beta, double* y, hipStream_t st) {
   const dim3 g((unsigned)((J + 3) / 4)), b(256);
 #define BELL_FLAT(NL, XMAX) hipLaunchKernelGGL((k_bell_spmv_flat<S, NL, XMAX>), g, b, 0, st, J, Nt, cols, blocks, alpha, x, beta, y)
   if (Nt <= 32) {
-    if (nl <= 1) BELL_FLAT(1, 32);
-    else if (nl <= 2) BELL_FLAT(2, 32);
-    else if (nl <= 4) BELL_FLAT(4, 32);
-    else if (nl <= 7) BELL_FLAT(7, 32);
-    else if (nl <= 8) BELL_FLAT(8, 32);
-    else if (nl <= 12) BELL_FLAT(12, 32);
-    else BELL_FLAT(16, 32);
+    switch (nl) {
+      case 1: BELL_FLAT(1, 32); break;
+      case 2: BELL_FLAT(2, 32); break;
+      case 4: BELL_FLAT(4, 32); break;
+      case 7: BELL_FLAT(7, 32); break;
+      case 8: BELL_FLAT(8, 32); break;
+      case 12: BELL_FLAT(12, 32); break;
+      case 16: BELL_FLAT(16, 32); break;
+      default: return false;
+    }
   } else if (Nt <= 128) {
-    BELL_FLAT(16, 128);
+    switch (nl) {
+      case 4: BELL_FLAT(4, 128); break;
+      case 8: BELL_FLAT(8, 128); break;
+      case 16: BELL_FLAT(16, 128); break;
+      default: return false;
+    }
   } else {
     return false;
   }
@@ -163,10 +203,15 @@ static bool launch_bell_flat(int J, int Nt, const int32_t* cols, const double* b
   return true;
 }
 
+// y = alpha A x + beta y; nl = 0 takes the fixed rule, another value must be one of
+// bell_candidates(Nt) (anything else falls back to the fixed rule).
 void launch_bell_spmv(int J, int Nt, int S, const int32_t* cols, const double* blocks, double alpha, const double* x, double beta,
-                      double* y, hipStream_t s) {
-  if (S == 7 && launch_bell_flat<7>(J, Nt, cols, blocks, alpha, x, beta, y, s)) return;
-  if (S == 5 && launch_bell_flat<5>(J, Nt, cols, blocks, alpha, x, beta, y, s)) return;
+                      double* y, hipStream_t s, int nl) {
+  std::vector<int> cand;
+  bell_candidates(Nt, cand);
+  if (std::find(cand.begin(), cand.end(), nl) == cand.end()) nl = bell_default_nl(Nt);
+  if (S == 7 && launch_bell_flat<7>(J, Nt, nl, cols, blocks, alpha, x, beta, y, s)) return;
+  if (S == 5 && launch_bell_flat<5>(J, Nt, nl, cols, blocks, alpha, x, beta, y, s)) return;
   hipLaunchKernelGGL(k_bell_spmv, dim3(J), dim3(64), 0, s, J, Nt, S, cols, blocks, alpha, x, beta, y);
 }
 

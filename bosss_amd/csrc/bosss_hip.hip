@@ -81,6 +81,11 @@ struct bosss_op {
   int ncolors;
   int32_t *d_color, *d_cols;
   std::vector<int32_t> cols_host;
+  // tile shape of the block-ELL apply (panel size NL): tuned on the first device apply
+  bool spmv_tuned = false;
+  int spmv_nl = 0;
+  std::vector<int> spmv_cand;
+  std::vector<float> spmv_ms;
 };
 
 namespace bosss {
@@ -88,7 +93,9 @@ void launch_probe(int J, int Nt, const int32_t* color, int k, int q, double* X, 
 void launch_extract(int J, int Nt, int S, const int32_t* cols, const int32_t* color, int k, int q, const double* Y, double alpha,
                     double beta, double* blocks, hipStream_t s);
 void launch_bell_spmv(int J, int Nt, int S, const int32_t* cols, const double* blocks, double alpha, const double* x, double beta,
-                      double* y, hipStream_t s);
+                      double* y, hipStream_t s, int nl);
+int bell_default_nl(int Nt);
+void bell_candidates(int Nt, std::vector<int>& out);
 }
 
 namespace bosss {
@@ -617,6 +624,57 @@ static int ensure_matrix_setup(bosss_op* op) {
   return BOSSS_OK;
 }
 
+// Autotuner of the block-ELL apply's panel size: times every candidate on this operator's
+// matrix and x (5 launches each after one warm-up, HIP events on the caller's stream, output
+// into a scratch vector) and keeps the fastest.  Under stream capture it does not time (events
+// and synchronisation are not allowed there) and the fixed rule is used until a later call.
+static int tune_spmv(bosss_op* op, const double* blocks, const double* x, hipStream_t s) {
+  const int J = op->grid->J, Nt = op->V * op->N, S = 1 + op->grid->NF;
+  std::vector<int> cand;
+  bell_candidates(Nt, cand);
+  op->spmv_cand = cand;
+  op->spmv_ms.assign(cand.size(), -1.0f);
+  if (cand.size() <= 1) {
+    op->spmv_nl = cand.empty() ? 0 : cand[0];
+    op->spmv_tuned = true;
+    return BOSSS_OK;
+  }
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIPCHK(hipStreamIsCapturing(s, &cs));
+  if (cs != hipStreamCaptureStatusNone) {
+    op->spmv_nl = bell_default_nl(Nt);
+    return BOSSS_OK;
+  }
+  double* ys = nullptr;
+  HIPCHK(hipMalloc((void**)&ys, op->ndof * sizeof(double)));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  int best = bell_default_nl(Nt);
+  float best_ms = 1e30f;
+  for (size_t i = 0; i < cand.size() && e == hipSuccess; i++) {
+    launch_bell_spmv(J, Nt, S, op->d_cols, blocks, 1.0, x, 0.0, ys, s, cand[i]);
+    e = hipEventRecord(e0, s);
+    for (int r = 0; r < 5 && e == hipSuccess; r++) launch_bell_spmv(J, Nt, S, op->d_cols, blocks, 1.0, x, 0.0, ys, s, cand[i]);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess) {
+      op->spmv_ms[i] = ms / 5;
+      if (ms < best_ms) { best_ms = ms; best = cand[i]; }
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  hipError_t ef = hipFree(ys);
+  if (e != hipSuccess) return fail(BOSSS_ERR_HIP, "apply autotuner: %s", hipGetErrorString(e));
+  if (ef != hipSuccess) return fail(BOSSS_ERR_HIP, "apply autotuner: %s", hipGetErrorString(ef));
+  op->spmv_nl = best;
+  op->spmv_tuned = true;
+  return BOSSS_OK;
+}
+
 extern "C" {
 
 int bosss_hip_op_matrix_layout(bosss_op* op, int32_t* blocks_per_row, int64_t* entries, int32_t* ncolors, int32_t* cols) {
@@ -655,8 +713,39 @@ int bosss_hip_bell_spmv_dev(bosss_op* op, const double* blocks, double alpha, co
   int rc;
   HIPCHK(hipSetDevice(op->ctx->device));
   if ((rc = ensure_matrix_setup(op))) return rc;
-  launch_bell_spmv(op->grid->J, op->V * op->N, 1 + op->grid->NF, op->d_cols, blocks, alpha, x, beta, y, (hipStream_t)stream);
+  if (!op->spmv_tuned && (rc = tune_spmv(op, blocks, x, (hipStream_t)stream))) return rc;
+  launch_bell_spmv(op->grid->J, op->V * op->N, 1 + op->grid->NF, op->d_cols, blocks, alpha, x, beta, y, (hipStream_t)stream,
+                   op->spmv_nl);
   HIPCHK(hipGetLastError());
+  return BOSSS_OK;
+}
+
+int bosss_hip_op_set_spmv_tile(bosss_op* op, int32_t nl) {
+  if (!op) return fail(BOSSS_ERR_ARG, "op is NULL");
+  std::vector<int> cand;
+  bell_candidates(op->V * op->N, cand);
+  if (nl == 0) {
+    op->spmv_tuned = false;
+    op->spmv_nl = 0;
+    return BOSSS_OK;
+  }
+  if (std::find(cand.begin(), cand.end(), (int)nl) == cand.end())
+    return fail(BOSSS_ERR_ARG, "panel size %d is not a candidate for %d x %d blocks", nl, op->V * op->N, op->V * op->N);
+  op->spmv_nl = nl;
+  op->spmv_tuned = true;
+  return BOSSS_OK;
+}
+
+int bosss_hip_op_spmv_tiles(const bosss_op* op, int32_t* chosen, int32_t* ncand, int32_t* cand, float* ms, int32_t max) {
+  if (!op) return fail(BOSSS_ERR_ARG, "op is NULL");
+  std::vector<int> c;
+  bell_candidates(op->V * op->N, c);
+  if (chosen) *chosen = op->spmv_tuned ? op->spmv_nl : 0;
+  if (ncand) *ncand = (int32_t)c.size();
+  for (int i = 0; i < (int)c.size() && i < max; i++) {
+    if (cand) cand[i] = c[i];
+    if (ms) ms[i] = (i < (int)op->spmv_ms.size() && op->spmv_cand == c) ? op->spmv_ms[i] : -1.0f;
+  }
   return BOSSS_OK;
 }
 

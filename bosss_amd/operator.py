@@ -207,6 +207,20 @@ class DifferentialOperator:
         self._check_matrix_args(blocks, x, y)
         check(lib().bosss_hip_bell_spmv_dev(self.h, self._ptr(blocks), alpha, self._ptr(x), beta, self._ptr(y), self._stream(stream)))
 
+    def set_spmv_tile(self, nl):
+        """Fix the panel size of the block-ELL apply (0: tune again at the next apply)."""
+        check(lib().bosss_hip_op_set_spmv_tile(self.h, int(nl)))
+
+    def spmv_tiles(self):
+        """{"chosen": NL or 0, "ms": {candidate NL: timed ms per apply or None}} of the apply's autotuner."""
+        chosen, nc = ctypes.c_int32(), ctypes.c_int32()
+        check(lib().bosss_hip_op_spmv_tiles(self.h, ctypes.byref(chosen), ctypes.byref(nc), None, None, 0))
+        cand = (ctypes.c_int32 * max(nc.value, 1))()
+        ms = (ctypes.c_float * max(nc.value, 1))()
+        check(lib().bosss_hip_op_spmv_tiles(self.h, None, None, cand, ms, nc.value))
+        return {"chosen": chosen.value,
+                "ms": {int(cand[i]): (round(float(ms[i]), 5) if ms[i] >= 0 else None) for i in range(nc.value)}}
+
     def GetMatrixBuilder(self):
         return EvaluatorLinear(self)
 

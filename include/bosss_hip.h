@@ -134,6 +134,15 @@ int bosss_hip_op_matrix_layout(bosss_op* op, int32_t* blocks_per_row, int64_t* e
 int bosss_hip_op_assemble_dev(bosss_op* op, const double* U, double alpha, double beta, double* blocks, void* stream);
 /* y <- alpha*A x + beta*y with A in the operator's block-ELL layout (BlockMsrMatrix.SpMV, BlockMsrMatrix.cs:2554) */
 int bosss_hip_bell_spmv_dev(bosss_op* op, const double* blocks, double alpha, const double* x, double beta, double* y, void* stream);
+/* Tile shape of that apply: the panel size NL (64-element loads per lane and panel) of the
+ * streaming kernel.  The first bosss_hip_bell_spmv_dev call on an operator times every candidate
+ * on its own matrix and keeps the fastest (an autotuner; not while the stream is being
+ * captured).  nl = 0 asks for a new tuning at the next apply; another value fixes the shape and
+ * must be one of the candidates.  Results do not depend on the shape (same summation order). */
+int bosss_hip_op_set_spmv_tile(bosss_op* op, int32_t nl);
+/* The chosen NL (0 until tuned), the number of candidates, and up to max candidates with their
+ * timed ms per apply (< 0: not timed). */
+int bosss_hip_op_spmv_tiles(const bosss_op* op, int32_t* chosen, int32_t* ncand, int32_t* cand, float* ms, int32_t max);
 
 /* ---- batched uneven cell-block factorisation (replaces BlockJacobi.InitImpl's
  *      ReadBlock -> InvertInPlace -> LAPACK DGETRF/DGETRI per block, BlockJacobi.cs:63-120,

@@ -101,3 +101,50 @@ def test_block_jacobi_and_pcg_on_sip():
     say("  PCG iterations %d, rel residual %.2e" % (its, rel))
     assert rel <= 1e-10
     assert np.abs(A @ sol.cpu().numpy() - bh).max() < 1e-8 * np.abs(bh).max()
+
+
+@pytest.mark.parametrize("form,n,p", [("advdiff", 4, 3), ("euler_hllc", 3, 2)])
+def test_spmv_tile_autotuner(form, n, p):
+    """The apply's autotuner: every candidate panel size gives bitwise the same y (each row sum
+    keeps its order), the tuned choice is one of the timed candidates, a shape outside the
+    candidates is refused, and the result still matches the oracle."""
+    import torch
+    from bosss_amd._lib import BosssError
+    from bosss_amd.grid import CartesianGrid
+    from bosss_amd.operator import DifferentialOperator, default_quad_order
+    from bosss_amd.workloads import euler_state
+    say("case spmv tiles", form, n, p)
+    g = CartesianGrid((n,) * 3, periodic=[True] * 3)
+    op = DifferentialOperator(g, form, p, nu=-0.01, velocity=(1.0, 0.5, 0.25))
+    ref = RefModel(3, (n,) * 3, p, default_quad_order(form, p), periodic=[1, 1, 1])
+    prm = params_vector(1.3, -0.01, (1.0, 0.5, 0.25), 1.4)
+    U = euler_state(g, op.N, seed=3) if form.startswith("euler") else np.zeros(op.ndof)
+    Ut = torch.from_numpy(U).cuda()
+    B = op.GetMatrixBuilder().new_matrix()
+    op.assemble_dev(Ut, B)
+    x = torch.from_numpy(np.random.default_rng(2).uniform(-1, 1, op.ndof)).cuda()
+    y = torch.zeros_like(x)
+    op.spmv_dev(B, x, y)                   # first apply: tunes
+    info = op.spmv_tiles()
+    say("  tiles", info)
+    cand = sorted(info["ms"])
+    assert len(cand) >= 2 and info["chosen"] in cand
+    assert all(info["ms"][c] is not None and info["ms"][c] > 0 for c in cand)
+    outs = {}
+    for c in cand:
+        op.set_spmv_tile(c)
+        yc = torch.zeros_like(x)
+        op.spmv_dev(B, x, yc)
+        outs[c] = yc
+    torch.cuda.synchronize()
+    for c in cand:
+        assert torch.equal(outs[c], y), c
+    with pytest.raises(BosssError):
+        op.set_spmv_tile(3)
+    op.set_spmv_tile(0)                    # tune again at the next apply
+    y2 = torch.zeros_like(x)
+    op.spmv_dev(B, x, y2)
+    torch.cuda.synchronize()
+    assert torch.equal(y2, y) and op.spmv_tiles()["chosen"] in cand
+    ref_y = ref.evaluate(KIND[form], prm, x.cpu().numpy()) if not form.startswith("euler") else ref.jvp(KIND[form], prm, U, x.cpu().numpy())
+    assert np.abs(y.cpu().numpy() - ref_y).max() < 1e-12 * np.abs(ref_y).max()
