@@ -1,7 +1,8 @@
 """GPU parity: the HIP kernels against the CPU oracle on identical seeded inputs.
 
 FP64 bar: 1e-12 relative, measured per field against max|r| of that field and
-normwise (SURVEY-defined metric; the reference's own tables carry ~15 digits).
+normwise (per-entry relative error blows up where entries cancel; the reference's own
+tables carry ~15 digits).
 """
 import sys
 
