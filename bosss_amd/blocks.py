@@ -20,8 +20,8 @@ class BlockBatch:
     def __init__(self, sizes, device="cuda:0"):
         import torch
         self.sizes = np.asarray(sizes, dtype=np.int32)
-        if self.sizes.min() < 1 or self.sizes.max() > 128:
-            raise ValueError("block sizes must be in 1..128")
+        if self.sizes.min() < 1 or self.sizes.max() > 4096:
+            raise ValueError("block sizes must be in 1..4096 (above 128: factored in global memory, no inverse)")
         n64 = self.sizes.astype(np.int64)
         off = np.zeros(len(n64), np.int64); off[1:] = np.cumsum(n64 * n64)[:-1]
         xoff = np.zeros(len(n64), np.int64); xoff[1:] = np.cumsum(n64)[:-1]

@@ -116,6 +116,75 @@ __global__ void __launch_bounds__(64) k_block_lu(int nblk, int nmax, const int32
   if (lane == 0 && info) info[b] = inf;
 }
 
+// Blocks above LU_MAXN (the multi-cell blocks of Schwarz smoothers, up to LU_BIG_MAXN): one
+// workgroup of 256 threads per block, DGETF2 in place in global memory with the operation order
+// of the small-block kernel -- pivot = first row with the largest |a| in the column (workgroup
+// argmax, ties to the smaller row), row swap, column scaled by the reciprocal pivot, rank-1
+// update of the trailing rows (one row per wavefront, lanes along the row: coalesced).  A zero
+// pivot sets INFO and the factorisation continues.  O(n^3) global-memory traffic per block,
+// served mostly by L2; it makes such blocks factorable on the device, it is not a blocked LU.
+constexpr int LU_BIG_MAXN = 4096;
+template <typename Real>
+__global__ void __launch_bounds__(256) k_block_lu_big(int nblk, int nmax, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
+                                                      Real* __restrict__ A, int32_t* __restrict__ ipiv, const int64_t* __restrict__ pivoff,
+                                                      int32_t* __restrict__ info) {
+  __shared__ Real sv[4];
+  __shared__ int si[4];
+  __shared__ int s_piv;
+  const int b = blockIdx.x;
+  if (b >= nblk) return;
+  const int n = nsz[b];
+  if (n <= 0 || n > nmax) {
+    if (threadIdx.x == 0 && info) info[b] = -1;
+    return;
+  }
+  Real* a = A + off[b];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int inf = 0;
+  for (int j = 0; j < n; j++) {
+    Real best = Real(-1);
+    int bi = n;
+    for (int i = j + t; i < n; i += 256) {
+      const Real v = fabs(a[(size_t)i * n + j]);
+      if (v > best || (v == best && i < bi)) { best = v; bi = i; }
+    }
+    wave_argmax(best, bi);
+    if (lane == 0) { sv[w] = best; si[w] = bi; }
+    __syncthreads();
+    if (t == 0) {
+      Real bv = sv[0];
+      int bx = si[0];
+      for (int k = 1; k < 4; k++)
+        if (sv[k] > bv || (sv[k] == bv && si[k] < bx)) { bv = sv[k]; bx = si[k]; }
+      s_piv = bx < n ? bx : j;   // no comparable candidate (NaN column): treated as a zero pivot below
+    }
+    __syncthreads();
+    const int p = s_piv;
+    if (t == 0 && ipiv) ipiv[pivoff[b] + j] = p + 1;
+    if (!(a[(size_t)p * n + j] != Real(0))) {
+      if (!inf) inf = j + 1;
+      __syncthreads();
+      continue;
+    }
+    if (p != j)
+      for (int c = t; c < n; c += 256) {
+        const Real x = a[(size_t)j * n + c];
+        a[(size_t)j * n + c] = a[(size_t)p * n + c];
+        a[(size_t)p * n + c] = x;
+      }
+    __syncthreads();
+    const Real inv = Real(1) / a[(size_t)j * n + j];
+    for (int i = j + 1 + t; i < n; i += 256) a[(size_t)i * n + j] *= inv;
+    __syncthreads();
+    for (int i = j + 1 + w; i < n; i += 4) {
+      const Real l = a[(size_t)i * n + j];
+      for (int c = j + 1 + lane; c < n; c += 64) a[(size_t)i * n + c] -= l * a[(size_t)j * n + c];
+    }
+    __syncthreads();
+  }
+  if (t == 0 && info) info[b] = inf;
+}
+
 size_t lu_lds_bytes(int nmax, int want_inverse) {
   return (size_t)(want_inverse ? 2 : 1) * nmax * (nmax | 1) * sizeof(double) + (size_t)nmax * sizeof(int);
 }
@@ -188,7 +257,14 @@ int bosss_hip_block_lu_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t
   if (nblk <= 0) return BOSSS_OK;
   if (int rc = select_device(ctx)) return rc;
   if (!n || !off || !A || (ipiv && !pivoff)) { g_lu_err = "NULL argument"; return BOSSS_ERR_ARG; }
-  if (nmax <= 0 || nmax > LU_MAXN) { g_lu_err = "block size out of range (1..128)"; return BOSSS_ERR_ARG; }
+  if (nmax <= 0 || nmax > LU_BIG_MAXN) { g_lu_err = "block size out of range (1..4096)"; return BOSSS_ERR_ARG; }
+  if (nmax > LU_MAXN) {
+    if (want_inverse) { g_lu_err = "block too large for an in-LDS inverse (n <= 100)"; return BOSSS_ERR_UNSUPPORTED; }
+    hipLaunchKernelGGL(k_block_lu_big<double>, dim3(nblk), dim3(256), 0, (hipStream_t)stream, nblk, nmax, n, off, A, ipiv, pivoff, info);
+    hipError_t eb = hipGetLastError();
+    if (eb != hipSuccess) { g_lu_err = hipGetErrorString(eb); return BOSSS_ERR_HIP; }
+    return BOSSS_OK;
+  }
   size_t lds = lu_lds_bytes(nmax, want_inverse);
   if (lds > 160 * 1024) { g_lu_err = "block too large for an in-LDS inverse (n <= 100)"; return BOSSS_ERR_UNSUPPORTED; }
   if (hipFuncSetAttribute((const void*)k_block_lu<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {

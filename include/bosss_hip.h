@@ -146,12 +146,15 @@ int bosss_hip_op_spmv_tiles(const bosss_op* op, int32_t* chosen, int32_t* ncand,
 
 /* ---- batched uneven cell-block factorisation (replaces BlockJacobi.InitImpl's
  *      ReadBlock -> InvertInPlace -> LAPACK DGETRF/DGETRI per block, BlockJacobi.cs:63-120,
- *      IMatrixExtensions.cs:2128-2230, and Schwarz's LAPACK block solver, Schwarz.cs:827-915,
- *      for Schwarz blocks of at most 128 DOF; larger multi-cell Schwarz blocks, which the
- *      reference factors with LAPACK up to 4096 DOF or PARDISO, are not covered).
- *      Blocks are dense row-major at element offsets off[b], sizes n[b] <= nmax <= 128.
+ *      IMatrixExtensions.cs:2128-2230, and Schwarz's LAPACK block solver for blocks up to
+ *      4096 DOF, Schwarz.cs:827-915; above that the reference switches to PARDISO).
+ *      Blocks are dense row-major at element offsets off[b], sizes n[b] <= nmax <= 4096:
+ *      nmax <= 128 stages each block in LDS (one wavefront per block); larger nmax
+ *      factors in place in global memory (one 256-thread workgroup per block, same
+ *      pivoting and operation order).
  *      ipiv (1-based, LAPACK convention) at pivoff[b]; info[b] as DGETRF's INFO
- *      (0, or j+1 for the first zero pivot; -1 if n[b] exceeds nmax).  In
+ *      (0, or j+1 for the first zero pivot; -1 if n[b] exceeds nmax).  The inverse
+ *      (want_inverse) needs nmax <= 100.  In
  *      bosss_hip_block_lu_dev, ipiv (with pivoff) and info may be NULL.  ctx selects the
  *      device (NULL: the calling thread's current device); stream must belong to it.
  *      want_inverse != 0 overwrites each block with its inverse (DGETRI), n <= 100.

@@ -88,3 +88,52 @@ def test_singular_block_reports_info():
     torch.cuda.synchronize()
     _, _, inforef, _, _ = ref_block_lu(sizes, blocks)
     assert list(info.cpu().numpy()) == [0, 3, 0] == list(inforef)
+
+
+@pytest.mark.parametrize("sizes", [[129, 200, 10, 257, 20], [512, 300]])
+def test_large_blocks_factor_in_global_memory(sizes):
+    """Blocks above 128 DOF (multi-cell Schwarz blocks): the global-memory getf2 gives the
+    oracle's pivots and factors, solves match numpy, small blocks in the same batch too."""
+    import torch
+    from bosss_amd.blocks import BlockBatch
+    say("case large blocks", sizes)
+    blocks = make_blocks(sizes, 7)
+    bb = BlockBatch(sizes)
+    A = torch.from_numpy(np.concatenate([b.ravel() for b in blocks])).to(bb.dev)
+    ipiv = torch.zeros(bb.ndof, dtype=torch.int32, device=bb.dev)
+    info = torch.zeros(bb.nblk, dtype=torch.int32, device=bb.dev)
+    bb.factor(A, ipiv, info)
+    torch.cuda.synchronize()
+    LUref, pivref, inforef, off, poff = ref_block_lu(sizes, blocks)
+    assert (info.cpu().numpy() == inforef).all() and (inforef == 0).all()
+    assert (ipiv.cpu().numpy() == pivref + 1).all()
+    LU = A.cpu().numpy()
+    assert np.abs(LU - LUref).max() < 1e-12 * np.abs(LUref).max()
+    x = np.random.default_rng(3).uniform(-1, 1, bb.ndof)
+    xt = torch.from_numpy(x.copy()).to(bb.dev)
+    bb.solve(A, ipiv, xt)
+    xs = xt.cpu().numpy()
+    for b, n in enumerate(sizes):
+        o = bb.xoff[b]
+        ref = np.linalg.solve(blocks[b], x[o:o + n])
+        assert np.abs(xs[o:o + n] - ref).max() < 1e-10 * max(1, np.abs(ref).max())
+
+
+def test_large_block_zero_pivot_and_inverse_limit():
+    import torch
+    from bosss_amd.blocks import BlockBatch
+    say("case large block singular")
+    sizes = [150, 140]
+    blocks = make_blocks(sizes, 9)
+    blocks[0][:, 37] = 0.0                  # zero column -> zero pivot at j = 37 (INFO = 38)
+    bb = BlockBatch(sizes)
+    A = torch.from_numpy(np.concatenate([b.ravel() for b in blocks])).to(bb.dev)
+    ipiv = torch.zeros(bb.ndof, dtype=torch.int32, device=bb.dev)
+    info = torch.zeros(bb.nblk, dtype=torch.int32, device=bb.dev)
+    bb.factor(A, ipiv, info)
+    torch.cuda.synchronize()
+    _, pivref, inforef, _, _ = ref_block_lu(sizes, blocks)
+    assert list(info.cpu().numpy()) == [38, 0] == list(inforef)
+    assert (ipiv.cpu().numpy() == pivref + 1).all()
+    with pytest.raises(RuntimeError):
+        bb.factor(A, None, info, inverse=True)   # inverses are limited to 100 DOF
