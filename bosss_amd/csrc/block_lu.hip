@@ -124,6 +124,10 @@ __global__ void __launch_bounds__(64) k_block_lu(int nblk, int nmax, const int32
 // pivot sets INFO and the factorisation continues.  O(n^3) global-memory traffic per block,
 // served mostly by L2; it makes such blocks factorable on the device, it is not a blocked LU.
 constexpr int LU_BIG_MAXN = 4096;
+// the two-launch-per-column form takes over for blocks of at least LU_SPLIT_MINN when the batch
+// has fewer than LU_SPLIT_MAXBLK blocks (fewer blocks than CUs leave the one-workgroup form idle)
+constexpr int LU_SPLIT_MINN = 512;
+constexpr int LU_SPLIT_MAXBLK = 256;
 template <typename Real>
 __global__ void __launch_bounds__(256) k_block_lu_big(int nblk, int nmax, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
                                                       Real* __restrict__ A, int32_t* __restrict__ ipiv, const int64_t* __restrict__ pivoff,
@@ -183,6 +187,84 @@ __global__ void __launch_bounds__(256) k_block_lu_big(int nblk, int nmax, const 
     __syncthreads();
   }
   if (t == 0 && info) info[b] = inf;
+}
+
+// The same factorisation with each column step split over two launches, for few large blocks
+// (one workgroup per block would leave most CUs idle): k_lu_step does the pivot search, row swap
+// and column scaling of step j (one workgroup per block), k_lu_update the rank-1 update of the
+// trailing rows (four rows per workgroup, grid over all blocks' rows).  st[b] holds the block's
+// INFO so far and st[nblk + b] whether step j updates it (0: finished, invalid, or a zero pivot).
+template <typename Real>
+__global__ void __launch_bounds__(256) k_lu_step(int j, int nblk, int nmax, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
+                                                 Real* __restrict__ A, int32_t* __restrict__ ipiv, const int64_t* __restrict__ pivoff,
+                                                 int32_t* __restrict__ st) {
+  __shared__ Real sv[4];
+  __shared__ int si[4];
+  __shared__ int s_piv;
+  const int b = blockIdx.x;
+  if (b >= nblk) return;
+  const int n = nsz[b];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (j == 0 && t == 0) st[b] = (n <= 0 || n > nmax) ? -1 : 0;
+  if (n <= 0 || n > nmax || j >= n) {
+    if (t == 0) st[nblk + b] = 0;
+    return;
+  }
+  Real* a = A + off[b];
+  Real best = Real(-1);
+  int bi = n;
+  for (int i = j + t; i < n; i += 256) {
+    const Real v = fabs(a[(size_t)i * n + j]);
+    if (v > best || (v == best && i < bi)) { best = v; bi = i; }
+  }
+  wave_argmax(best, bi);
+  if (lane == 0) { sv[w] = best; si[w] = bi; }
+  __syncthreads();
+  if (t == 0) {
+    Real bv = sv[0];
+    int bx = si[0];
+    for (int k = 1; k < 4; k++)
+      if (sv[k] > bv || (sv[k] == bv && si[k] < bx)) { bv = sv[k]; bx = si[k]; }
+    s_piv = bx < n ? bx : j;
+  }
+  __syncthreads();
+  const int p = s_piv;
+  if (t == 0 && ipiv) ipiv[pivoff[b] + j] = p + 1;
+  if (!(a[(size_t)p * n + j] != Real(0))) {
+    if (t == 0) {
+      if (st[b] == 0) st[b] = j + 1;
+      st[nblk + b] = 0;
+    }
+    return;
+  }
+  if (p != j)
+    for (int c = t; c < n; c += 256) {
+      const Real x = a[(size_t)j * n + c];
+      a[(size_t)j * n + c] = a[(size_t)p * n + c];
+      a[(size_t)p * n + c] = x;
+    }
+  __syncthreads();
+  const Real inv = Real(1) / a[(size_t)j * n + j];
+  for (int i = j + 1 + t; i < n; i += 256) a[(size_t)i * n + j] *= inv;
+  if (t == 0) st[nblk + b] = 1;
+}
+
+template <typename Real>
+__global__ void __launch_bounds__(256) k_lu_update(int j, int nblk, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
+                                                   Real* __restrict__ A, const int32_t* __restrict__ st) {
+  const int b = blockIdx.y;
+  if (b >= nblk || !st[nblk + b]) return;
+  const int n = nsz[b];
+  const int i = j + 1 + blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (i >= n) return;
+  Real* a = A + off[b];
+  const Real l = a[(size_t)i * n + j];
+  for (int c = j + 1 + lane; c < n; c += 64) a[(size_t)i * n + c] -= l * a[(size_t)j * n + c];
+}
+
+__global__ void k_lu_info(int nblk, const int32_t* __restrict__ st, int32_t* __restrict__ info) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nblk) info[b] = st[b];
 }
 
 size_t lu_lds_bytes(int nmax, int want_inverse) {
@@ -260,7 +342,25 @@ int bosss_hip_block_lu_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t
   if (nmax <= 0 || nmax > LU_BIG_MAXN) { g_lu_err = "block size out of range (1..4096)"; return BOSSS_ERR_ARG; }
   if (nmax > LU_MAXN) {
     if (want_inverse) { g_lu_err = "block too large for an in-LDS inverse (n <= 100)"; return BOSSS_ERR_UNSUPPORTED; }
-    hipLaunchKernelGGL(k_block_lu_big<double>, dim3(nblk), dim3(256), 0, (hipStream_t)stream, nblk, nmax, n, off, A, ipiv, pivoff, info);
+    hipStream_t s = (hipStream_t)stream;
+    if (nmax >= LU_SPLIT_MINN && nblk < LU_SPLIT_MAXBLK) {
+      // few large blocks: two launches per column step, the trailing update over many workgroups
+      int32_t* st = nullptr;
+      hipError_t e = hipMalloc((void**)&st, 2 * (size_t)nblk * sizeof(int32_t));
+      if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
+      for (int j = 0; j < nmax; j++) {
+        hipLaunchKernelGGL(k_lu_step<double>, dim3(nblk), dim3(256), 0, s, j, nblk, nmax, n, off, A, ipiv, pivoff, st);
+        const unsigned rows = (unsigned)((nmax - j - 1 + 3) / 4);
+        if (rows) hipLaunchKernelGGL(k_lu_update<double>, dim3(rows, nblk), dim3(256), 0, s, j, nblk, n, off, A, st);
+      }
+      if (info) hipLaunchKernelGGL(k_lu_info, dim3((nblk + 255) / 256), dim3(256), 0, s, nblk, st, info);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(s);   // st is freed next
+      (void)hipFree(st);
+      if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
+      return BOSSS_OK;
+    }
+    hipLaunchKernelGGL(k_block_lu_big<double>, dim3(nblk), dim3(256), 0, s, nblk, nmax, n, off, A, ipiv, pivoff, info);
     hipError_t eb = hipGetLastError();
     if (eb != hipSuccess) { g_lu_err = hipGetErrorString(eb); return BOSSS_ERR_HIP; }
     return BOSSS_OK;

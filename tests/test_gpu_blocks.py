@@ -90,7 +90,7 @@ def test_singular_block_reports_info():
     assert list(info.cpu().numpy()) == [0, 3, 0] == list(inforef)
 
 
-@pytest.mark.parametrize("sizes", [[129, 200, 10, 257, 20], [512, 300]])
+@pytest.mark.parametrize("sizes", [[129, 200, 10, 257, 20], [512, 300], [700, 64, 515]])
 def test_large_blocks_factor_in_global_memory(sizes):
     """Blocks above 128 DOF (multi-cell Schwarz blocks): the global-memory getf2 gives the
     oracle's pivots and factors, solves match numpy, small blocks in the same batch too."""
@@ -119,11 +119,11 @@ def test_large_blocks_factor_in_global_memory(sizes):
         assert np.abs(xs[o:o + n] - ref).max() < 1e-10 * max(1, np.abs(ref).max())
 
 
-def test_large_block_zero_pivot_and_inverse_limit():
+@pytest.mark.parametrize("sizes", [[150, 140], [600, 520]])   # one-workgroup form; two launches per column
+def test_large_block_zero_pivot_and_inverse_limit(sizes):
     import torch
     from bosss_amd.blocks import BlockBatch
-    say("case large block singular")
-    sizes = [150, 140]
+    say("case large block singular", sizes)
     blocks = make_blocks(sizes, 9)
     blocks[0][:, 37] = 0.0                  # zero column -> zero pivot at j = 37 (INFO = 38)
     bb = BlockBatch(sizes)
