@@ -271,21 +271,46 @@ size_t lu_lds_bytes(int nmax, int want_inverse) {
   return (size_t)(want_inverse ? 2 : 1) * nmax * (nmax | 1) * sizeof(double) + (size_t)nmax * sizeof(int);
 }
 
-// x_b <- LU_b \ x_b   (getrs with the 1-based pivots)
+// x_b <- LU_b \ x_b   (getrs with the 1-based pivots), one wavefront per block, any size up to
+// 4096.  Lane l owns the rows i = l (mod 64) of x_b and is the only one to write them: the
+// substitutions run column by column, the finished x_j reaches the other lanes by a shuffle from
+// its owner, and each owner subtracts L_ij x_j (U_ij x_j) from its rows.  The forward sweep adds
+// the terms of a row in getrs' order (k = 0 .. i-1), the backward sweep from the last column.
 template <typename Real>
 __global__ void __launch_bounds__(64) k_block_solve(int nblk, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
                                                     const Real* __restrict__ LU, const int32_t* __restrict__ ipiv,
                                                     const int64_t* __restrict__ pivoff, const int64_t* __restrict__ xoff,
                                                     Real* __restrict__ x) {
-  const int b = blockIdx.x * 64 + threadIdx.x;   // one lane per block (small blocks, many of them)
+  const int b = blockIdx.x;
   if (b >= nblk) return;
   const int n = nsz[b];
+  if (n <= 0) return;
+  const int lane = threadIdx.x;
   const Real* L = LU + off[b];
   const int32_t* pv = ipiv + pivoff[b];
   Real* xb = x + xoff[b];
-  for (int j = 0; j < n; j++) { int p = pv[j] - 1; if (p != j) { Real t = xb[j]; xb[j] = xb[p]; xb[p] = t; } }
-  for (int i = 0; i < n; i++) { Real a = xb[i]; for (int k = 0; k < i; k++) a -= L[i * n + k] * xb[k]; xb[i] = a; }
-  for (int i = n - 1; i >= 0; i--) { Real a = xb[i]; for (int k = i + 1; k < n; k++) a -= L[i * n + k] * xb[k]; xb[i] = a / L[i * n + i]; }
+  if (lane == 0)
+    for (int j = 0; j < n; j++) {
+      const int p = pv[j] - 1;
+      if (p != j) { const Real t = xb[j]; xb[j] = xb[p]; xb[p] = t; }
+    }
+  __syncthreads();   // the interchanged entries are read by their owners below
+  for (int j = 0; j < n; j++) {
+    Real xj = lane == (j & 63) ? xb[j] : Real(0);
+    xj = __shfl(xj, j & 63, 64);
+    int i = lane;
+    if (i <= j) i += ((j - lane) / 64 + 1) * 64;   // first owned row below j
+    for (; i < n; i += 64) xb[i] -= L[(size_t)i * n + j] * xj;
+  }
+  for (int j = n - 1; j >= 0; j--) {
+    Real xj = Real(0);
+    if (lane == (j & 63)) {
+      xj = xb[j] / L[(size_t)j * n + j];
+      xb[j] = xj;
+    }
+    xj = __shfl(xj, j & 63, 64);
+    for (int i = lane; i < j; i += 64) xb[i] -= L[(size_t)i * n + j] * xj;
+  }
 }
 
 // y_b <- alpha * Ainv_b x_b + beta * y_b   (block-diagonal apply with explicit inverses, one wave per block)
@@ -382,7 +407,7 @@ int bosss_hip_block_solve_dev(bosss_ctx* ctx, int nblk, const int32_t* n, const 
   if (nblk <= 0) return BOSSS_OK;
   if (int rc = select_device(ctx)) return rc;
   if (!n || !off || !LU || !ipiv || !pivoff || !xoff || !x) { g_lu_err = "NULL argument"; return BOSSS_ERR_ARG; }
-  hipLaunchKernelGGL(k_block_solve<double>, dim3((nblk + 63) / 64), dim3(64), 0, (hipStream_t)stream, nblk, n, off, LU, ipiv,
+  hipLaunchKernelGGL(k_block_solve<double>, dim3(nblk), dim3(64), 0, (hipStream_t)stream, nblk, n, off, LU, ipiv,
                      pivoff, xoff, x);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
