@@ -154,3 +154,116 @@ def pcg(apply_A, b, precond, tol=1e-10, maxit=1000):
         p.mul_(rz_new / rz).add_(z)
         rz = rz_new
     return x, it, rel
+
+
+class Schwarz:
+    """Additive Schwarz smoother over multi-cell blocks (host mirror of Schwarz,
+    Schwarz.cs:985-1080; block factorisation as InitBlockSolver, Schwarz.cs:827-915).
+
+    parts: lists of cell indices; with overlap > 0 each part grows by that many layers of
+    neighbouring cells (the matrix pattern's neighbours).  Init reads every part's dense
+    block from the operator matrix in block-ELL layout (one gather) and factors all parts in
+    one batched launch (getrf semantics; parts above 128 DOF factor in global memory).
+    Solve(x, b, sweeps): per sweep, Res = b - M x, then x += sum_parts R_p^T A_p^-1 R_p Res,
+    the corrections of overlapping cells summed in part order (no overlap scaling: the
+    reference's default EnableOverlapScaling = false).
+    """
+
+    def __init__(self, op, blocks, parts, overlap=0):
+        import torch
+        self.op = op
+        self.blocks = blocks
+        S, _, _, cols = op.matrix_layout()
+        Nt = op.nvars * op.N
+        J = op.grid.J
+        parts = [sorted(set(int(c) for c in p)) for p in parts]
+        for _ in range(int(overlap)):
+            grown = []
+            for p in parts:
+                nb = set(p)
+                for c in p:
+                    nb.update(int(k) for k in cols[c] if k >= 0)
+                grown.append(sorted(nb))
+            parts = grown
+        self.parts = parts
+        sizes = [len(p) * Nt for p in parts]
+        if max(sizes) > 4096:
+            raise ValueError("a Schwarz block holds %d DOF; at most 4096" % max(sizes))
+        # gather maps of every part's dense block into the block-ELL storage (-1: no coupling);
+        # a column cell that occupies several slots of a row (a periodic direction of one or
+        # two cells) contributes the sum of those blocks, one gather layer per repeat
+        layers = []
+        r = np.arange(Nt)
+        for pi, p in enumerate(parts):
+            m = len(p)
+            slots = {}
+            for ca in p:
+                for s in range(S):
+                    slots.setdefault((ca, int(cols[ca][s])), []).append(s)
+            depth = max(len(v) for v in slots.values())
+            while len(layers) < depth:
+                layers.append([np.full(n * n, -1, np.int64) for n in sizes[:pi]])
+            P = [np.full((m * Nt, m * Nt), -1, np.int64) for _ in range(len(layers))]
+            for a, ca in enumerate(p):
+                for b_, cb in enumerate(p):
+                    for li, s in enumerate(slots.get((ca, cb), [])):
+                        P[li][a * Nt:(a + 1) * Nt, b_ * Nt:(b_ + 1) * Nt] = (ca * S + s) * Nt * Nt + r[:, None] * Nt + r[None, :]
+            for li in range(len(layers)):
+                layers[li].append(P[li].ravel())
+        self.batch = BlockBatch(sizes, device=blocks.device)
+        flat = blocks.view(-1)
+        vals = None
+        for layer in layers:
+            idx = torch.from_numpy(np.concatenate(layer)).to(blocks.device)
+            v = flat[idx.clamp(min=0)] * (idx >= 0).to(blocks.dtype)
+            vals = v if vals is None else vals + v
+        self.LU = vals.contiguous()
+        self.ipiv = torch.zeros(self.batch.ndof, dtype=torch.int32, device=blocks.device)
+        self.info = torch.zeros(self.batch.nblk, dtype=torch.int32, device=blocks.device)
+        self.batch.factor(self.LU, self.ipiv, self.info)
+        # DOFs of the parts in part order, and for every DOF its positions there (fixed order)
+        dof = np.concatenate([(np.asarray(p)[:, None] * Nt + np.arange(Nt)[None, :]).ravel() for p in parts])
+        self.dof = torch.from_numpy(dof).to(blocks.device)
+        order = np.argsort(dof, kind="stable")
+        counts = np.bincount(dof, minlength=J * Nt)
+        width = int(counts.max()) if len(counts) else 1
+        back = np.full((J * Nt, width), -1, np.int64)
+        start = np.concatenate([[0], np.cumsum(counts)[:-1]])
+        rank = np.arange(len(dof)) - np.repeat(start, counts)
+        back[dof[order], rank] = order
+        self.back = torch.from_numpy(back).to(blocks.device)
+        self.res = torch.empty(op.ndof, dtype=torch.float64, device=blocks.device)
+
+    def check(self):
+        bad = int((self.info != 0).sum())
+        if bad:
+            raise ArithmeticException("%d singular Schwarz blocks" % bad)
+
+    def apply(self, r, z):
+        """z <- sum_parts R_p^T A_p^-1 R_p r (the additive Schwarz preconditioner)."""
+        zp = r[self.dof].contiguous()
+        self.batch.solve(self.LU, self.ipiv, zp)
+        g = zp[self.back.clamp(min=0)] * (self.back >= 0).to(zp.dtype)
+        z.copy_(g.sum(dim=1))
+
+    def Solve(self, x, b, sweeps=1):
+        import torch
+        z = torch.empty_like(x)
+        for _ in range(sweeps):
+            self.res.copy_(b)
+            self.op.spmv_dev(self.blocks, x, self.res, alpha=-1.0, beta=1.0)
+            self.apply(self.res, z)
+            x.add_(z)
+        return x
+
+
+def cartesian_parts(grid, agg):
+    """Cells of a Cartesian grid grouped into boxes of agg[d] cells per direction (the last box
+    of a direction takes the remainder)."""
+    n = grid.ncells
+    D = len(n)
+    idx = np.indices(n).reshape(D, -1).T
+    key = np.stack([idx[:, d] // agg[d] for d in range(D)], 1)
+    nb = [(n[d] + agg[d] - 1) // agg[d] for d in range(D)]
+    flat = np.ravel_multi_index(tuple(key.T), nb)
+    return [np.nonzero(flat == k)[0] for k in range(int(np.prod(nb)))]

@@ -38,10 +38,13 @@ __global__ void k_extract(int J, int Nt, int S, const int32_t* __restrict__ cols
   int j = (int)(i / Nt), r = (int)(i - (size_t)j * Nt);
   for (int s = 0; s < S; s++) {
     int c = cols[(size_t)j * S + s];
-    if (c >= 0 && color[c] == k) {
-      double* b = blocks + (((size_t)j * S + s) * Nt + r) * Nt + q;
-      *b = alpha * Y[i] + (beta != 0.0 ? beta * *b : 0.0);
-      return;   // at most one slot per colour (distance-2 colouring)
+    double* b = blocks + (((size_t)j * S + s) * Nt + r) * Nt + q;
+    if (c < 0) {
+      // no block here (a boundary face, or a neighbour an earlier slot already holds): the
+      // slot becomes beta * old, written once, at the first colour
+      if (k == 0) *b = beta != 0.0 ? beta * *b : 0.0;
+    } else if (color[c] == k) {
+      *b = alpha * Y[i] + (beta != 0.0 ? beta * *b : 0.0);   // at most one slot per colour
     }
   }
 }

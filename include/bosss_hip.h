@@ -128,7 +128,10 @@ int bosss_hip_evaluate_vjp_dev(bosss_op* op, double alpha, double beta, const do
  *      FDJacobianBuilder, DifferentialOperator.cs:1905-2880, with exact JVPs).
  *      Layout "block-ELL": blocks[j][s] is the Nt x Nt (Nt = V*N) row-major block of
  *      cell row j and column cell cols[j][s]; s = 0 is the diagonal, s = 1 + f the
- *      neighbour across local face f (-1: no block).  entries = J*(1+2D)*Nt*Nt. ---- */
+ *      neighbour across local face f (-1: no block -- a boundary face, or a neighbour an
+ *      earlier slot of the row already holds, across a periodic direction of two cells;
+ *      assembly sets such slots to beta * old, i.e. zero for beta = 0).
+ *      entries = J*(1+2D)*Nt*Nt. ---- */
 int bosss_hip_op_matrix_layout(bosss_op* op, int32_t* blocks_per_row, int64_t* entries, int32_t* ncolors, int32_t* cols);
 /* blocks <- beta*blocks + alpha*dR/dU(U)  (device pointers; the Jacobian of a linear form is its matrix) */
 int bosss_hip_op_assemble_dev(bosss_op* op, const double* U, double alpha, double beta, double* blocks, void* stream);

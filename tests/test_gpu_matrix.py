@@ -148,3 +148,29 @@ def test_spmv_tile_autotuner(form, n, p):
     assert torch.equal(y2, y) and op.spmv_tiles()["chosen"] in cand
     ref_y = ref.evaluate(KIND[form], prm, x.cpu().numpy()) if not form.startswith("euler") else ref.jvp(KIND[form], prm, U, x.cpu().numpy())
     assert np.abs(y.cpu().numpy() - ref_y).max() < 1e-12 * np.abs(ref_y).max()
+
+
+@pytest.mark.parametrize("shape,per", [((2, 3, 4), True), ((3, 3, 3), False)])
+def test_assembly_overwrites_every_slot(shape, per):
+    """Slots without a block (a neighbour that an earlier slot already holds, here across a
+    periodic direction of two cells) are written too: with beta = 0 storage that held NaN
+    becomes a valid matrix, and the apply equals the matrix-free operator."""
+    import torch
+    from bosss_amd.grid import CartesianGrid
+    from bosss_amd.operator import DifferentialOperator, default_quad_order
+    say("case assembly into NaN storage", shape, per)
+    p, vel = 2, (1.0, 0.5, 0.25)
+    g = CartesianGrid(shape, periodic=[per] * 3)
+    op = DifferentialOperator(g, "advdiff", p, nu=-0.01, velocity=vel)
+    ref = RefModel(3, shape, p, default_quad_order("advdiff", p), periodic=[int(per)] * 3)
+    B = torch.full((op.GetMatrixBuilder().entries,), float("nan"), dtype=torch.float64, device="cuda:0")
+    op.assemble_dev(torch.zeros(op.ndof, dtype=torch.float64, device="cuda:0"), B)
+    x = np.random.default_rng(6).uniform(-1, 1, op.ndof)
+    y = torch.zeros(op.ndof, dtype=torch.float64, device="cuda:0")
+    op.spmv_dev(B, torch.from_numpy(x).cuda(), y)
+    torch.cuda.synchronize()
+    assert not torch.isnan(B).any()
+    want = ref.evaluate(OP_ADVDIFF, params_vector(1.3, -0.01, vel), x)
+    err = np.abs(y.cpu().numpy() - want).max() / np.abs(want).max()
+    say("  spmv vs matrix-free: %.2e" % err)
+    assert err < 1e-12
