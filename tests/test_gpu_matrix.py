@@ -174,3 +174,38 @@ def test_assembly_overwrites_every_slot(shape, per):
     err = np.abs(y.cpu().numpy() - want).max() / np.abs(want).max()
     say("  spmv vs matrix-free: %.2e" % err)
     assert err < 1e-12
+
+
+def test_spmv_under_graph_capture_before_tuning():
+    """An apply first issued inside a captured graph does not time (the autotuner waits for an
+    uncaptured call): replays, interleaved with eager work, equal the eager apply bitwise, and
+    the later eager call tunes."""
+    import torch
+    from bosss_amd.grid import CartesianGrid
+    from bosss_amd.operator import DifferentialOperator
+    say("case spmv graph capture")
+    g = CartesianGrid((4, 4, 4), periodic=[True] * 3)
+    op = DifferentialOperator(g, "advdiff", 3, nu=-0.01, velocity=(1.0, 0.5, 0.25))
+    B = op.GetMatrixBuilder().new_matrix()
+    op.assemble_dev(torch.zeros(op.ndof, dtype=torch.float64, device="cuda:0"), B)
+    torch.cuda.synchronize()
+    x = torch.from_numpy(np.random.default_rng(7).uniform(-1, 1, op.ndof)).cuda()
+    yg = torch.zeros_like(x)
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(graph, stream=side):
+            op.spmv_dev(B, x, yg)
+    torch.cuda.current_stream().wait_stream(side)
+    assert op.spmv_tiles()["chosen"] == 0          # not tuned under capture
+    y = torch.zeros_like(x)
+    op.spmv_dev(B, x, y)                           # eager: tunes, then applies
+    torch.cuda.synchronize()
+    assert op.spmv_tiles()["chosen"] != 0
+    for rep in range(3):
+        yg.zero_()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(yg, y), rep
+        x.mul_(1.0)                                # eager work between replays
