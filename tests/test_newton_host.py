@@ -3,6 +3,7 @@
 The operator is a stand-in that evaluates the residual and its JVP/VJP with the
 CPU oracle on CPU tensors, so the driver (GMRES, defect correction, adjoint solve)
 is checked without a GPU.  The GPU runs of the same driver are in test_gpu_newton.py.
+Also the host-side partitioning of the Schwarz smoother (its GPU runs: test_gpu_schwarz.py).
 """
 import numpy as np
 import torch
@@ -75,3 +76,21 @@ def test_newton_fp64_and_defect_correction_converge():
     Um = st.solve_mixed(Un, tol=1e-10)
     assert st.stats["history"][-1] <= 1e-10
     assert torch.linalg.vector_norm(U64 - Um) <= 1e-9
+
+
+def test_cartesian_parts_cover_every_cell_once():
+    """Schwarz parts of a Cartesian grid: boxes of agg cells per direction (the last box of a
+    direction takes the remainder), every cell in exactly one part, each part a box."""
+    from bosss_amd.blocks import cartesian_parts
+    g = CartesianGrid((5, 4, 3))
+    parts = cartesian_parts(g, (2, 2, 2))
+    assert len(parts) == 3 * 2 * 2
+    allc = np.sort(np.concatenate(parts))
+    assert (allc == np.arange(g.J)).all()
+    idx = np.indices(g.ncells).reshape(3, -1).T
+    sizes = sorted(len(p) for p in parts)
+    assert sizes == sorted([2 * 2 * 2] * 4 + [2 * 2 * 1] * 4 + [1 * 2 * 2] * 2 + [1 * 2 * 1] * 2)
+    for p in parts:
+        box = idx[p]
+        ext = box.max(0) - box.min(0) + 1
+        assert np.prod(ext) == len(p)                # a full box of cells
