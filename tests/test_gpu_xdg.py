@@ -1,7 +1,8 @@
 """GPU parity of the XDG Poisson path against the oracle (bosss_ref XDG assembly).
 
-Cases: the reference's ParabolaTest setting (2D planar interface, p = 2) and a sphere in a
-3D Cartesian box (p = 2, and p = 3 for 40 x 40 cut-cell blocks).  Matrix and affine vector
+Cases: the reference's ParabolaTest setting (2D planar interface, p = 2), a sphere in a
+3D Cartesian box (p = 2, and p = 3 for 40 x 40 cut-cell blocks) and an off-centre sphere on
+a grid graded in every direction.  Matrix and affine vector
 to 1e-12 (relative to the largest entry), the block-sparse apply, and the block-Jacobi PCG
 solves: FP64 and FP32-with-refinement both to ||b - M x|| <= 1e-10 ||b|| (5e-12 for the
 parabola, whose exact-solution check needs the algebraic error far below 1e-10), with and without
@@ -33,13 +34,30 @@ def sphere_case(n, p):
     return R, (-1.0, -10.0), (1.0, 0.5), g_D
 
 
-CASES = [("parabola", 0.0, 2), ("parabola", 0.3, 2), ("sphere", 8, 2), ("sphere", 6, 3)]
+def graded_sphere_case(n, p):
+    """Off-centre sphere on a grid graded in every direction (n + d cells in direction d,
+    widths varying 2.5x): cut cells of many shapes and per-cell metrics in all directions."""
+    rng = np.random.default_rng(12)
+    nodes = []
+    for d in range(3):
+        w = rng.uniform(1.0, 2.5, n + d)
+        nodes.append(np.concatenate([[0.0], np.cumsum(w) / w.sum()]))
+    g = CartesianGrid(nodes=nodes)
+    R = XdgRules(g, QuadLevelSet.sphere([0.47, 0.52, 0.5], 0.3), n=p + 1)
+    g_D = lambda X: 0.1 * X[:, 0] - 0.2 * X[:, 1] * X[:, 2]
+    return R, (-1.0, -10.0), (1.0, 0.5), g_D
+
+
+CASES = [("parabola", 0.0, 2), ("parabola", 0.3, 2), ("sphere", 8, 2), ("sphere", 6, 3), ("sphere_graded", 6, 2)]
 
 
 def build(case, arg, p):
     if case == "parabola":
         R, mu, ex, g_D = parabola_case(arg, p=p)
         src = (1.0, 1.0)
+    elif case == "sphere_graded":
+        R, mu, src, g_D = graded_sphere_case(arg, p)
+        ex = None
     else:
         R, mu, src, g_D = sphere_case(arg, p)
         ex = None
