@@ -34,6 +34,7 @@ def dense_sweep(A, parts, Nt, x0, b):
     ("sip", (4, 4, 4), 2, False, (2, 2, 2), 1),        # grown by one cell layer: up to 270 DOF
     ("advdiff", (4, 4, 4), 3, True, (2, 2, 2), 0),     # 160-DOF parts: global-memory factorisation
     ("advdiff", (2, 4, 4), 2, True, (2, 2, 2), 0),     # the x neighbour occupies two slots of a row
+    ("advdiff", (6, 6, 6), 3, True, (3, 3, 3), 0),     # 8 parts of 540 DOF: two launches per column step
 ])
 def test_schwarz_sweep_matches_dense(form, shape, p, per, agg, overlap):
     import torch
