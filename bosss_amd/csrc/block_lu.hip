@@ -189,28 +189,17 @@ __global__ void __launch_bounds__(256) k_block_lu_big(int nblk, int nmax, const 
   if (t == 0 && info) info[b] = inf;
 }
 
-// The same factorisation with each column step split over two launches, for few large blocks
-// (one workgroup per block would leave most CUs idle): k_lu_step does the pivot search, row swap
-// and column scaling of step j (one workgroup per block), k_lu_update the rank-1 update of the
-// trailing rows (four rows per workgroup, grid over all blocks' rows).  st[b] holds the block's
-// INFO so far and st[nblk + b] whether step j updates it (0: finished, invalid, or a zero pivot).
+// The same factorisation for few large blocks (one workgroup per block would leave most CUs
+// idle), two columns per pass over the trailing matrix: k_lu_step pivots column j, applies its
+// rank-1 update to column j+1 and to row j+1 (getf2 does these at step j), pivots column j+1
+// and scales both columns (one workgroup per block); k_lu_update then applies both columns'
+// updates to every trailing element, a -= l u_j, then a -= m u_j+1, getf2's order (four rows
+// per workgroup, grid over all blocks' rows).  st[b] holds the block's INFO so far,
+// st[nblk + b] / st[2 nblk + b] whether column j / j+1 eliminates (0: done, invalid, or a zero
+// pivot).
 template <typename Real>
-__global__ void __launch_bounds__(256) k_lu_step(int j, int nblk, int nmax, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
-                                                 Real* __restrict__ A, int32_t* __restrict__ ipiv, const int64_t* __restrict__ pivoff,
-                                                 int32_t* __restrict__ st) {
-  __shared__ Real sv[4];
-  __shared__ int si[4];
-  __shared__ int s_piv;
-  const int b = blockIdx.x;
-  if (b >= nblk) return;
-  const int n = nsz[b];
+__device__ int lu_pivot_wg(const Real* a, int n, int j, Real* sv, int* si, int* s_piv) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  if (j == 0 && t == 0) st[b] = (n <= 0 || n > nmax) ? -1 : 0;
-  if (n <= 0 || n > nmax || j >= n) {
-    if (t == 0) st[nblk + b] = 0;
-    return;
-  }
-  Real* a = A + off[b];
   Real best = Real(-1);
   int bi = n;
   for (int i = j + t; i < n; i += 256) {
@@ -225,41 +214,94 @@ __global__ void __launch_bounds__(256) k_lu_step(int j, int nblk, int nmax, cons
     int bx = si[0];
     for (int k = 1; k < 4; k++)
       if (sv[k] > bv || (sv[k] == bv && si[k] < bx)) { bv = sv[k]; bx = si[k]; }
-    s_piv = bx < n ? bx : j;
+    *s_piv = bx < n ? bx : j;   // no comparable candidate (NaN column): treated as a zero pivot
   }
   __syncthreads();
-  const int p = s_piv;
-  if (t == 0 && ipiv) ipiv[pivoff[b] + j] = p + 1;
-  if (!(a[(size_t)p * n + j] != Real(0))) {
-    if (t == 0) {
-      if (st[b] == 0) st[b] = j + 1;
-      st[nblk + b] = 0;
-    }
+  return *s_piv;
+}
+
+template <typename Real>
+__device__ void lu_swap_rows(Real* a, int n, int r1, int r2) {
+  for (int c = threadIdx.x; c < n; c += 256) {
+    const Real x = a[(size_t)r1 * n + c];
+    a[(size_t)r1 * n + c] = a[(size_t)r2 * n + c];
+    a[(size_t)r2 * n + c] = x;
+  }
+}
+
+template <typename Real>
+__global__ void __launch_bounds__(256) k_lu_step(int j, int nblk, int nmax, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
+                                                 Real* __restrict__ A, int32_t* __restrict__ ipiv, const int64_t* __restrict__ pivoff,
+                                                 int32_t* __restrict__ st) {
+  __shared__ Real sv[4];
+  __shared__ int si[4];
+  __shared__ int s_piv;
+  const int b = blockIdx.x;
+  if (b >= nblk) return;
+  const int n = nsz[b];
+  const int t = threadIdx.x;
+  if (j == 0 && t == 0) st[b] = (n <= 0 || n > nmax) ? -1 : 0;
+  if (n <= 0 || n > nmax || j >= n) {
+    if (t == 0) { st[nblk + b] = 0; st[2 * nblk + b] = 0; }
     return;
   }
-  if (p != j)
-    for (int c = t; c < n; c += 256) {
-      const Real x = a[(size_t)j * n + c];
-      a[(size_t)j * n + c] = a[(size_t)p * n + c];
-      a[(size_t)p * n + c] = x;
-    }
+  Real* a = A + off[b];
+  // column j
+  const int p1 = lu_pivot_wg(a, n, j, sv, si, &s_piv);
+  if (t == 0 && ipiv) ipiv[pivoff[b] + j] = p1 + 1;
+  const bool act1 = a[(size_t)p1 * n + j] != Real(0);
+  if (act1) {
+    if (p1 != j) lu_swap_rows(a, n, j, p1);
+    __syncthreads();
+    const Real inv = Real(1) / a[(size_t)j * n + j];
+    for (int i = j + 1 + t; i < n; i += 256) a[(size_t)i * n + j] *= inv;
+  } else if (t == 0 && st[b] == 0) {
+    st[b] = j + 1;
+  }
   __syncthreads();
-  const Real inv = Real(1) / a[(size_t)j * n + j];
-  for (int i = j + 1 + t; i < n; i += 256) a[(size_t)i * n + j] *= inv;
-  if (t == 0) st[nblk + b] = 1;
+  bool act2 = false;
+  if (j + 1 < n) {
+    const int c1 = j + 1;
+    if (act1)   // step j's update of column j+1
+      for (int i = j + 1 + t; i < n; i += 256) a[(size_t)i * n + c1] -= a[(size_t)i * n + j] * a[(size_t)j * n + c1];
+    __syncthreads();
+    const int p2 = lu_pivot_wg(a, n, c1, sv, si, &s_piv);
+    if (t == 0 && ipiv) ipiv[pivoff[b] + c1] = p2 + 1;
+    act2 = a[(size_t)p2 * n + c1] != Real(0);
+    if (act2 && p2 != c1) lu_swap_rows(a, n, c1, p2);
+    __syncthreads();
+    if (act1)   // step j's update of row j+1 beyond column j+1
+      for (int c = c1 + 1 + t; c < n; c += 256) a[(size_t)c1 * n + c] -= a[(size_t)c1 * n + j] * a[(size_t)j * n + c];
+    __syncthreads();
+    if (act2) {
+      const Real inv = Real(1) / a[(size_t)c1 * n + c1];
+      for (int i = c1 + 1 + t; i < n; i += 256) a[(size_t)i * n + c1] *= inv;
+    } else if (t == 0 && st[b] == 0) {
+      st[b] = c1 + 1;
+    }
+  }
+  if (t == 0) { st[nblk + b] = act1 ? 1 : 0; st[2 * nblk + b] = act2 ? 1 : 0; }
 }
 
 template <typename Real>
 __global__ void __launch_bounds__(256) k_lu_update(int j, int nblk, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
                                                    Real* __restrict__ A, const int32_t* __restrict__ st) {
   const int b = blockIdx.y;
-  if (b >= nblk || !st[nblk + b]) return;
+  if (b >= nblk) return;
+  const bool act1 = st[nblk + b] != 0, act2 = st[2 * nblk + b] != 0;
+  if (!act1 && !act2) return;
   const int n = nsz[b];
-  const int i = j + 1 + blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int i = j + 2 + blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (i >= n) return;
   Real* a = A + off[b];
-  const Real l = a[(size_t)i * n + j];
-  for (int c = j + 1 + lane; c < n; c += 64) a[(size_t)i * n + c] -= l * a[(size_t)j * n + c];
+  const Real l = act1 ? a[(size_t)i * n + j] : Real(0);
+  const Real m = act2 ? a[(size_t)i * n + j + 1] : Real(0);
+  for (int c = j + 2 + lane; c < n; c += 64) {
+    Real x = a[(size_t)i * n + c];
+    if (act1) x -= l * a[(size_t)j * n + c];
+    if (act2) x -= m * a[(size_t)(j + 1) * n + c];
+    a[(size_t)i * n + c] = x;
+  }
 }
 
 __global__ void k_lu_info(int nblk, const int32_t* __restrict__ st, int32_t* __restrict__ info) {
@@ -369,14 +411,14 @@ int bosss_hip_block_lu_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t
     if (want_inverse) { g_lu_err = "block too large for an in-LDS inverse (n <= 100)"; return BOSSS_ERR_UNSUPPORTED; }
     hipStream_t s = (hipStream_t)stream;
     if (nmax >= LU_SPLIT_MINN && nblk < LU_SPLIT_MAXBLK) {
-      // few large blocks: two launches per column step, the trailing update over many workgroups
+      // few large blocks: two launches per pair of columns, the trailing update over many workgroups
       int32_t* st = nullptr;
-      hipError_t e = hipMalloc((void**)&st, 2 * (size_t)nblk * sizeof(int32_t));
+      hipError_t e = hipMalloc((void**)&st, 3 * (size_t)nblk * sizeof(int32_t));
       if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
-      for (int j = 0; j < nmax; j++) {
+      for (int j = 0; j < nmax; j += 2) {
         hipLaunchKernelGGL(k_lu_step<double>, dim3(nblk), dim3(256), 0, s, j, nblk, nmax, n, off, A, ipiv, pivoff, st);
-        const unsigned rows = (unsigned)((nmax - j - 1 + 3) / 4);
-        if (rows) hipLaunchKernelGGL(k_lu_update<double>, dim3(rows, nblk), dim3(256), 0, s, j, nblk, n, off, A, st);
+        const int left = nmax - j - 2;
+        if (left > 0) hipLaunchKernelGGL(k_lu_update<double>, dim3((unsigned)((left + 3) / 4), nblk), dim3(256), 0, s, j, nblk, n, off, A, st);
       }
       if (info) hipLaunchKernelGGL(k_lu_info, dim3((nblk + 255) / 256), dim3(256), 0, s, nblk, st, info);
       e = hipGetLastError();

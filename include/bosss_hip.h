@@ -155,8 +155,8 @@ int bosss_hip_op_spmv_tiles(const bosss_op* op, int32_t* chosen, int32_t* ncand,
  *      nmax <= 128 stages each block in LDS (one wavefront per block); larger nmax
  *      factors in place in global memory with the same pivoting and operation order
  *      (one 256-thread workgroup per block, or, for fewer than 256 blocks with nmax >= 512,
- *      two launches per column step with the trailing update over many workgroups; that
- *      form synchronises the stream before it returns).
+ *      two launches per pair of columns with the trailing update over many workgroups;
+ *      that form synchronises the stream before it returns).
  *      ipiv (1-based, LAPACK convention) at pivoff[b]; info[b] as DGETRF's INFO
  *      (0, or j+1 for the first zero pivot; -1 if n[b] exceeds nmax).  The inverse
  *      (want_inverse) needs nmax <= 100.  In
