@@ -48,9 +48,11 @@ __device__ int lds_getrf(Real* A, int n, int ld, int* piv) {
         if (v > best || (v == best && i < bi)) { best = v; bi = i; }
       }
     wave_argmax(best, bi);
-    if (bi >= n) bi = j;   // no comparable candidate (NaN column): treated as a zero pivot below
+    if (bi >= n) bi = j;   // no comparable candidate (all NaN): the diagonal, as idamax
     if (lane == 0) piv[j] = bi;
     __syncthreads();
+    // one wavefront: every lane reads the pivot before any lane swaps it.  NaN passes the test
+    // and leaves INFO unset, as in DGETF2 (A(JP,J).NE.ZERO).
     if (!(A[bi * ld + j] != Real(0))) {
       if (!info) info = j + 1;
       __syncthreads();
@@ -134,7 +136,7 @@ __global__ void __launch_bounds__(256) k_block_lu_big(int nblk, int nmax, const 
                                                       int32_t* __restrict__ info) {
   __shared__ Real sv[4];
   __shared__ int si[4];
-  __shared__ int s_piv;
+  __shared__ int s_piv, s_act;
   const int b = blockIdx.x;
   if (b >= nblk) return;
   const int n = nsz[b];
@@ -160,12 +162,16 @@ __global__ void __launch_bounds__(256) k_block_lu_big(int nblk, int nmax, const 
       int bx = si[0];
       for (int k = 1; k < 4; k++)
         if (sv[k] > bv || (sv[k] == bv && si[k] < bx)) { bv = sv[k]; bx = si[k]; }
-      s_piv = bx < n ? bx : j;   // no comparable candidate (NaN column): treated as a zero pivot below
+      s_piv = bx < n ? bx : j;   // no comparable candidate (all NaN): the diagonal, as idamax
+      // The pivot test is read here, once, before the barrier: after it the row swap overwrites
+      // a[p][j] (the thread owning column j may be in another wavefront).  NaN passes the test,
+      // as in DGETF2 (A(JP,J).NE.ZERO), and leaves INFO unset.
+      s_act = a[(size_t)s_piv * n + j] != Real(0);
     }
     __syncthreads();
     const int p = s_piv;
     if (t == 0 && ipiv) ipiv[pivoff[b] + j] = p + 1;
-    if (!(a[(size_t)p * n + j] != Real(0))) {
+    if (!s_act) {
       if (!inf) inf = j + 1;
       __syncthreads();
       continue;
@@ -197,8 +203,11 @@ __global__ void __launch_bounds__(256) k_block_lu_big(int nblk, int nmax, const 
 // per workgroup, grid over all blocks' rows).  st[b] holds the block's INFO so far,
 // st[nblk + b] / st[2 nblk + b] whether column j / j+1 eliminates (0: done, invalid, or a zero
 // pivot).
+// Returns the pivot row and, in act, whether the pivot is non-zero.  Thread 0 reads the pivot
+// entry before the final barrier, so every thread decides on the same value even though the
+// row swap that follows overwrites it (NaN counts as non-zero, as in DGETF2).
 template <typename Real>
-__device__ int lu_pivot_wg(const Real* a, int n, int j, Real* sv, int* si, int* s_piv) {
+__device__ int lu_pivot_wg(const Real* a, int n, int j, Real* sv, int* si, int* s_piv, int* s_act, bool& act) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   Real best = Real(-1);
   int bi = n;
@@ -214,9 +223,11 @@ __device__ int lu_pivot_wg(const Real* a, int n, int j, Real* sv, int* si, int* 
     int bx = si[0];
     for (int k = 1; k < 4; k++)
       if (sv[k] > bv || (sv[k] == bv && si[k] < bx)) { bv = sv[k]; bx = si[k]; }
-    *s_piv = bx < n ? bx : j;   // no comparable candidate (NaN column): treated as a zero pivot
+    *s_piv = bx < n ? bx : j;   // no comparable candidate (all NaN): the diagonal, as idamax
+    *s_act = a[(size_t)*s_piv * n + j] != Real(0);
   }
   __syncthreads();
+  act = *s_act != 0;
   return *s_piv;
 }
 
@@ -235,7 +246,7 @@ __global__ void __launch_bounds__(256) k_lu_step(int j, int nblk, int nmax, cons
                                                  int32_t* __restrict__ st) {
   __shared__ Real sv[4];
   __shared__ int si[4];
-  __shared__ int s_piv;
+  __shared__ int s_piv, s_act;
   const int b = blockIdx.x;
   if (b >= nblk) return;
   const int n = nsz[b];
@@ -247,9 +258,9 @@ __global__ void __launch_bounds__(256) k_lu_step(int j, int nblk, int nmax, cons
   }
   Real* a = A + off[b];
   // column j
-  const int p1 = lu_pivot_wg(a, n, j, sv, si, &s_piv);
+  bool act1;
+  const int p1 = lu_pivot_wg(a, n, j, sv, si, &s_piv, &s_act, act1);
   if (t == 0 && ipiv) ipiv[pivoff[b] + j] = p1 + 1;
-  const bool act1 = a[(size_t)p1 * n + j] != Real(0);
   if (act1) {
     if (p1 != j) lu_swap_rows(a, n, j, p1);
     __syncthreads();
@@ -265,9 +276,8 @@ __global__ void __launch_bounds__(256) k_lu_step(int j, int nblk, int nmax, cons
     if (act1)   // step j's update of column j+1
       for (int i = j + 1 + t; i < n; i += 256) a[(size_t)i * n + c1] -= a[(size_t)i * n + j] * a[(size_t)j * n + c1];
     __syncthreads();
-    const int p2 = lu_pivot_wg(a, n, c1, sv, si, &s_piv);
+    const int p2 = lu_pivot_wg(a, n, c1, sv, si, &s_piv, &s_act, act2);
     if (t == 0 && ipiv) ipiv[pivoff[b] + c1] = p2 + 1;
-    act2 = a[(size_t)p2 * n + c1] != Real(0);
     if (act2 && p2 != c1) lu_swap_rows(a, n, c1, p2);
     __syncthreads();
     if (act1)   // step j's update of row j+1 beyond column j+1

@@ -1165,7 +1165,10 @@ int bosss_hip_xdg_solve_dev(bosss_xdg* x, const double* blocks, const double* b,
   int total = 0, k = 0;
   double rel = 0.0;
   const double inner_tol = 1e-4;
-  for (k = 0; k < 40; k++) {
+  const int max_outer = 40;
+  // Every exit passes through the residual check below, which synchronises the stream: the last
+  // correction of sol is complete and relres describes the returned sol.
+  for (k = 0;; k++) {
     hipLaunchKernelGGL((k_scale_copy<double, double>), dim3(grid_for(n)), dim3(256), 0, s, n, 1.0, b, x->r64, 0);
     launch_spmv_cm<double>(x, x->At, -1.0, sol, 1.0, x->r64, s);
     launch_dot<double>(x, n, x->r64, x->r64, 0, S_RR, -1, s);
@@ -1173,7 +1176,7 @@ int bosss_hip_xdg_solve_dev(bosss_xdg* x, const double* blocks, const double* b,
     XHIP(hipStreamSynchronize(s));
     rel = bb > 0 ? std::sqrt(h[S_RR] / bb) : 0.0;
     if (!std::isfinite(rel)) return xfail(BOSSS_ERR_NONFINITE, "refinement residual is not finite");
-    if (rel <= tol || total >= maxit) break;
+    if (rel <= tol || total >= maxit || k >= max_outer) break;
     const double rn = std::sqrt(h[S_RR]);
     hipLaunchKernelGGL((k_lower_apply<double, float>), dim3(x->a.J), dim3(64), 0, s, x->a.J, x->d_nrows, x->d_doff, x->d_dof, x->Lchol, 0,
                        1.0 / rn, x->r64, x->b32, 0);

@@ -14,6 +14,13 @@ Two drivers:
   the correction solved in FP32 (FP32 JVP on MFMA, FP32 vectors).  FP64 is used
   only for the outer residual and the state update.
 
+Both globalise the step like GlobalizationOption.LineSearch (Newton.cs:807-860): Armijo
+backtracking on ||F(U + lambda d)|| with alpha = 1e-4, first reduction sigma1 = 0.5, then
+Kelley's safeguarded three-point parabola ``parab3p`` (Newton.cs:1622-1656), at most
+``maxStep`` = 30 reductions; the accepted trial residual is the next iterate's residual.
+``globalization="none"`` takes full steps (GlobalizationOption.None).  ``stats["converged"]``
+says whether ||F|| <= tol was reached (GlobalizedNewton's `success`, Newton.cs:189-241).
+
 GMRES here is restarted, right-preconditioned (optional block-diagonal
 preconditioner callable), classical Gram-Schmidt with one re-orthogonalisation,
 in the dtype of its right-hand side; Givens rotations on the host.
@@ -103,6 +110,43 @@ def gmres(apply_A, b, tol, restart=30, maxit=300, precond=None, true_residual=Tr
     return x, it, float(torch.linalg.vector_norm(r)) / bnorm
 
 
+def parab3p(lambdac, lambdam, ff0, ffc, ffm, sigma0=0.1, sigma1=0.5):
+    """Three-point safeguarded parabolic step-length model (Newton.cs:1622-1656, after
+    C. T. Kelley): the minimiser of the parabola through (0, ff0), (lambdac, ffc),
+    (lambdam, ffm), clipped to [sigma0, sigma1] * lambdac; sigma1 * lambdac if the parabola
+    has no minimum."""
+    c2 = lambdam * (ffc - ff0) - lambdac * (ffm - ff0)
+    if c2 >= 0:
+        return sigma1 * lambdac
+    c1 = lambdac * lambdac * (ffm - ff0) - lambdam * lambdam * (ffc - ff0)
+    lp = -c1 * 0.5 / c2
+    return min(max(lp, sigma0 * lambdac), sigma1 * lambdac)
+
+
+def line_search(F, U, d, nf0, Ut, rt, alpha=1e-4, sigma1=0.5, max_step=30):
+    """Armijo backtracking of LineSearch (Newton.cs:807-860): Ut <- U + lambda d with the first
+    lambda in 1, sigma1, parab3p(...) ... for which ||F(Ut)|| < (1 - alpha lambda) ||F(U)||, or
+    the last one tried after max_step reductions.  F(x, out) writes the residual; rt holds
+    F(Ut) on return.  Returns (lambda, ||F(Ut)||, number of residual evaluations)."""
+    import torch
+    lam, lamm, lamc = 1.0, 1.0, 1.0
+    torch.add(U, d, out=Ut)
+    F(Ut, rt)
+    nft = float(torch.linalg.vector_norm(rt))
+    ff0, ffc, ffm = nf0 * nf0, nft * nft, nft * nft
+    iarm, nev = 0, 1
+    while not (nft < (1.0 - alpha * lam) * nf0) and iarm < max_step:
+        lam = sigma1 * lam if iarm == 0 else parab3p(lamc, lamm, ff0, ffc, ffm)
+        torch.add(U, d, alpha=lam, out=Ut)
+        lamm, lamc = lamc, lam
+        F(Ut, rt)
+        nft = float(torch.linalg.vector_norm(rt))
+        ffm, ffc = ffc, nft * nft
+        iarm += 1
+        nev += 1
+    return lam, nft, nev
+
+
 class ImplicitStep:
     """One implicit Euler step of the DG operator, solved by Newton-Krylov on the GPU."""
 
@@ -135,52 +179,66 @@ class ImplicitStep:
             out.add_(v, alpha=inv_dt)
         return apply
 
-    def solve_fp64(self, U0, tol=1e-10, inner_tol=1e-4, max_newton=20, restart=40, maxit=400, precond=None):
-        """Newton-GMRES in FP64.  precond(v, z): optional right preconditioner, z ~ (I/dt + J)^-1 v
-        (e.g. the cell-block Jacobi inverse of the assembled diagonal blocks, frozen)."""
+    def _newton(self, U0, tol, max_newton, step, globalization):
+        """Common loop of both drivers: step(U, r, rn) -> (correction d, Krylov iterations)."""
         import torch
+        if globalization not in ("linesearch", "none"):
+            raise ValueError("globalization must be 'linesearch' or 'none'")
         U = U0.clone()
         r = torch.empty_like(U)
-        hist, n_jvp = [], 0
+        Ut, rt = torch.empty_like(U), torch.empty_like(U)
+        hist, lams, n_jvp, n_res = [], [], 0, 1
         t0 = time.perf_counter()
+        self.F(U, r)
+        rn = float(torch.linalg.vector_norm(r))
+        hist.append(rn)
         for k in range(max_newton):
-            self.F(U, r)
-            rn = float(torch.linalg.vector_norm(r))
-            hist.append(rn)
             if rn <= tol:
                 break
-            d, its, _ = gmres(self._jac(self.op64, U), -r, inner_tol, restart, maxit, precond=precond, true_residual=False)
+            d, its = step(U, r, rn)
             n_jvp += its
-            U += d
+            if globalization == "linesearch":
+                lam, rn, nev = line_search(self.F, U, d, rn, Ut, rt)
+                n_res += nev
+                U, Ut = Ut, U
+                r, rt = rt, r
+            else:
+                lam = 1.0
+                U += d
+                self.F(U, r)
+                rn = float(torch.linalg.vector_norm(r))
+                n_res += 1
+            lams.append(lam)
+            hist.append(rn)
         _sync(U)
-        self.stats = dict(history=hist, jvps=n_jvp, seconds=time.perf_counter() - t0, newton=len(hist) - 1)
+        self.stats = dict(history=hist, jvps=n_jvp, residuals=n_res, step_lengths=lams, converged=rn <= tol,
+                          seconds=time.perf_counter() - t0, newton=len(hist) - 1)
         return U
 
-    def solve_mixed(self, U0, tol=1e-10, inner_tol=1e-4, max_outer=20, restart=40, maxit=400, precond=None):
+    def solve_fp64(self, U0, tol=1e-10, inner_tol=1e-4, max_newton=20, restart=40, maxit=400, precond=None,
+                   globalization="linesearch"):
+        """Newton-GMRES in FP64.  precond(v, z): optional right preconditioner, z ~ (I/dt + J)^-1 v
+        (e.g. the cell-block Jacobi inverse of the assembled diagonal blocks, frozen).  Returns the
+        last iterate; stats["converged"] is False if max_newton steps did not reach tol."""
+        def step(U, r, rn):
+            d, its, _ = gmres(self._jac(self.op64, U), -r, inner_tol, restart, maxit, precond=precond, true_residual=False)
+            return d, its
+        return self._newton(U0, tol, max_newton, step, globalization)
+
+    def solve_mixed(self, U0, tol=1e-10, inner_tol=1e-4, max_outer=20, restart=40, maxit=400, precond=None,
+                    globalization="linesearch"):
         """FP64 defect correction around an FP32 Newton-GMRES correction.  precond(v, z) acts on
         the FP32 Krylov vectors."""
-        import torch
         if self.op32 is None:
             raise ValueError("solve_mixed needs an FP32 operator")
-        U = U0.clone()
-        r = torch.empty_like(U)
-        hist, n_jvp = [], 0
-        t0 = time.perf_counter()
-        for k in range(max_outer):
-            self.F(U, r)                                   # FP64 outer residual
-            rn = float(torch.linalg.vector_norm(r))
-            hist.append(rn)
-            if rn <= tol:
-                break
+
+        def step(U, r, rn):
             U32 = U.float()
             b32 = (-r / rn).float()                        # scaled so FP32 never underflows
             d32, its, _ = gmres(self._jac(self.op32, U32), b32, inner_tol, restart, maxit, precond=precond,
                                 true_residual=False)
-            n_jvp += its
-            U += d32.double() * rn
-        _sync(U)
-        self.stats = dict(history=hist, jvps=n_jvp, seconds=time.perf_counter() - t0, newton=len(hist) - 1)
-        return U
+            return d32.double() * rn, its
+        return self._newton(U0, tol, max_outer, step, globalization)
 
     def adjoint(self, U, g, tol=1e-10, restart=40, maxit=600):
         """Solve (I/dt + J(U))^T lam = g with the VJP kernel (FP64)."""

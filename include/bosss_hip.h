@@ -226,9 +226,14 @@ int bosss_hip_xdg_assemble_dev(bosss_xdg* xdg, double* blocks, double* affine, v
 int bosss_hip_xdg_spmv_dev(bosss_xdg* xdg, const double* blocks, double alpha, const double* x, double beta, double* y, void* stream);
 /* Solve M sol = b by block-Jacobi preconditioned CG (diagonal blocks inverted by the batched
  * uneven block LU), sol holding the initial guess, to ||b - M sol|| <= tol ||b||.
- * precision BOSSS_FP64: CG in FP64; BOSSS_FP32: CG in FP32 (matrix and preconditioner
- * rounded once) inside FP64 iterative refinement.  All scalars stay on the device; each
- * group of check_every iterations is one HIP graph launch when use_graph != 0.
+ * precision BOSSS_FP64: CG in FP64 preconditioned by the inverted diagonal blocks;
+ * BOSSS_FP32: unpreconditioned CG in FP32 on the diagonal-block equilibrated matrix
+ * L^-1 M L^-T (L the Cholesky factor of each diagonal block, so the equilibrated diagonal
+ * blocks are the identity and block-Jacobi would be a no-op; the diagonal blocks must be
+ * symmetric positive definite, else BOSSS_ERR_ARG), rounded to FP32 once, inside FP64
+ * iterative refinement (at most 40 outer steps).  All scalars stay on the device; each
+ * group of check_every iterations is one HIP graph launch when use_graph != 0.  The call
+ * returns with its work complete (sol final, relres = ||b - M sol|| / ||b|| of that sol).
  * Returns > 0 (the 1-based cell) if a diagonal block is singular. */
 int bosss_hip_xdg_solve_dev(bosss_xdg* xdg, const double* blocks, const double* b, double* sol, double tol, int32_t maxit,
                             int32_t precision, int32_t use_graph, int32_t check_every, int32_t* iterations, int32_t* outer,

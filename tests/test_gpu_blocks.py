@@ -119,21 +119,58 @@ def test_large_blocks_factor_in_global_memory(sizes):
         assert np.abs(xs[o:o + n] - ref).max() < 1e-10 * max(1, np.abs(ref).max())
 
 
+@pytest.mark.parametrize("col", [36, 37])                      # first or second column of a pair
 @pytest.mark.parametrize("sizes", [[150, 140], [600, 520]])   # one-workgroup form; two launches per column
-def test_large_block_zero_pivot_and_inverse_limit(sizes):
+def test_large_block_zero_pivot_and_inverse_limit(sizes, col):
     import torch
     from bosss_amd.blocks import BlockBatch
-    say("case large block singular", sizes)
+    say("case large block singular", sizes, "column", col)
     blocks = make_blocks(sizes, 9)
-    blocks[0][:, 37] = 0.0                  # zero column -> zero pivot at j = 37 (INFO = 38)
+    blocks[0][:, col] = 0.0                 # zero column -> zero pivot at j = col (INFO = col + 1)
     bb = BlockBatch(sizes)
     A = torch.from_numpy(np.concatenate([b.ravel() for b in blocks])).to(bb.dev)
     ipiv = torch.zeros(bb.ndof, dtype=torch.int32, device=bb.dev)
     info = torch.zeros(bb.nblk, dtype=torch.int32, device=bb.dev)
     bb.factor(A, ipiv, info)
     torch.cuda.synchronize()
-    _, pivref, inforef, _, _ = ref_block_lu(sizes, blocks)
-    assert list(info.cpu().numpy()) == [38, 0] == list(inforef)
+    LUref, pivref, inforef, _, _ = ref_block_lu(sizes, blocks)
+    assert list(info.cpu().numpy()) == [col + 1, 0] == list(inforef)
     assert (ipiv.cpu().numpy() == pivref + 1).all()
+    # getf2 skips the zero column and goes on: the factors after it match too
+    assert np.abs(A.cpu().numpy() - LUref).max() < 1e-12 * np.abs(LUref).max()
     with pytest.raises(RuntimeError):
         bb.factor(A, None, info, inverse=True)   # inverses are limited to 100 DOF
+
+
+def zero_diagonal_block(n, j, seed):
+    """A block whose diagonal entry is exactly zero when getf2 reaches column j while a row below
+    holds a non-zero pivot: columns < j are unit vectors (no pivoting, no update of rows >= j),
+    a[j][j] = 0, the rest random with a dominant diagonal."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(-1, 1, (n, n)) + n * 0.2 * np.eye(n)
+    A[:, :j] = 0.0
+    A[:j, :j] = np.eye(j)
+    A[j, j] = 0.0
+    return A
+
+
+@pytest.mark.parametrize("col", [36, 37])
+@pytest.mark.parametrize("sizes", [[150, 140], [600, 520]])   # one-workgroup form; two launches per column
+def test_large_block_zero_diagonal_swaps_in_the_pivot(sizes, col):
+    """The row swap overwrites the pivot entry: every thread of the workgroup must have taken the
+    same non-zero decision before it.  Factors and pivots equal the oracle's getf2."""
+    import torch
+    from bosss_amd.blocks import BlockBatch
+    say("case large block zero diagonal", sizes, "column", col)
+    blocks = [zero_diagonal_block(n, col, 11 + i) for i, n in enumerate(sizes)]
+    bb = BlockBatch(sizes)
+    A = torch.from_numpy(np.concatenate([b.ravel() for b in blocks])).to(bb.dev)
+    ipiv = torch.zeros(bb.ndof, dtype=torch.int32, device=bb.dev)
+    info = torch.zeros(bb.nblk, dtype=torch.int32, device=bb.dev)
+    bb.factor(A, ipiv, info)
+    torch.cuda.synchronize()
+    LUref, pivref, inforef, _, poff = ref_block_lu(sizes, blocks)
+    assert (inforef == 0).all() and (info.cpu().numpy() == 0).all()
+    assert all(pivref[poff[b] + col] != col for b in range(len(sizes)))   # a real swap at column col
+    assert (ipiv.cpu().numpy() == pivref + 1).all()
+    assert np.abs(A.cpu().numpy() - LUref).max() < 1e-12 * np.abs(LUref).max()

@@ -78,6 +78,44 @@ def test_newton_fp64_and_defect_correction_converge():
     assert torch.linalg.vector_norm(U64 - Um) <= 1e-9
 
 
+class AtanOp:
+    """Stand-in operator R(U) = atan(U) elementwise: Newton's method on atan diverges from
+    |x0| > 1.39 with full steps and converges with a line search."""
+
+    def evaluate_dev(self, U, R):
+        torch.atan(U, out=R)
+
+    def jvp_dev(self, U, dU, out):
+        out.copy_(dU / (1.0 + U * U))
+
+
+def test_parab3p_minimises_the_parabola_and_is_safeguarded():
+    from bosss_amd.newton import parab3p
+    ff = lambda lam: (lam - 0.3) ** 2 + 1.0
+    assert abs(parab3p(1.0, 2.0, ff(0.0), ff(1.0), ff(2.0)) - 0.3) < 1e-14      # inside [0.1, 0.5]
+    assert parab3p(0.5, 1.0, ff(0.0), ff(0.5), ff(1.0)) == 0.25                    # clipped to sigma1 * lambdac
+    ff2 = lambda lam: (lam - 0.01) ** 2
+    assert parab3p(1.0, 2.0, ff2(0.0), ff2(1.0), ff2(2.0)) == 0.1                  # clipped to sigma0 * lambdac
+    concave = lambda lam: -lam * lam
+    assert parab3p(1.0, 2.0, concave(0.0), concave(1.0), concave(2.0)) == 0.5      # no minimum: sigma1 * lambdac
+
+
+def test_line_search_damps_a_diverging_newton_iteration():
+    """F(U) = atan(U) (dt -> infinity): full Newton steps from U0 = 10 diverge and the driver
+    reports converged = False; the Armijo line search (Newton.cs:807-860) converges to 0."""
+    Un = torch.full((8,), 10.0, dtype=torch.float64)
+    st = ImplicitStep(AtanOp(), Un, 1e30)
+    U = st.solve_fp64(Un, tol=1e-12, max_newton=12, globalization="none")
+    assert not st.stats["converged"] and st.stats["newton"] == 12
+    assert not np.isfinite(st.stats["history"][-1]) or st.stats["history"][-1] > st.stats["history"][0]
+    U = st.solve_fp64(Un, tol=1e-12, max_newton=40)
+    s = st.stats
+    assert s["converged"] and float(U.abs().max()) < 1e-12
+    assert s["step_lengths"][0] < 1.0                    # the first step was damped
+    hist = s["history"]
+    assert all(hist[i + 1] < (1 - 1e-4 * s["step_lengths"][i]) * hist[i] for i in range(len(hist) - 1))
+
+
 def test_cartesian_parts_cover_every_cell_once():
     """Schwarz parts of a Cartesian grid: boxes of agg cells per direction (the last box of a
     direction takes the remainder), every cell in exactly one part, each part a box."""
