@@ -92,6 +92,20 @@ SIGNATURES = {
     "bosss_hip_measure_peaks": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                                ctypes.POINTER(ctypes.c_double)]),
     "bosss_hip_peaks_last_error": (ctypes.c_char_p, []),
+    "bosss_hip_buffer_create": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.POINTER(_vp)]),
+    "bosss_hip_buffer_destroy": (ctypes.c_int, [_vp, _vp]),
+    "bosss_hip_buffer_upload": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64]),
+    "bosss_hip_buffer_download": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64]),
+    "bosss_hip_op_set_check_finite": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "bosss_hip_op_assemble": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.c_double, _vp]),
+    "bosss_hip_bell_spmv": (ctypes.c_int, [_vp, _vp, ctypes.c_double, _vp, ctypes.c_double, _vp]),
+    "bosss_hip_block_lu": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32]),
+    "bosss_hip_block_solve": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "bosss_hip_block_gemv": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, ctypes.c_double, _vp]),
+    "bosss_hip_xdg_assemble": (ctypes.c_int, [_vp, _vp, _vp]),
+    "bosss_hip_xdg_spmv": (ctypes.c_int, [_vp, _vp, ctypes.c_double, _vp, ctypes.c_double, _vp]),
+    "bosss_hip_xdg_solve": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), _dp]),
 }
 
 _lib = None

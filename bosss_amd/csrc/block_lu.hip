@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/bosss_hip.h"
+#include "devbuf.hpp"
 
 namespace bosss {
 
@@ -474,6 +475,93 @@ int bosss_hip_block_gemv_dev(bosss_ctx* ctx, int nblk, const int32_t* n, const i
   hipLaunchKernelGGL(k_block_gemv<double>, dim3(nblk), dim3(64), 0, (hipStream_t)stream, nblk, n, off, Ainv, xoff, alpha, x, beta, y);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
+  return BOSSS_OK;
+}
+
+// ---- host-pointer forms: the arrays are host memory, copied to the device and back on the
+//      context's stream; the call returns with the results in the host arrays.
+static int host_batch_check(bosss_ctx* ctx, int nblk, const int32_t* n, const int64_t* off, hipStream_t* s) {
+  if (!ctx) { g_lu_err = "ctx is NULL (host-pointer calls copy on the context's stream)"; return BOSSS_ERR_ARG; }
+  int dev = 0;
+  if (bosss::ctx_device_stream(ctx, &dev, s) != BOSSS_OK) { g_lu_err = "bad context"; return BOSSS_ERR_ARG; }
+  if (int rc = select_device(ctx)) return rc;
+  if (!n || !off) { g_lu_err = "NULL argument"; return BOSSS_ERR_ARG; }
+  for (int b = 0; b < nblk; b++)
+    if (n[b] < 0 || off[b] < 0) { g_lu_err = "negative block size or offset"; return BOSSS_ERR_ARG; }
+  return BOSSS_OK;
+}
+static int host_fail(hipError_t e, const char* what) {
+  g_lu_err = std::string(what) + ": " + hipGetErrorString(e);
+  return BOSSS_ERR_HIP;
+}
+
+int bosss_hip_block_lu(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t* n, const int64_t* off, double* A, int32_t* ipiv,
+                       const int64_t* pivoff, int32_t* info, int32_t want_inverse) {
+  if (nblk <= 0) return BOSSS_OK;
+  hipStream_t s = nullptr;
+  if (int rc = host_batch_check(ctx, nblk, n, off, &s)) return rc;
+  if (!A || (ipiv && !pivoff)) { g_lu_err = "NULL argument"; return BOSSS_ERR_ARG; }
+  const int64_t na = batch_extent(nblk, off, [&](int b) { return (int64_t)n[b] * n[b]; });
+  const int64_t np = ipiv ? batch_extent(nblk, pivoff, [&](int b) { return (int64_t)n[b]; }) : 0;
+  DevArrays d(s);
+  const int32_t* dn = d.in(n, nblk);
+  const int64_t* doff = d.in(off, nblk);
+  double* dA = d.in((const double*)A, na);
+  int32_t* dpiv = ipiv ? d.in((const int32_t*)ipiv, np) : nullptr;
+  const int64_t* dpo = ipiv ? d.in(pivoff, nblk) : nullptr;
+  int32_t* dinfo = info ? d.out<int32_t>(nblk) : nullptr;
+  if (d.error() != hipSuccess) return host_fail(d.error(), "block_lu staging");
+  if (int rc = bosss_hip_block_lu_dev(ctx, nblk, nmax, dn, doff, dA, dpiv, dpo, dinfo, want_inverse, s)) return rc;
+  d.back(A, (const double*)dA, na);
+  if (ipiv) d.back(ipiv, (const int32_t*)dpiv, np);
+  if (info) d.back(info, (const int32_t*)dinfo, nblk);
+  if (hipError_t e = d.finish()) return host_fail(e, "block_lu");
+  return BOSSS_OK;
+}
+
+int bosss_hip_block_solve(bosss_ctx* ctx, int nblk, const int32_t* n, const int64_t* off, const double* LU, const int32_t* ipiv,
+                          const int64_t* pivoff, const int64_t* xoff, double* x) {
+  if (nblk <= 0) return BOSSS_OK;
+  hipStream_t s = nullptr;
+  if (int rc = host_batch_check(ctx, nblk, n, off, &s)) return rc;
+  if (!LU || !ipiv || !pivoff || !xoff || !x) { g_lu_err = "NULL argument"; return BOSSS_ERR_ARG; }
+  const int64_t na = batch_extent(nblk, off, [&](int b) { return (int64_t)n[b] * n[b]; });
+  const int64_t np = batch_extent(nblk, pivoff, [&](int b) { return (int64_t)n[b]; });
+  const int64_t nx = batch_extent(nblk, xoff, [&](int b) { return (int64_t)n[b]; });
+  DevArrays d(s);
+  const int32_t* dn = d.in(n, nblk);
+  const int64_t* doff = d.in(off, nblk);
+  const double* dLU = d.in(LU, na);
+  const int32_t* dpiv = d.in(ipiv, np);
+  const int64_t* dpo = d.in(pivoff, nblk);
+  const int64_t* dxo = d.in(xoff, nblk);
+  double* dx = d.in((const double*)x, nx);
+  if (d.error() != hipSuccess) return host_fail(d.error(), "block_solve staging");
+  if (int rc = bosss_hip_block_solve_dev(ctx, nblk, dn, doff, dLU, dpiv, dpo, dxo, dx, s)) return rc;
+  d.back(x, (const double*)dx, nx);
+  if (hipError_t e = d.finish()) return host_fail(e, "block_solve");
+  return BOSSS_OK;
+}
+
+int bosss_hip_block_gemv(bosss_ctx* ctx, int nblk, const int32_t* n, const int64_t* off, const double* Ainv, const int64_t* xoff,
+                         double alpha, const double* x, double beta, double* y) {
+  if (nblk <= 0) return BOSSS_OK;
+  hipStream_t s = nullptr;
+  if (int rc = host_batch_check(ctx, nblk, n, off, &s)) return rc;
+  if (!Ainv || !xoff || !x || !y) { g_lu_err = "NULL argument"; return BOSSS_ERR_ARG; }
+  const int64_t na = batch_extent(nblk, off, [&](int b) { return (int64_t)n[b] * n[b]; });
+  const int64_t nx = batch_extent(nblk, xoff, [&](int b) { return (int64_t)n[b]; });
+  DevArrays d(s);
+  const int32_t* dn = d.in(n, nblk);
+  const int64_t* doff = d.in(off, nblk);
+  const double* dAi = d.in(Ainv, na);
+  const int64_t* dxo = d.in(xoff, nblk);
+  const double* dx = d.in(x, nx);
+  double* dy = d.in((const double*)y, nx);
+  if (d.error() != hipSuccess) return host_fail(d.error(), "block_gemv staging");
+  if (int rc = bosss_hip_block_gemv_dev(ctx, nblk, dn, doff, dAi, dxo, alpha, dx, beta, dy, s)) return rc;
+  d.back(y, (const double*)dy, nx);
+  if (hipError_t e = d.finish()) return host_fail(e, "block_gemv");
   return BOSSS_OK;
 }
 

@@ -7,11 +7,13 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <type_traits>
 #include <vector>
 
 #include "../../include/bosss_hip.h"
+#include "devbuf.hpp"
 #include "dg_kernels.hpp"
 #include "registry.hpp"
 
@@ -86,6 +88,9 @@ struct bosss_op {
   int spmv_nl = 0;
   std::vector<int> spmv_cand;
   std::vector<float> spmv_ms;
+  // CheckForNanOrInf of every output (off by default): index of the first non-finite entry
+  bool check_finite = false;
+  unsigned long long* d_first = nullptr;
 };
 
 namespace bosss {
@@ -419,6 +424,7 @@ int bosss_hip_op_destroy(bosss_op* op) {
   if (op->hY) (void)hipFree(op->hY);
   if (op->d_color) (void)hipFree(op->d_color);
   if (op->d_cols) (void)hipFree(op->d_cols);
+  if (op->d_first) (void)hipFree(op->d_first);
   delete op;
   return BOSSS_OK;
 }
@@ -495,24 +501,74 @@ static int launch(bosss_op* op, int which, double alpha, double beta, const void
   return BOSSS_OK;
 }
 
+// CheckForNanOrInf (DifferentialOperator.cs:1602,1623,1658; there in DEBUG builds): the smallest
+// index of a NaN or Inf entry of v, or ~0 if every entry is finite.
+template <typename Real>
+__global__ void k_first_nonfinite(int64_t n, const Real* __restrict__ v, unsigned long long* first) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if (!isfinite(v[i])) atomicMin(first, (unsigned long long)i);
+}
+__global__ void k_set_u64(unsigned long long* p, unsigned long long v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// With the check enabled: scan each output of the call (nullptr entries skipped), wait for the
+// stream and report the first non-finite entry.
+static int check_outputs(bosss_op* op, hipStream_t s, const char* what, std::initializer_list<const void*> outs) {
+  if (!op->check_finite) return BOSSS_OK;
+  if (!op->d_first) HIPCHK(hipMalloc((void**)&op->d_first, sizeof(unsigned long long)));
+  int k = 0;
+  for (const void* v : outs) {
+    if (v) {
+      hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, op->d_first, ~0ull);
+      const unsigned nb = (unsigned)std::min<size_t>(4096, (op->ndof + 255) / 256);
+      if (op->precision == BOSSS_FP64)
+        hipLaunchKernelGGL(k_first_nonfinite<double>, dim3(nb), dim3(256), 0, s, (int64_t)op->ndof, (const double*)v, op->d_first);
+      else
+        hipLaunchKernelGGL(k_first_nonfinite<float>, dim3(nb), dim3(256), 0, s, (int64_t)op->ndof, (const float*)v, op->d_first);
+      HIPCHK(hipGetLastError());
+      unsigned long long first = 0;
+      HIPCHK(hipMemcpyAsync(&first, op->d_first, sizeof first, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if (first != ~0ull) {
+        const size_t nt = (size_t)op->V * op->N, i = (size_t)first;
+        return fail(BOSSS_ERR_NONFINITE, "%s: output %d entry %zu (cell %zu, variable %zu, mode %zu) is NaN or Inf", what, k, i,
+                    i / nt, (i % nt) / op->N, i % op->N);
+      }
+    }
+    k++;
+  }
+  return BOSSS_OK;
+}
+
 extern "C" {
+
+int bosss_hip_op_set_check_finite(bosss_op* op, int32_t enable) {
+  if (!op) return fail(BOSSS_ERR_ARG, "op is NULL");
+  op->check_finite = enable != 0;
+  return BOSSS_OK;
+}
 
 int bosss_hip_evaluate_dev(bosss_op* op, double alpha, double beta, const double* U, double* R, void* stream) {
   if (!op || !U || !R) return fail(BOSSS_ERR_ARG, "NULL argument");
-  return launch(op, KernelSet::RES, alpha, beta, U, nullptr, R, nullptr, (hipStream_t)stream);
+  if (int rc = launch(op, KernelSet::RES, alpha, beta, U, nullptr, R, nullptr, (hipStream_t)stream)) return rc;
+  return check_outputs(op, (hipStream_t)stream, "evaluate", {R});
 }
 int bosss_hip_jvp_dev(bosss_op* op, const double* U, const double* dU, double* JdU, void* stream) {
   if (!op || !U || !dU || !JdU) return fail(BOSSS_ERR_ARG, "NULL argument");
-  return launch(op, KernelSet::JVP, 1.0, 0.0, U, dU, nullptr, JdU, (hipStream_t)stream);
+  if (int rc = launch(op, KernelSet::JVP, 1.0, 0.0, U, dU, nullptr, JdU, (hipStream_t)stream)) return rc;
+  return check_outputs(op, (hipStream_t)stream, "jvp", {JdU});
 }
 int bosss_hip_vjp_dev(bosss_op* op, const double* U, const double* W, double* JtW, void* stream) {
   if (!op || !U || !W || !JtW) return fail(BOSSS_ERR_ARG, "NULL argument");
-  return launch(op, KernelSet::VJP, 1.0, 0.0, U, W, nullptr, JtW, (hipStream_t)stream);
+  if (int rc = launch(op, KernelSet::VJP, 1.0, 0.0, U, W, nullptr, JtW, (hipStream_t)stream)) return rc;
+  return check_outputs(op, (hipStream_t)stream, "vjp", {JtW});
 }
 int bosss_hip_evaluate_vjp_dev(bosss_op* op, double alpha, double beta, const double* U, double* R, const double* W,
                                double* JtW, void* stream) {
   if (!op || !U || !R || !W || !JtW) return fail(BOSSS_ERR_ARG, "NULL argument");
-  return launch(op, KernelSet::RES_VJP, alpha, beta, U, W, R, JtW, (hipStream_t)stream);
+  if (int rc = launch(op, KernelSet::RES_VJP, alpha, beta, U, W, R, JtW, (hipStream_t)stream)) return rc;
+  return check_outputs(op, (hipStream_t)stream, "evaluate_vjp", {R, JtW});
 }
 
 static int ensure_staging(bosss_op* op) {
@@ -559,7 +615,9 @@ int bosss_hip_evaluate(bosss_op* op, double time, double alpha, double beta, con
   HIPCHK(hipSetDevice(op->ctx->device));
   if ((rc = ensure_staging(op)) || (rc = to_device(op, op->hU, U))) return rc;
   if (beta != 0.0 && (rc = to_device(op, op->hR, R))) return rc;
-  if ((rc = launch(op, KernelSet::RES, alpha, beta, op->hU, nullptr, op->hR, nullptr, op->ctx->stream))) return rc;
+  if ((rc = launch(op, KernelSet::RES, alpha, beta, op->hU, nullptr, op->hR, nullptr, op->ctx->stream)) ||
+      (rc = check_outputs(op, op->ctx->stream, "evaluate", {op->hR})))
+    return rc;
   return to_host(op, R, op->hR);
 }
 
@@ -568,7 +626,9 @@ int bosss_hip_jvp(bosss_op* op, const double* U, const double* dU, double* JdU) 
   int rc;
   HIPCHK(hipSetDevice(op->ctx->device));
   if ((rc = ensure_staging(op)) || (rc = to_device(op, op->hU, U)) || (rc = to_device(op, op->hX, dU))) return rc;
-  if ((rc = launch(op, KernelSet::JVP, 1.0, 0.0, op->hU, op->hX, nullptr, op->hY, op->ctx->stream))) return rc;
+  if ((rc = launch(op, KernelSet::JVP, 1.0, 0.0, op->hU, op->hX, nullptr, op->hY, op->ctx->stream)) ||
+      (rc = check_outputs(op, op->ctx->stream, "jvp", {op->hY})))
+    return rc;
   return to_host(op, JdU, op->hY);
 }
 
@@ -577,7 +637,9 @@ int bosss_hip_vjp(bosss_op* op, const double* U, const double* W, double* JtW) {
   int rc;
   HIPCHK(hipSetDevice(op->ctx->device));
   if ((rc = ensure_staging(op)) || (rc = to_device(op, op->hU, U)) || (rc = to_device(op, op->hX, W))) return rc;
-  if ((rc = launch(op, KernelSet::VJP, 1.0, 0.0, op->hU, op->hX, nullptr, op->hY, op->ctx->stream))) return rc;
+  if ((rc = launch(op, KernelSet::VJP, 1.0, 0.0, op->hU, op->hX, nullptr, op->hY, op->ctx->stream)) ||
+      (rc = check_outputs(op, op->ctx->stream, "vjp", {op->hY})))
+    return rc;
   return to_host(op, JtW, op->hY);
 }
 
@@ -746,6 +808,90 @@ int bosss_hip_op_spmv_tiles(const bosss_op* op, int32_t* chosen, int32_t* ncand,
     if (cand) cand[i] = c[i];
     if (ms) ms[i] = (i < (int)op->spmv_ms.size() && op->spmv_cand == c) ? op->spmv_ms[i] : -1.0f;
   }
+  return BOSSS_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ device buffers and host-pointer matrix calls
+extern "C" {
+
+int bosss_hip_buffer_create(bosss_ctx* ctx, int64_t bytes, void** dptr) {
+  if (!ctx || !dptr) return fail(BOSSS_ERR_ARG, "NULL argument");
+  if (bytes < 0) return fail(BOSSS_ERR_ARG, "negative size %lld", (long long)bytes);
+  HIPCHK(hipSetDevice(ctx->device));
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, bytes ? (size_t)bytes : 1));
+  hipError_t e = hipMemsetAsync(p, 0, bytes ? (size_t)bytes : 1, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(p);
+    return fail(BOSSS_ERR_HIP, "buffer_create: %s", hipGetErrorString(e));
+  }
+  *dptr = p;
+  return BOSSS_OK;
+}
+
+int bosss_hip_buffer_destroy(bosss_ctx* ctx, void* dptr) {
+  if (!ctx) return fail(BOSSS_ERR_ARG, "ctx is NULL");
+  if (!dptr) return BOSSS_OK;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipFree(dptr));
+  return BOSSS_OK;
+}
+
+int bosss_hip_buffer_upload(bosss_ctx* ctx, void* dptr, const void* host, int64_t bytes) {
+  if (!ctx || !dptr || (!host && bytes)) return fail(BOSSS_ERR_ARG, "NULL argument");
+  if (bytes < 0) return fail(BOSSS_ERR_ARG, "negative size %lld", (long long)bytes);
+  if (!bytes) return BOSSS_OK;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipMemcpyAsync(dptr, host, (size_t)bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BOSSS_OK;
+}
+
+int bosss_hip_buffer_download(bosss_ctx* ctx, void* host, const void* dptr, int64_t bytes) {
+  if (!ctx || !dptr || (!host && bytes)) return fail(BOSSS_ERR_ARG, "NULL argument");
+  if (bytes < 0) return fail(BOSSS_ERR_ARG, "negative size %lld", (long long)bytes);
+  if (!bytes) return BOSSS_OK;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipMemcpyAsync(host, dptr, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BOSSS_OK;
+}
+
+int bosss_hip_op_assemble(bosss_op* op, const double* U, double alpha, double beta, double* blocks) {
+  if (!op || !U || !blocks) return fail(BOSSS_ERR_ARG, "NULL argument");
+  if (op->precision != BOSSS_FP64) return fail(BOSSS_ERR_UNSUPPORTED, "assembly is FP64 only");
+  int rc;
+  HIPCHK(hipSetDevice(op->ctx->device));
+  if ((rc = ensure_matrix_setup(op))) return rc;
+  const size_t entries = (size_t)op->grid->J * (1 + op->grid->NF) * op->V * op->N * op->V * op->N;
+  DevArrays d(op->ctx->stream);
+  const double* dU = d.in(U, op->ndof);
+  double* dB = beta != 0.0 ? d.in((const double*)blocks, entries) : d.out<double>(entries);
+  if (d.error() != hipSuccess) return fail(BOSSS_ERR_HIP, "op_assemble staging: %s", hipGetErrorString(d.error()));
+  if ((rc = bosss_hip_op_assemble_dev(op, dU, alpha, beta, dB, op->ctx->stream))) return rc;
+  d.back(blocks, (const double*)dB, entries);
+  if (hipError_t e = d.finish()) return fail(BOSSS_ERR_HIP, "op_assemble: %s", hipGetErrorString(e));
+  return BOSSS_OK;
+}
+
+int bosss_hip_bell_spmv(bosss_op* op, const double* blocks, double alpha, const double* x, double beta, double* y) {
+  if (!op || !blocks || !x || !y) return fail(BOSSS_ERR_ARG, "NULL argument");
+  int rc;
+  HIPCHK(hipSetDevice(op->ctx->device));
+  const size_t Nt = (size_t)op->V * op->N, entries = (size_t)op->grid->J * (1 + op->grid->NF) * Nt * Nt;
+  const size_t n = (size_t)op->grid->J * Nt;   // the matrix acts on FP64 vectors whatever the operator's precision
+  DevArrays d(op->ctx->stream);
+  const double* dB = d.in(blocks, entries);
+  const double* dx = d.in(x, n);
+  double* dy = beta != 0.0 ? d.in((const double*)y, n) : d.out<double>(n);
+  if (d.error() != hipSuccess) return fail(BOSSS_ERR_HIP, "bell_spmv staging: %s", hipGetErrorString(d.error()));
+  if ((rc = bosss_hip_bell_spmv_dev(op, dB, alpha, dx, beta, dy, op->ctx->stream))) return rc;
+  d.back(y, (const double*)dy, n);
+  if (hipError_t e = d.finish()) return fail(BOSSS_ERR_HIP, "bell_spmv: %s", hipGetErrorString(e));
   return BOSSS_OK;
 }
 

@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "../../include/bosss_hip.h"
+#include "devbuf.hpp"
 
 namespace bosss {
 namespace {
@@ -1195,6 +1196,51 @@ int bosss_hip_xdg_solve_dev(bosss_xdg* x, const double* blocks, const double* b,
   *relres = rel;
   XHIP(hipGetLastError());
   return BOSSS_OK;
+}
+
+// ---- host-pointer forms: host arrays copied to the device and back on the operator's stream
+int bosss_hip_xdg_assemble(bosss_xdg* x, double* blocks, double* affine) {
+  if (!x || !blocks || !affine) return xfail(BOSSS_ERR_ARG, "NULL argument");
+  XHIP(hipSetDevice(x->device));
+  DevArrays d(x->stream);
+  double* dB = d.out<double>(x->nentries);
+  double* da = d.out<double>(x->ndof);
+  if (d.error() != hipSuccess) return xfail(BOSSS_ERR_HIP, "xdg_assemble staging: %s", hipGetErrorString(d.error()));
+  if (int rc = bosss_hip_xdg_assemble_dev(x, dB, da, x->stream)) return rc;
+  d.back(blocks, (const double*)dB, x->nentries);
+  d.back(affine, (const double*)da, x->ndof);
+  if (hipError_t e = d.finish()) return xfail(BOSSS_ERR_HIP, "xdg_assemble: %s", hipGetErrorString(e));
+  return BOSSS_OK;
+}
+
+int bosss_hip_xdg_spmv(bosss_xdg* x, const double* blocks, double alpha, const double* v, double beta, double* y) {
+  if (!x || !blocks || !v || !y) return xfail(BOSSS_ERR_ARG, "NULL argument");
+  XHIP(hipSetDevice(x->device));
+  DevArrays d(x->stream);
+  const double* dB = d.in(blocks, x->nentries);
+  const double* dv = d.in(v, x->ndof);
+  double* dy = d.in((const double*)y, x->ndof);
+  if (d.error() != hipSuccess) return xfail(BOSSS_ERR_HIP, "xdg_spmv staging: %s", hipGetErrorString(d.error()));
+  if (int rc = bosss_hip_xdg_spmv_dev(x, dB, alpha, dv, beta, dy, x->stream)) return rc;
+  d.back(y, (const double*)dy, x->ndof);
+  if (hipError_t e = d.finish()) return xfail(BOSSS_ERR_HIP, "xdg_spmv: %s", hipGetErrorString(e));
+  return BOSSS_OK;
+}
+
+int bosss_hip_xdg_solve(bosss_xdg* x, const double* blocks, const double* b, double* sol, double tol, int32_t maxit, int32_t precision,
+                        int32_t* iterations, int32_t* outer, double* relres) {
+  if (!x || !blocks || !b || !sol || !iterations || !outer || !relres) return xfail(BOSSS_ERR_ARG, "NULL argument");
+  XHIP(hipSetDevice(x->device));
+  DevArrays d(x->stream);
+  const double* dB = d.in(blocks, x->nentries);
+  const double* db = d.in(b, x->ndof);
+  double* ds = d.in((const double*)sol, x->ndof);
+  if (d.error() != hipSuccess) return xfail(BOSSS_ERR_HIP, "xdg_solve staging: %s", hipGetErrorString(d.error()));
+  int rc = bosss_hip_xdg_solve_dev(x, dB, db, ds, tol, maxit, precision, 1, 16, iterations, outer, relres, x->stream);
+  if (rc < 0) return rc;
+  d.back(sol, (const double*)ds, x->ndof);
+  if (hipError_t e = d.finish()) return xfail(BOSSS_ERR_HIP, "xdg_solve: %s", hipGetErrorString(e));
+  return rc;
 }
 
 }  // extern "C"

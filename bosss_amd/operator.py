@@ -207,6 +207,12 @@ class DifferentialOperator:
         self._check_matrix_args(blocks, x, y)
         check(lib().bosss_hip_bell_spmv_dev(self.h, self._ptr(blocks), alpha, self._ptr(x), beta, self._ptr(y), self._stream(stream)))
 
+    def set_check_finite(self, enable=True):
+        """CheckForNanOrInf of every output (DifferentialOperator.cs:1602,1623,1658, there in DEBUG
+        builds): with it on, each evaluate / jvp / vjp synchronises and raises BosssError with code
+        -4 (BOSSS_ERR_NONFINITE) naming the first NaN or Inf entry."""
+        check(lib().bosss_hip_op_set_check_finite(self.h, 1 if enable else 0))
+
     def set_spmv_tile(self, nl):
         """Fix the panel size of the block-ELL apply (0: tune again at the next apply)."""
         check(lib().bosss_hip_op_set_spmv_tile(self.h, int(nl)))

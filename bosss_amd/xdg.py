@@ -464,6 +464,28 @@ class XdgRules:
         return gd
 
 
+def desc_arrays(rules, p, g_D=None):
+    """The arrays of the C descriptor ``bosss_xdg_desc`` for these cut-cell rules at degree p,
+    keyed by short names (contiguous, with the descriptor's element types; ``gd`` is None
+    without Dirichlet data)."""
+    from .refelement import modes
+    ex = np.ascontiguousarray(np.array(modes(rules.D, p), np.int32))
+    lay = rules.block_layout(len(ex))
+    g = rules.grid
+    gd = rules.boundary_values(g_D) if g_D is not None else None
+    c = np.ascontiguousarray
+    return dict(ex=ex, nspec=c(rules.nspec, np.int32), spec0=c(rules.spec0, np.int32), invJ=c(g.invJ, np.float64),
+                scal=c(g.scaling, np.float64), ls=c(rules.lenscale, np.float64), cols=c(lay["cols"], np.int32),
+                boff=c(lay["boff"], np.int64), vc=c(rules.vol_cell, np.int32), vs=c(rules.vol_spec, np.int32),
+                vo=c(rules.vol_off, np.int64), vn=c(rules.vol_nodes, np.float64), vw=c(rules.vol_w, np.float64),
+                fc=c(rules.face_cells, np.int32), fsl=c(lay["fslot"], np.int32), fsp=c(rules.face_spec, np.int32),
+                fo=c(rules.face_off, np.int64), fni=c(rules.face_nodes_in, np.float64), fno=c(rules.face_nodes_out, np.float64),
+                fw=c(rules.face_w, np.float64), fnm=c(rules.face_normal, np.float64),
+                gd=None if gd is None else c(gd, np.float64),
+                ic=c(rules.ifc_cell, np.int32), io=c(rules.ifc_off, np.int64), inn=c(rules.ifc_nodes, np.float64),
+                iw=c(rules.ifc_w, np.float64), inm=c(rules.ifc_normal, np.float64))
+
+
 class XdgOperator:
     """Two-species XDG Poisson operator on the GPU (C ABI ``bosss_hip_xdg_*``).
 
@@ -474,28 +496,15 @@ class XdgOperator:
 
     def __init__(self, rules, p, mu, source=(0.0, 0.0), penalty=2.0, g_D=None, device=0):
         import ctypes
-        from ._lib import lib, XdgDesc, BosssError
+        from ._lib import lib, XdgDesc
         from .operator import Context
-        from .refelement import modes
         self.rules, self.p = rules, p
         self.ctx = Context.get(device)
         D, J = rules.D, rules.J
-        ex = np.ascontiguousarray(np.array(modes(D, p), np.int32))
+        keep = desc_arrays(rules, p, g_D)
+        ex = keep["ex"]
         N = self.N = len(ex)
         lay = self.layout = rules.block_layout(N)
-        g = rules.grid
-        gd = rules.boundary_values(g_D) if g_D is not None else None
-        c = np.ascontiguousarray
-        keep = dict(ex=ex, nspec=c(rules.nspec, np.int32), spec0=c(rules.spec0, np.int32), invJ=c(g.invJ, np.float64),
-                    scal=c(g.scaling, np.float64), ls=c(rules.lenscale, np.float64), cols=c(lay["cols"], np.int32),
-                    boff=c(lay["boff"], np.int64), vc=c(rules.vol_cell, np.int32), vs=c(rules.vol_spec, np.int32),
-                    vo=c(rules.vol_off, np.int64), vn=c(rules.vol_nodes, np.float64), vw=c(rules.vol_w, np.float64),
-                    fc=c(rules.face_cells, np.int32), fsl=c(lay["fslot"], np.int32), fsp=c(rules.face_spec, np.int32),
-                    fo=c(rules.face_off, np.int64), fni=c(rules.face_nodes_in, np.float64), fno=c(rules.face_nodes_out, np.float64),
-                    fw=c(rules.face_w, np.float64), fnm=c(rules.face_normal, np.float64),
-                    gd=None if gd is None else c(gd, np.float64),
-                    ic=c(rules.ifc_cell, np.int32), io=c(rules.ifc_off, np.int64), inn=c(rules.ifc_nodes, np.float64),
-                    iw=c(rules.ifc_w, np.float64), inm=c(rules.ifc_normal, np.float64))
         self._keep = keep
 
         def P(a, t):

@@ -83,6 +83,16 @@ typedef struct {
 
 const char* bosss_hip_last_error(void);
 
+/* ---- device buffers (the "device-resident buffer handles" of a host without a GPU array
+ *      library: a C# host keeps the returned pointer as an IntPtr and passes it, or an offset
+ *      into it, to the *_dev entry points; BlockMsrMatrix- and DGField-sized arrays stay in HBM
+ *      across calls, e.g. for the Newton loop, Newton.cs:189-365).  Copies run on the context's
+ *      stream and return with the copy complete; create returns zero-filled memory. ---- */
+int bosss_hip_buffer_create(bosss_ctx* ctx, int64_t bytes, void** dptr);
+int bosss_hip_buffer_destroy(bosss_ctx* ctx, void* dptr);
+int bosss_hip_buffer_upload(bosss_ctx* ctx, void* dptr, const void* host, int64_t bytes);
+int bosss_hip_buffer_download(bosss_ctx* ctx, void* host, const void* dptr, int64_t bytes);
+
 /* ---- context / grid / operator (replaces DifferentialOperator.GetEvaluatorEx setup,
  *      DifferentialOperator.cs:1013-1033, and the grid metrics it reads) ---- */
 int bosss_hip_ctx_create(int device, bosss_ctx** ctx);
@@ -108,6 +118,13 @@ int bosss_hip_evaluate(bosss_op* op, double time, double alpha, double beta, con
  *      transpose of FDJacobianBuilder, DifferentialOperator.cs:1905-2880) ---- */
 int bosss_hip_jvp(bosss_op* op, const double* U, const double* dU, double* JdU);
 int bosss_hip_vjp(bosss_op* op, const double* U, const double* W, double* JtW);
+
+/* CheckForNanOrInf of the outputs (DifferentialOperator.cs:1602,1623,1658, there in DEBUG
+ * builds), off by default.  When enabled, evaluate / jvp / vjp and their _dev forms scan every
+ * output vector after the launch, wait for the stream (so the _dev forms become synchronous and
+ * cannot be captured into a graph) and return BOSSS_ERR_NONFINITE, with the first NaN or Inf
+ * entry (cell, variable, mode) in bosss_hip_last_error(), if there is one. */
+int bosss_hip_op_set_check_finite(bosss_op* op, int32_t enable);
 
 /* ---- device-resident variants: pointers are device memory, work is queued on
  *      `stream` (a hipStream_t; NULL is HIP's null stream, bosss_hip_ctx_stream()
@@ -137,6 +154,11 @@ int bosss_hip_op_matrix_layout(bosss_op* op, int32_t* blocks_per_row, int64_t* e
 int bosss_hip_op_assemble_dev(bosss_op* op, const double* U, double alpha, double beta, double* blocks, void* stream);
 /* y <- alpha*A x + beta*y with A in the operator's block-ELL layout (BlockMsrMatrix.SpMV, BlockMsrMatrix.cs:2554) */
 int bosss_hip_bell_spmv_dev(bosss_op* op, const double* blocks, double alpha, const double* x, double beta, double* y, void* stream);
+/* Host-pointer forms of the two calls above (blocks, U, x, y host arrays of entries / ndof
+ * doubles): BlockMsrMatrix-side callers that hold the matrix in host memory (AccBlock,
+ * BlockMsrMatrix.cs:1388-1420; ReadBlock :1497).  Copies per call, synchronous. */
+int bosss_hip_op_assemble(bosss_op* op, const double* U, double alpha, double beta, double* blocks);
+int bosss_hip_bell_spmv(bosss_op* op, const double* blocks, double alpha, const double* x, double beta, double* y);
 /* Tile shape of that apply: the panel size NL (64-element loads per lane and panel) of the
  * streaming kernel.  The first bosss_hip_bell_spmv_dev call on an operator times every candidate
  * on its own matrix and keeps the fastest (an autotuner; not while the stream is being
@@ -172,6 +194,18 @@ int bosss_hip_block_solve_dev(bosss_ctx* ctx, int nblk, const int32_t* n, const 
 /* y_b <- alpha Ainv_b x_b + beta y_b  (block-diagonal apply of BlockJacobi.Solve, BlockJacobi.cs:148-197) */
 int bosss_hip_block_gemv_dev(bosss_ctx* ctx, int nblk, const int32_t* n, const int64_t* off, const double* Ainv,
                              const int64_t* xoff, double alpha, const double* x, double beta, double* y, void* stream);
+/* Host-pointer forms of the three batch calls (BlockJacobi.InitImpl holds the blocks it
+ * ReadBlock's in host arrays, BlockJacobi.cs:63-120; InvertInPlace works in place on them,
+ * IMatrixExtensions.cs:2128-2230): same arguments without the stream; every array is host memory,
+ * copied to the device and back on ctx's stream (ctx is required); synchronous.  The array
+ * extents are max over b of off[b] + n[b]^2 (A, LU, Ainv), pivoff[b] + n[b] (ipiv) and
+ * xoff[b] + n[b] (x, y); entries between blocks are copied back unchanged. */
+int bosss_hip_block_lu(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t* n, const int64_t* off, double* A, int32_t* ipiv,
+                       const int64_t* pivoff, int32_t* info, int32_t want_inverse);
+int bosss_hip_block_solve(bosss_ctx* ctx, int nblk, const int32_t* n, const int64_t* off, const double* LU, const int32_t* ipiv,
+                          const int64_t* pivoff, const int64_t* xoff, double* x);
+int bosss_hip_block_gemv(bosss_ctx* ctx, int nblk, const int32_t* n, const int64_t* off, const double* Ainv, const int64_t* xoff,
+                         double alpha, const double* x, double beta, double* y);
 const char* bosss_hip_block_last_error(void);
 
 /* ---- two-species XDG Poisson (XdgPoisson3 with XLaplace_Bulk + XLaplace_Interface + RHSSource,
@@ -238,6 +272,13 @@ int bosss_hip_xdg_spmv_dev(bosss_xdg* xdg, const double* blocks, double alpha, c
 int bosss_hip_xdg_solve_dev(bosss_xdg* xdg, const double* blocks, const double* b, double* sol, double tol, int32_t maxit,
                             int32_t precision, int32_t use_graph, int32_t check_every, int32_t* iterations, int32_t* outer,
                             double* relres, void* stream);
+/* Host-pointer forms (blocks: nentries doubles, vectors: ndof doubles, host memory; copies on the
+ * operator's stream, synchronous).  bosss_hip_xdg_solve runs the solver above with use_graph = 1
+ * and check_every = 16. */
+int bosss_hip_xdg_assemble(bosss_xdg* xdg, double* blocks, double* affine);
+int bosss_hip_xdg_spmv(bosss_xdg* xdg, const double* blocks, double alpha, const double* x, double beta, double* y);
+int bosss_hip_xdg_solve(bosss_xdg* xdg, const double* blocks, const double* b, double* sol, double tol, int32_t maxit, int32_t precision,
+                        int32_t* iterations, int32_t* outer, double* relres);
 const char* bosss_hip_xdg_last_error(void);
 
 /* ---- measured arithmetic ceilings (no counterpart in the reference: the rates the roofline

@@ -90,3 +90,18 @@ def test_block_lu_refuses_sizes_before_any_launch():
     assert rc == -3 and "100" in L.bosss_hip_block_last_error().decode()
     rc = L.bosss_hip_block_lu_dev(None, 1, 20, None, ptr(off), ptr(A), None, None, ptr(info), 0, None)
     assert rc == -1
+
+
+def test_cpp_host_binds_every_declared_symbol_by_name():
+    """The plain C++ host (tests/cpp/abi_harness.cpp) lists exactly the entry points of
+    include/bosss_hip.h and resolves each one by name from the built library (no GPU needed)."""
+    import subprocess
+    src = open(os.path.join(ROOT, "tests", "cpp", "abi_harness.cpp")).read()
+    m = re.search(r"#define BOSSS_API\(X\)(.*?)\n\n", src, flags=re.S)
+    listed = sorted("bosss_hip_" + s for s in re.findall(r"X\((\w+)\)", m.group(1)))
+    assert listed == declared_symbols()
+    exe = os.path.join(ROOT, "tests", "cpp", "abi_harness")
+    if not os.path.exists(exe):
+        pytest.skip("harness not built")
+    r = subprocess.run([exe, _lib.LIBPATH, "bind"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr

@@ -219,3 +219,36 @@ def test_graded_grid_matches_oracle(form, D, n, p, per):
         pf, nw = rel_err(got.cpu().numpy(), want, op.nvars, op.N)
         say("  %s per-field %.2e normwise %.2e" % (name, pf, nw))
         assert pf < TOL and nw < TOL
+
+
+def test_check_finite_reports_the_first_nan():
+    """CheckForNanOrInf (DifferentialOperator.cs:1602,1623,1658): off by default; on, a NaN injected
+    into one coefficient of U makes the residual, JVP, VJP and fused calls raise BOSSS_ERR_NONFINITE
+    naming a non-finite output entry, and a finite state passes."""
+    import torch
+    from bosss_amd._lib import BosssError
+    from bosss_amd.grid import CartesianGrid
+    from bosss_amd.operator import DifferentialOperator
+    from bosss_amd.workloads import euler_state
+    print("case check_finite euler_hllc p=2 3^3", flush=True)
+    g = CartesianGrid((3, 3, 3), periodic=[True] * 3)
+    op = DifferentialOperator(g, "euler_hllc", 2)
+    U = torch.from_numpy(euler_state(g, op.N, seed=1)).cuda()
+    W = torch.rand_like(U)
+    R, G = torch.zeros_like(U), torch.zeros_like(U)
+    Ubad = U.clone()
+    Ubad[7 * op.nvars * op.N + 3] = float("nan")     # cell 7, variable 0, mode 3
+    op.evaluate_dev(Ubad, R)                          # unchecked: no error
+    torch.cuda.synchronize()
+    op.set_check_finite(True)
+    op.evaluate_dev(U, R)
+    op.evaluate_vjp_dev(U, R, W, G)
+    for call in (lambda: op.evaluate_dev(Ubad, R), lambda: op.jvp_dev(Ubad, W, G), lambda: op.vjp_dev(Ubad, W, G),
+                 lambda: op.evaluate_vjp_dev(Ubad, R, W, G)):
+        with pytest.raises(BosssError) as e:
+            call()
+        assert e.value.code == -4 and "NaN or Inf" in str(e.value)
+    with pytest.raises(BosssError):
+        op.evaluate(Ubad.cpu().numpy())
+    op.set_check_finite(False)
+    op.close()
