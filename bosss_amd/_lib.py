@@ -77,6 +77,8 @@ SIGNATURES = {
     "bosss_hip_op_set_spmv_tile": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "bosss_hip_op_spmv_tiles": (ctypes.c_int, [_vp, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_float), ctypes.c_int32]),
     "bosss_hip_block_lu_dev": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32, _vp]),
+    "bosss_hip_block_lu_noalloc_dev": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32,
+                                                      _vp]),
     "bosss_hip_block_solve_dev": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "bosss_hip_block_gemv_dev": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp]),
     "bosss_hip_block_last_error": (ctypes.c_char_p, []),

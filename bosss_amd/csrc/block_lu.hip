@@ -14,6 +14,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -407,6 +408,79 @@ int select_device(bosss_ctx* ctx) {
 }
 }
 
+// ---- scratch of the allocation-free factorisation (bosss_hip_block_lu_noalloc_dev) ----
+// The split form keeps three status words per block between its launches.  The no-alloc entry
+// point takes them from this file-scope array instead of a per-call allocation: every device the
+// library runs on holds its own copy (12 KiB) until the process ends, and every call on one device
+// shares it, so calls that overlap on one device race.  Every word a call reads it has written
+// itself earlier in the same call (k_lu_step writes a block's words before k_lu_update reads
+// them), so nothing needs clearing between calls.
+constexpr int LU_NOALLOC_MAXBLK = 1024;
+static __device__ __align__(256) int32_t scratch[3 * LU_NOALLOC_MAXBLK];
+
+namespace {
+constexpr int LU_NOALLOC_MAXDEV = 64;
+std::mutex g_scratch_mu;
+int32_t* g_scratch_ptr[LU_NOALLOC_MAXDEV] = {};   // device address of each device's copy of scratch
+
+// The address of the current device's copy of scratch, looked up once per device (a host-side
+// name of a __device__ array is not a device address; hipGetSymbolAddress gives that device's).
+int scratch_on_current_device(int32_t** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) { g_lu_err = std::string("hipGetDevice: ") + hipGetErrorString(e); return BOSSS_ERR_HIP; }
+  if (dev < 0 || dev >= LU_NOALLOC_MAXDEV) { g_lu_err = "device index out of range for the scratch table"; return BOSSS_ERR_ARG; }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      g_lu_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return BOSSS_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<int32_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return BOSSS_OK;
+}
+}  // namespace
+
+// Argument checks shared by the factorisation entry points (selects ctx's device).
+static int lu_check_args(bosss_ctx* ctx, int32_t nmax, const int32_t* n, const int64_t* off, const double* A, const int32_t* ipiv,
+                         const int64_t* pivoff) {
+  if (int rc = select_device(ctx)) return rc;
+  if (!n || !off || !A || (ipiv && !pivoff)) { g_lu_err = "NULL argument"; return BOSSS_ERR_ARG; }
+  if (nmax <= 0 || nmax > LU_BIG_MAXN) { g_lu_err = "block size out of range (1..4096)"; return BOSSS_ERR_ARG; }
+  return BOSSS_OK;
+}
+
+// Blocks up to LU_MAXN: one wavefront per block, the block staged in LDS.
+static int lu_launch_lds(int nblk, int nmax, const int32_t* n, const int64_t* off, double* A, int32_t* ipiv, const int64_t* pivoff,
+                         int32_t* info, int want_inverse, hipStream_t s) {
+  size_t lds = lu_lds_bytes(nmax, want_inverse);
+  if (lds > 160 * 1024) { g_lu_err = "block too large for an in-LDS inverse (n <= 100)"; return BOSSS_ERR_UNSUPPORTED; }
+  if (hipFuncSetAttribute((const void*)k_block_lu<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
+    g_lu_err = "hipFuncSetAttribute failed"; return BOSSS_ERR_HIP;
+  }
+  hipLaunchKernelGGL(k_block_lu<double>, dim3(nblk), dim3(64), lds, s, nblk, nmax, n, off, A, ipiv, pivoff, info, want_inverse);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
+  return BOSSS_OK;
+}
+
+// The split form (two launches per pair of columns) with its per-block status words st[3 nblk]
+// in device memory the caller provides; queued on s, not synchronised.
+static hipError_t lu_launch_split(int nblk, int nmax, const int32_t* n, const int64_t* off, double* A, int32_t* ipiv,
+                                  const int64_t* pivoff, int32_t* info, int32_t* st, hipStream_t s) {
+  for (int j = 0; j < nmax; j += 2) {
+    hipLaunchKernelGGL(k_lu_step<double>, dim3(nblk), dim3(256), 0, s, j, nblk, nmax, n, off, A, ipiv, pivoff, st);
+    const int left = nmax - j - 2;
+    if (left > 0) hipLaunchKernelGGL(k_lu_update<double>, dim3((unsigned)((left + 3) / 4), nblk), dim3(256), 0, s, j, nblk, n, off, A, st);
+  }
+  if (info) hipLaunchKernelGGL(k_lu_info, dim3((nblk + 255) / 256), dim3(256), 0, s, nblk, st, info);
+  return hipGetLastError();
+}
+
 extern "C" {
 
 // Host-side error text of the block routines (the context-level message lives in bosss_hip.hip).
@@ -415,42 +489,41 @@ const char* bosss_hip_block_last_error(void) { return g_lu_err.c_str(); }
 int bosss_hip_block_lu_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t* n, const int64_t* off, double* A,
                            int32_t* ipiv, const int64_t* pivoff, int32_t* info, int32_t want_inverse, void* stream) {
   if (nblk <= 0) return BOSSS_OK;
-  if (int rc = select_device(ctx)) return rc;
-  if (!n || !off || !A || (ipiv && !pivoff)) { g_lu_err = "NULL argument"; return BOSSS_ERR_ARG; }
-  if (nmax <= 0 || nmax > LU_BIG_MAXN) { g_lu_err = "block size out of range (1..4096)"; return BOSSS_ERR_ARG; }
-  if (nmax > LU_MAXN) {
-    if (want_inverse) { g_lu_err = "block too large for an in-LDS inverse (n <= 100)"; return BOSSS_ERR_UNSUPPORTED; }
-    hipStream_t s = (hipStream_t)stream;
-    if (nmax >= LU_SPLIT_MINN && nblk < LU_SPLIT_MAXBLK) {
-      // few large blocks: two launches per pair of columns, the trailing update over many workgroups
-      int32_t* st = nullptr;
-      hipError_t e = hipMalloc((void**)&st, 3 * (size_t)nblk * sizeof(int32_t));
-      if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
-      for (int j = 0; j < nmax; j += 2) {
-        hipLaunchKernelGGL(k_lu_step<double>, dim3(nblk), dim3(256), 0, s, j, nblk, nmax, n, off, A, ipiv, pivoff, st);
-        const int left = nmax - j - 2;
-        if (left > 0) hipLaunchKernelGGL(k_lu_update<double>, dim3((unsigned)((left + 3) / 4), nblk), dim3(256), 0, s, j, nblk, n, off, A, st);
-      }
-      if (info) hipLaunchKernelGGL(k_lu_info, dim3((nblk + 255) / 256), dim3(256), 0, s, nblk, st, info);
-      e = hipGetLastError();
-      if (e == hipSuccess) e = hipStreamSynchronize(s);   // st is freed next
-      (void)hipFree(st);
-      if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
-      return BOSSS_OK;
-    }
-    hipLaunchKernelGGL(k_block_lu_big<double>, dim3(nblk), dim3(256), 0, s, nblk, nmax, n, off, A, ipiv, pivoff, info);
-    hipError_t eb = hipGetLastError();
-    if (eb != hipSuccess) { g_lu_err = hipGetErrorString(eb); return BOSSS_ERR_HIP; }
+  if (int rc = lu_check_args(ctx, nmax, n, off, A, ipiv, pivoff)) return rc;
+  if (nmax <= LU_MAXN) return lu_launch_lds(nblk, nmax, n, off, A, ipiv, pivoff, info, want_inverse, (hipStream_t)stream);
+  if (want_inverse) { g_lu_err = "block too large for an in-LDS inverse (n <= 100)"; return BOSSS_ERR_UNSUPPORTED; }
+  hipStream_t s = (hipStream_t)stream;
+  if (nmax >= LU_SPLIT_MINN && nblk < LU_SPLIT_MAXBLK) {
+    // few large blocks: two launches per pair of columns, the trailing update over many workgroups
+    int32_t* st = nullptr;
+    hipError_t e = hipMalloc((void**)&st, 3 * (size_t)nblk * sizeof(int32_t));
+    if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
+    e = lu_launch_split(nblk, nmax, n, off, A, ipiv, pivoff, info, st, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);   // st is freed next
+    (void)hipFree(st);
+    if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
     return BOSSS_OK;
   }
-  size_t lds = lu_lds_bytes(nmax, want_inverse);
-  if (lds > 160 * 1024) { g_lu_err = "block too large for an in-LDS inverse (n <= 100)"; return BOSSS_ERR_UNSUPPORTED; }
-  if (hipFuncSetAttribute((const void*)k_block_lu<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
-    g_lu_err = "hipFuncSetAttribute failed"; return BOSSS_ERR_HIP;
+  hipLaunchKernelGGL(k_block_lu_big<double>, dim3(nblk), dim3(256), 0, s, nblk, nmax, n, off, A, ipiv, pivoff, info);
+  hipError_t eb = hipGetLastError();
+  if (eb != hipSuccess) { g_lu_err = hipGetErrorString(eb); return BOSSS_ERR_HIP; }
+  return BOSSS_OK;
+}
+
+int bosss_hip_block_lu_noalloc_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t* n, const int64_t* off, double* A,
+                                   int32_t* ipiv, const int64_t* pivoff, int32_t* info, int32_t want_inverse, void* stream) {
+  if (nblk <= 0) return BOSSS_OK;
+  if (int rc = lu_check_args(ctx, nmax, n, off, A, ipiv, pivoff)) return rc;
+  if (nmax <= LU_MAXN) return lu_launch_lds(nblk, nmax, n, off, A, ipiv, pivoff, info, want_inverse, (hipStream_t)stream);
+  if (want_inverse) { g_lu_err = "block too large for an in-LDS inverse (n <= 100)"; return BOSSS_ERR_UNSUPPORTED; }
+  if (nblk > LU_NOALLOC_MAXBLK) {
+    g_lu_err = "bosss_hip_block_lu_noalloc_dev: " + std::to_string(nblk) + " blocks above " + std::to_string(LU_MAXN) +
+               " DOF exceed its scratch (at most " + std::to_string(LU_NOALLOC_MAXBLK) + " per call)";
+    return BOSSS_ERR_ARG;
   }
-  hipLaunchKernelGGL(k_block_lu<double>, dim3(nblk), dim3(64), lds, (hipStream_t)stream, nblk, nmax, n, off, A, ipiv, pivoff, info,
-                     want_inverse);
-  hipError_t e = hipGetLastError();
+  int32_t* st = nullptr;
+  if (int rc = scratch_on_current_device(&st)) return rc;
+  hipError_t e = lu_launch_split(nblk, nmax, n, off, A, ipiv, pivoff, info, st, (hipStream_t)stream);
   if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
   return BOSSS_OK;
 }

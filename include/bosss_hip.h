@@ -188,6 +188,16 @@ int bosss_hip_op_spmv_tiles(const bosss_op* op, int32_t* chosen, int32_t* ncand,
  *      All arrays are device memory; one launch covers the whole batch. ---- */
 int bosss_hip_block_lu_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t* n, const int64_t* off, double* A,
                            int32_t* ipiv, const int64_t* pivoff, int32_t* info, int32_t want_inverse, void* stream);
+/* The same factorisation without allocating: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight solver loops, hosts that
+ * forbid hipMalloc).  Blocks above 128 DOF always use the split form (two launches per pair of
+ * columns, same pivots and factors as above) with its per-block status words in a fixed device
+ * array of the library (12 KiB per device, held until the process ends), so at most 1024 such
+ * blocks per call (BOSSS_ERR_ARG beyond); up to 128 DOF it is the LDS kernel above, which needs
+ * no scratch.  The call only queues work on `stream` and never synchronises.  Every call on one
+ * device shares that array: calls that overlap on one device race. */
+int bosss_hip_block_lu_noalloc_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t* n, const int64_t* off, double* A,
+                                   int32_t* ipiv, const int64_t* pivoff, int32_t* info, int32_t want_inverse, void* stream);
 /* x_b <- (LU_b)^-1 x_b  (DGETRS), x_b at element offset xoff[b] */
 int bosss_hip_block_solve_dev(bosss_ctx* ctx, int nblk, const int32_t* n, const int64_t* off, const double* LU,
                               const int32_t* ipiv, const int64_t* pivoff, const int64_t* xoff, double* x, void* stream);
