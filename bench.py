@@ -208,6 +208,9 @@ def secondary_workload(args):
             t = time.perf_counter() - t0
             res["pcg_%s_%s" % (prec, "graph" if graph else "eager")] = {
                 "seconds": round(t, 4), "iterations": it, "outer": outer, "relres": rel, "us_per_iteration": round(t / max(it, 1) * 1e6, 2)}
+        for prec in ("fp64", "fp32"):
+            chosen, ms = op.tiles(prec)
+            res["tile_autotuner_%s" % prec] = {"chosen_groups_of_4_rows": chosen, "ms_per_apply_update": ms}
         dof, desc = op.ndof, ("XDG Poisson, sphere r=0.3 in [0,1]^3, 32^3 cells, p=2, mu_A=-1, mu_B=-10, penalty 2, "
                               "agglomeration threshold 0: assembly, block-sparse apply, block-Jacobi PCG")
         op.close()
@@ -303,29 +306,72 @@ def fp32_comparison(grid, form, p, Ut, Wt, R64, G64, steps, warmup):
 
 def newton_comparison(grid, op64, Un, form, p):
     """One implicit Euler step solved to ||F|| <= 1e-10: FP64 Newton-GMRES vs FP64 defect
-    correction around FP32 JVPs (same driver, bosss_amd.newton)."""
+    correction around FP32 JVPs (same driver, bosss_amd.newton), both with the device-resident
+    GMRES (Arnoldi steps captured as HIP graphs); and the FP64 solve once more with the Givens
+    rotations on the host (one synchronisation per Krylov step) for comparison."""
     import torch
     from bosss_amd.newton import ImplicitStep
     from bosss_amd.operator import DifferentialOperator
     op32 = DifferentialOperator(grid, form, p, gamma=1.4, precision="fp32")
     dt = 0.5 / grid.ncells[0] / (2 * p + 1)
     st = ImplicitStep(op64, Un, dt, op32=op32)
-    st.solve_fp64(Un, tol=1e-10)        # warm-up of both drivers (allocations, first launches)
-    st.solve_mixed(Un, tol=1e-10)
+    sth = ImplicitStep(op64, Un, dt, krylov="host")
     res = {}
-    for name, fn in (("fp64", st.solve_fp64), ("mixed", st.solve_mixed)):
+    for name, fn in (("fp64", st.solve_fp64), ("mixed", st.solve_mixed), ("fp64_host_krylov", sth.solve_fp64)):
+        fn(Un, tol=1e-10)                # warm-up (allocations, first launches, graph capture)
         best = None
         for _ in range(2):               # best of two solves
             torch.cuda.synchronize()
             U = fn(Un, tol=1e-10)
-            s = dict(st.stats)
+            s = dict((st if fn != sth.solve_fp64 else sth).stats)
             if best is None or s["seconds"] < best["seconds"]:
                 best = s
             del U
         res[name] = {"seconds": round(best["seconds"], 4), "newton_steps": best["newton"], "jvps": best["jvps"],
-                     "final_residual": best["history"][-1]}
+                     "final_residual": best["history"][-1],
+                     "ms_per_krylov_step": round(best["seconds"] / max(best["jvps"], 1) * 1e3, 4)}
     res["dt"] = dt
+    res["krylov"] = "device-resident GMRES, Arnoldi steps as HIP graphs, residual estimate read every 8 steps"
     res["speedup_mixed_vs_fp64"] = round(res["fp64"]["seconds"] / res["mixed"]["seconds"], 3)
+    return res
+
+
+def implicit_cfl_comparison(grid, op64, Un, form, p, mult=10.0):
+    """The same implicit step at mult times the time step (a CFL number mult times larger), where the
+    Krylov iterations grow: plain FP64 Newton-GMRES against the cell-block preconditioned variants
+    (blocks of I/dt + dR_j/dU_j from own-cell JVP probes, inverted in FP64): FP64 with the FP64
+    preconditioner, and the FP64 defect correction around FP32 GMRES with the FP32 preconditioner
+    (FP32 probes and apply).  One solve each after a warm-up; the preconditioner set-up (probes,
+    batched LU with inverse, transposition) is timed separately."""
+    import torch
+    from bosss_amd.blocks import CellBlockPreconditioner
+    from bosss_amd.newton import ImplicitStep
+    from bosss_amd.operator import DifferentialOperator
+    op32 = DifferentialOperator(grid, form, p, gamma=1.4, precision="fp32")
+    dt = mult * 0.5 / grid.ncells[0] / (2 * p + 1)
+    st = ImplicitStep(op64, Un, dt, op32=op32)
+    res = {"dt": dt, "dt_multiple_of_bench_step": mult}
+    for name in ("fp64_plain", "fp64_cellblock_fp64", "mixed_cellblock_fp32"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if name == "fp64_plain":
+            pc = None
+            fn = lambda: st.solve_fp64(Un, tol=1e-10, maxit=2000)
+        elif name == "fp64_cellblock_fp64":
+            pc = CellBlockPreconditioner(op64, Un, 1.0 / dt, precision="fp64")
+            fn = lambda: st.solve_fp64(Un, tol=1e-10, precond=pc)
+        else:
+            pc = CellBlockPreconditioner(op32, Un.float(), 1.0 / dt, precision="fp32")
+            fn = lambda: st.solve_mixed(Un, tol=1e-10, precond=pc)
+        setup = time.perf_counter() - t0
+        fn()                                   # warm-up (graph capture)
+        U = fn()
+        s = dict(st.stats)
+        res[name] = {"seconds": round(s["seconds"], 4), "setup_seconds": round(setup, 4), "newton_steps": s["newton"],
+                     "jvps": s["jvps"], "final_residual": s["history"][-1], "converged": s["converged"]}
+        del U, pc
+        torch.cuda.empty_cache()
+    op32.close()
     return res
 
 
@@ -450,6 +496,7 @@ def main():
         out["fp32_fused"] = f32
     if form.startswith("euler") and not args.no_newton:
         out["newton_implicit_step"] = newton_comparison(grid, op, Ut, form, p)
+        out["newton_implicit_step_cfl10"] = implicit_cfl_comparison(grid, op, Ut, form, p)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(form, p)
     print(json.dumps(out), flush=True)

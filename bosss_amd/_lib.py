@@ -73,6 +73,7 @@ SIGNATURES = {
     "bosss_hip_op_matrix_layout": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
                                                   ctypes.POINTER(ctypes.c_int32), _vp]),
     "bosss_hip_op_assemble_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.c_double, _vp, _vp]),
+    "bosss_hip_op_diag_blocks_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.c_double, _vp, _vp]),
     "bosss_hip_bell_spmv_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp]),
     "bosss_hip_op_set_spmv_tile": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "bosss_hip_op_spmv_tiles": (ctypes.c_int, [_vp, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_float), ctypes.c_int32]),
@@ -90,10 +91,20 @@ SIGNATURES = {
     "bosss_hip_xdg_solve_dev": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                                _dp, _vp]),
+    "bosss_hip_xdg_set_tile": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
+    "bosss_hip_xdg_tiles": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_float)]),
     "bosss_hip_xdg_last_error": (ctypes.c_char_p, []),
     "bosss_hip_measure_peaks": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                                ctypes.POINTER(ctypes.c_double)]),
     "bosss_hip_peaks_last_error": (ctypes.c_char_p, []),
+    "bosss_hip_gmres_workspace": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
+    "bosss_hip_gmres_arnoldi_dev": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp,
+                                                   ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "bosss_hip_gmres_update_dev": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp,
+                                                  ctypes.c_int64, _vp, _vp, _vp, _vp, _vp]),
+    "bosss_hip_block_gemv_uniform_dev": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int32, ctypes.c_int32, _vp, ctypes.c_double, _vp,
+                                                        ctypes.c_double, _vp, _vp]),
+    "bosss_hip_krylov_last_error": (ctypes.c_char_p, []),
     "bosss_hip_buffer_create": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.POINTER(_vp)]),
     "bosss_hip_buffer_destroy": (ctypes.c_int, [_vp, _vp]),
     "bosss_hip_buffer_upload": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64]),

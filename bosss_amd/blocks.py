@@ -82,6 +82,49 @@ class BlockBatch:
             raise RuntimeError("block gemv failed (%d): %s" % (rc, lib().bosss_hip_block_last_error().decode()))
 
 
+class CellBlockPreconditioner:
+    """Right preconditioner z = D^-1 v with D the diagonal cell blocks of shift I + dR/dU(U): the
+    block-Jacobi of a Newton-Krylov step (BlockJacobi.InitImpl / Solve, BlockJacobi.cs:63-197, on the
+    diagonal blocks of the linearised operator), without assembling the off-diagonal blocks.
+
+    The blocks come from own-cell JVP probes of ``op`` (``bosss_hip_op_diag_blocks_dev``; an FP32
+    operator probes in FP32), are inverted in FP64 by the batched LU (pivots and inverse in FP64),
+    and are stored transposed in the apply precision ("fp32": rounded once to FP32), so that the
+    apply (``bosss_hip_block_gemv_uniform_dev``) reads each block column by column, coalesced."""
+
+    def __init__(self, op, U, shift, precision="fp32"):
+        import time
+        import torch
+        t0 = time.perf_counter()
+        J, Nt = op.grid.J, op.nvars * op.N
+        self.op, self.J, self.Nt = op, J, Nt
+        self.dtype = torch.float32 if precision == "fp32" else torch.float64
+        B = torch.empty(J * Nt * Nt, dtype=torch.float64, device=U.device)
+        op.diag_blocks_dev(U, B, alpha=1.0, shift=shift)
+        batch = BlockBatch([Nt] * J, device=U.device)
+        info = torch.zeros(J, dtype=torch.int32, device=U.device)
+        batch.factor(B, None, info, inverse=True)
+        bad = int((info != 0).sum())
+        if bad:
+            raise ArithmeticException("%d singular diagonal blocks" % bad)
+        self.invT = B.view(J, Nt, Nt).transpose(1, 2).to(self.dtype).contiguous().view(-1)
+        del B
+        torch.cuda.synchronize(U.device)
+        self.setup_seconds = time.perf_counter() - t0
+
+    def __call__(self, v, z):
+        import torch
+        d = self.op.ctx.device
+        device_array(v, self.dtype, self.J * self.Nt, d, "vector")
+        device_array(z, self.dtype, self.J * self.Nt, d, "vector")
+        prec = 0 if self.dtype == torch.float64 else 1
+        rc = lib().bosss_hip_block_gemv_uniform_dev(self.op.ctx.h, self.J, self.Nt, prec, ctypes.c_void_p(self.invT.data_ptr()), 1.0,
+                                                    ctypes.c_void_p(v.data_ptr()), 0.0, ctypes.c_void_p(z.data_ptr()),
+                                                    ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        if rc != 0:
+            raise RuntimeError("block gemv failed (%d): %s" % (rc, lib().bosss_hip_krylov_last_error().decode()))
+
+
 class BlockJacobi:
     """Block-Jacobi smoother on an operator matrix in block-ELL layout (host mirror of BlockJacobi).
 

@@ -47,6 +47,7 @@ __device__ int lds_getrf(Real* A, int n, int ld, int* piv) {
     for (int i = lane; i < n; i += 64)
       if (i >= j) {
         Real v = fabs(A[i * ld + j]);
+        if (i == j && v != v) v = Real(INFINITY);   // idamax: a NaN first element is never beaten
         if (v > best || (v == best && i < bi)) { best = v; bi = i; }
       }
     wave_argmax(best, bi);
@@ -120,6 +121,127 @@ __global__ void __launch_bounds__(64) k_block_lu(int nblk, int nmax, const int32
   if (lane == 0 && info) info[b] = inf;
 }
 
+// Small blocks (n <= 20: the 10- and 20-DOF cell blocks of XDG and p <= 2 scalar operators): G lanes
+// per block (G = 16 or 32, so 64 / G blocks per wavefront), lane r of a group holds row r of its
+// block in registers.  DGETF2 with the operation order of lds_getrf -- pivot = first row with the
+// largest |a| in the column (group argmax over the lanes' rows, ties to the smaller row; as in
+// IDAMAX a NaN on the diagonal is kept and any other NaN is never chosen), row swap (register exchange through lane shuffles), column scaled by the reciprocal
+// pivot, rank-1 update a_rc -= l_r u_jc with u_jc broadcast from the pivot row's lane -- so the
+// factors and pivots are those of the LDS kernel.  The inverse (want_inverse) solves L U X = P for
+// all columns at once with the rows of X in registers: the same result as DGETRI up to rounding.
+// No lane returns early: every shuffle has all lanes of the wavefront taking part, and column
+// loops that no block of the launch needs (c >= nmax) are skipped uniformly.
+template <int G, typename Real>
+__device__ __forceinline__ Real grp_shfl(Real v, int src_lane) {
+  return __shfl(v, src_lane, 64);
+}
+
+template <typename Real, int G, int NM>
+__global__ void __launch_bounds__(256) k_block_lu_rows(int nblk, int nmax, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
+                                                       Real* __restrict__ A, int32_t* __restrict__ ipiv, const int64_t* __restrict__ pivoff,
+                                                       int32_t* __restrict__ info, int want_inverse) {
+  const int r = threadIdx.x % G;                       // row of the block this lane holds
+  const int b = blockIdx.x * (256 / G) + threadIdx.x / G;
+  const int base = (threadIdx.x & 63) - r;             // first lane of the group in the wavefront
+  const bool inb = b < nblk;
+  const int n0 = inb ? nsz[b] : 0;
+  if (inb && (n0 <= 0 || n0 > nmax) && r == 0 && info) info[b] = -1;   // size outside the launch: report
+  const bool ok = inb && n0 > 0 && n0 <= nmax;
+  const int n = ok ? n0 : 0;
+  Real* g = ok ? A + off[b] : A;
+  Real row[NM];
+#pragma unroll
+  for (int c = 0; c < NM; c++) row[c] = (r < n && c < n) ? g[(size_t)r * n + c] : Real(0);
+  int inf = 0, mypiv = r;
+  int pv[NM];                                          // the group's pivot rows (same in every lane)
+#pragma unroll
+  for (int j = 0; j < NM; j++) {
+    pv[j] = j;
+    const bool act = j < n;
+    Real v = (act && r >= j && r < n) ? fabs(row[j]) : Real(-1);
+    if (act && r == j && v != v) v = Real(INFINITY);   // idamax: a NaN first element is never beaten
+    int bi = G;
+    if (v >= Real(0)) bi = r; else v = Real(-1);       // any other NaN is never a candidate
+#pragma unroll
+    for (int o = G / 2; o >= 1; o >>= 1) {
+      const Real ov = __shfl_xor(v, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ov > v || (ov == v && oi < bi)) { v = ov; bi = oi; }
+    }
+    const int p = bi < G ? bi : j;                     // no comparable candidate (all NaN): the diagonal
+    pv[j] = p;
+    if (r == j) mypiv = p;
+    const Real apj = grp_shfl<G>(row[j], base + p);
+    const bool elim = act && (apj != Real(0));          // NaN passes, as DGETF2's A(JP,J).NE.ZERO
+    if (act && !elim && !inf) inf = j + 1;
+    const int src = (elim && r == j) ? p : ((elim && r == p) ? j : r);
+    if (__any(src != r)) {
+#pragma unroll
+      for (int c = 0; c < NM; c++) row[c] = grp_shfl<G>(row[c], base + src);
+    }
+    const Real inv = Real(1) / grp_shfl<G>(row[j], base + j);
+    const bool mine = elim && r > j && r < n;
+    const Real l = row[j] * inv;
+    if (mine) row[j] = l;
+#pragma unroll
+    for (int c = j + 1; c < NM; c++) {
+      const Real u = grp_shfl<G>(row[c], base + j);
+      if (mine) row[c] -= l * u;
+    }
+  }
+  if (want_inverse && __any(ok)) {
+    // X = (LU)^-1 P: rows of the identity permuted by the pivots, then L and U solved for all columns.
+    Real x[NM];
+#pragma unroll
+    for (int c = 0; c < NM; c++) x[c] = (r == c) ? Real(1) : Real(0);
+#pragma unroll
+    for (int j = 0; j < NM; j++) {
+      const int p = pv[j];
+      const int src = (j < n && r == j) ? p : ((j < n && r == p) ? j : r);
+      if (__any(src != r)) {
+#pragma unroll
+        for (int c = 0; c < NM; c++) x[c] = grp_shfl<G>(x[c], base + src);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NM; j++) {                     // forward: unit lower L
+      const bool mine = r > j && r < n;
+      const Real lrj = row[j];
+#pragma unroll
+      for (int c = 0; c < NM; c++) {
+        const Real xj = grp_shfl<G>(x[c], base + j);
+        if (mine) x[c] -= lrj * xj;
+      }
+    }
+#pragma unroll
+    for (int j = NM - 1; j >= 0; j--) {                // backward: U
+      const Real ujj = grp_shfl<G>(row[j], base + j);
+      if (r == j && r < n) {
+#pragma unroll
+        for (int c = 0; c < NM; c++) x[c] = x[c] / ujj;
+      }
+      const bool mine = r < j && j < n;
+      const Real urj = row[j];
+#pragma unroll
+      for (int c = 0; c < NM; c++) {
+        const Real xj = grp_shfl<G>(x[c], base + j);
+        if (mine) x[c] -= urj * xj;
+      }
+    }
+    if (inf == 0) {
+#pragma unroll
+      for (int c = 0; c < NM; c++) row[c] = x[c];
+    }
+  }
+  if (ok && r < n) {
+#pragma unroll
+    for (int c = 0; c < NM; c++)
+      if (c < n) g[(size_t)r * n + c] = row[c];
+    if (ipiv) ipiv[pivoff[b] + r] = mypiv + 1;
+  }
+  if (ok && r == 0 && info) info[b] = inf;
+}
+
 // Blocks above LU_MAXN (the multi-cell blocks of Schwarz smoothers, up to LU_BIG_MAXN): one
 // workgroup of 256 threads per block, DGETF2 in place in global memory with the operation order
 // of the small-block kernel -- pivot = first row with the largest |a| in the column (workgroup
@@ -153,7 +275,8 @@ __global__ void __launch_bounds__(256) k_block_lu_big(int nblk, int nmax, const 
     Real best = Real(-1);
     int bi = n;
     for (int i = j + t; i < n; i += 256) {
-      const Real v = fabs(a[(size_t)i * n + j]);
+      Real v = fabs(a[(size_t)i * n + j]);
+      if (i == j && v != v) v = Real(INFINITY);   // idamax: a NaN first element is never beaten
       if (v > best || (v == best && i < bi)) { best = v; bi = i; }
     }
     wave_argmax(best, bi);
@@ -214,7 +337,8 @@ __device__ int lu_pivot_wg(const Real* a, int n, int j, Real* sv, int* si, int* 
   Real best = Real(-1);
   int bi = n;
   for (int i = j + t; i < n; i += 256) {
-    const Real v = fabs(a[(size_t)i * n + j]);
+    Real v = fabs(a[(size_t)i * n + j]);
+    if (i == j && v != v) v = Real(INFINITY);   // idamax: a NaN first element is never beaten
     if (v > best || (v == best && i < bi)) { best = v; bi = i; }
   }
   wave_argmax(best, bi);
@@ -457,6 +581,19 @@ static int lu_check_args(bosss_ctx* ctx, int32_t nmax, const int32_t* n, const i
 // Blocks up to LU_MAXN: one wavefront per block, the block staged in LDS.
 static int lu_launch_lds(int nblk, int nmax, const int32_t* n, const int64_t* off, double* A, int32_t* ipiv, const int64_t* pivoff,
                          int32_t* info, int want_inverse, hipStream_t s) {
+  if (nmax <= 20) {
+    // rows in registers, 64 / G blocks per wavefront (larger blocks would no longer unroll into
+    // registers: those take the LDS kernel)
+    if (nmax <= 16)
+      hipLaunchKernelGGL((k_block_lu_rows<double, 16, 16>), dim3((unsigned)((nblk + 15) / 16)), dim3(256), 0, s, nblk, nmax, n, off, A,
+                         ipiv, pivoff, info, want_inverse);
+    else
+      hipLaunchKernelGGL((k_block_lu_rows<double, 32, 20>), dim3((unsigned)((nblk + 7) / 8)), dim3(256), 0, s, nblk, nmax, n, off, A,
+                         ipiv, pivoff, info, want_inverse);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
+    return BOSSS_OK;
+  }
   size_t lds = lu_lds_bytes(nmax, want_inverse);
   if (lds > 160 * 1024) { g_lu_err = "block too large for an in-LDS inverse (n <= 100)"; return BOSSS_ERR_UNSUPPORTED; }
   if (hipFuncSetAttribute((const void*)k_block_lu<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {

@@ -441,8 +441,10 @@ int bosss_hip_op_info(const bosss_op* op, int64_t* ndof, int32_t* nvars, int32_t
 
 // ---------------------------------------------------------------- launching
 template <typename Real>
-static KArgs<Real> make_args(const bosss_op* op, double alpha, double beta, const void* U, const void* X, void* R, void* out) {
+static KArgs<Real> make_args(const bosss_op* op, double alpha, double beta, const void* U, const void* X, void* R, void* out,
+                             int own_only = 0) {
   KArgs<Real> a;
+  a.own_only = own_only;
   const bosss_grid* g = op->grid;
   a.J = g->J;
   a.detJ = g->detJ; a.invJ = g->invJ; a.scaling = g->scaling; a.lenscale = g->lenscale;
@@ -457,9 +459,11 @@ static KArgs<Real> make_args(const bosss_op* op, double alpha, double beta, cons
 
 // `stream` is a hipStream_t; NULL is HIP's null (default) stream, as everywhere in HIP.
 template <typename Real>
-static TArgs<Real> make_targs(const bosss_op* op, double alpha, double beta, const void* U, const void* X, void* R, void* out) {
-  KArgs<Real> k = make_args<Real>(op, alpha, beta, U, X, R, out);
+static TArgs<Real> make_targs(const bosss_op* op, double alpha, double beta, const void* U, const void* X, void* R, void* out,
+                              int own_only = 0) {
+  KArgs<Real> k = make_args<Real>(op, alpha, beta, U, X, R, out, own_only);
   TArgs<Real> a;
+  a.own_only = own_only;
   a.J = k.J; a.detJ = k.detJ; a.invJ = k.invJ; a.scaling = k.scaling; a.lenscale = k.lenscale; a.cf = k.cf;
   a.tab = (const Real*)op->ttab;
   a.fp = k.fp; a.alpha = alpha; a.beta = beta;
@@ -468,18 +472,20 @@ static TArgs<Real> make_targs(const bosss_op* op, double alpha, double beta, con
   return a;
 }
 
-static int launch(bosss_op* op, int which, double alpha, double beta, const void* U, const void* X, void* R, void* out, hipStream_t s) {
+// own_only (JVP): the neighbour tangents are taken as zero, so the result is J(U)_jj X_j per cell.
+static int launch(bosss_op* op, int which, double alpha, double beta, const void* U, const void* X, void* R, void* out, hipStream_t s,
+                  int own_only = 0) {
   HIPCHK(hipSetDevice(op->ctx->device));
   if (op->tensor) {
     dim3 grid((op->grid->J + 16 * TWAVES - 1) / (16 * TWAVES)), block(64 * TWAVES);
     if (op->precision == BOSSS_FP64) {
       void (*k)(TArgs<double>) = op->ks->kt64[which];
       if (!k) return fail(BOSSS_ERR_UNSUPPORTED, "tensor kernel variant %d not built", which);
-      hipLaunchKernelGGL(k, grid, block, op->ks->ktlds64[which], s, make_targs<double>(op, alpha, beta, U, X, R, out));
+      hipLaunchKernelGGL(k, grid, block, op->ks->ktlds64[which], s, make_targs<double>(op, alpha, beta, U, X, R, out, own_only));
     } else {
       void (*k)(TArgs<float>) = op->ks->kt32[which];
       if (!k) return fail(BOSSS_ERR_UNSUPPORTED, "tensor kernel variant %d not built", which);
-      hipLaunchKernelGGL(k, grid, block, op->ks->ktlds32[which], s, make_targs<float>(op, alpha, beta, U, X, R, out));
+      hipLaunchKernelGGL(k, grid, block, op->ks->ktlds32[which], s, make_targs<float>(op, alpha, beta, U, X, R, out, own_only));
     }
     HIPCHK(hipGetLastError());
     return BOSSS_OK;
@@ -487,12 +493,12 @@ static int launch(bosss_op* op, int which, double alpha, double beta, const void
   const int cells_per_block = 16 * (KernelSet::BLOCK / 64);
   dim3 grid((op->grid->J + cells_per_block - 1) / cells_per_block), block(KernelSet::BLOCK);
   if (op->precision == BOSSS_FP64) {
-    KArgs<double> a = make_args<double>(op, alpha, beta, U, X, R, out);
+    KArgs<double> a = make_args<double>(op, alpha, beta, U, X, R, out, own_only);
     void (*k)(KArgs<double>) = op->ks->k64[which];
     if (!k) return fail(BOSSS_ERR_UNSUPPORTED, "kernel variant %d not built", which);
     hipLaunchKernelGGL(k, grid, block, op->ks->lds64, s, a);
   } else {
-    KArgs<float> a = make_args<float>(op, alpha, beta, U, X, R, out);
+    KArgs<float> a = make_args<float>(op, alpha, beta, U, X, R, out, own_only);
     void (*k)(KArgs<float>) = op->ks->k32[which];
     if (!k) return fail(BOSSS_ERR_UNSUPPORTED, "kernel variant %d not built", which);
     hipLaunchKernelGGL(k, grid, block, op->ks->lds32, s, a);
@@ -686,6 +692,23 @@ static int ensure_matrix_setup(bosss_op* op) {
   return BOSSS_OK;
 }
 
+// Diagonal cell blocks by probing the own-cell JVP: X[c][e] = (e == q) in every cell, then
+// blocks[j][r][q] = alpha * (J(U)_jj e_q)[r] + shift * (r == q).
+template <typename Real>
+__global__ void k_probe_all(int J, int Nt, int q, Real* __restrict__ X) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)J * Nt) return;
+  X[i] = (int)(i % Nt) == q ? Real(1) : Real(0);
+}
+template <typename Real>
+__global__ void k_extract_diag(int J, int Nt, int q, const Real* __restrict__ Y, double alpha, double shift, double* __restrict__ blocks) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)J * Nt) return;
+  const size_t j = i / Nt;
+  const int r = (int)(i - j * Nt);
+  blocks[(j * Nt + r) * Nt + q] = alpha * (double)Y[i] + (r == q ? shift : 0.0);
+}
+
 // Autotuner of the block-ELL apply's panel size: times every candidate on this operator's
 // matrix and x (5 launches each after one warm-up, HIP events on the caller's stream, output
 // into a scratch vector) and keeps the fastest.  Under stream capture it does not time (events
@@ -766,6 +789,28 @@ int bosss_hip_op_assemble_dev(bosss_op* op, const double* U, double alpha, doubl
       if ((rc = launch(op, KernelSet::JVP, 1.0, 0.0, U, op->hX, nullptr, op->hY, s))) return rc;
       launch_extract(J, Nt, S, op->d_cols, op->d_color, k, q, op->hY, alpha, beta, blocks, s);
     }
+  HIPCHK(hipGetLastError());
+  return BOSSS_OK;
+}
+
+int bosss_hip_op_diag_blocks_dev(bosss_op* op, const void* U, double alpha, double shift, double* blocks, void* stream) {
+  if (!op || !U || !blocks) return fail(BOSSS_ERR_ARG, "NULL argument");
+  int rc;
+  HIPCHK(hipSetDevice(op->ctx->device));
+  if ((rc = ensure_staging(op))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int J = op->grid->J, Nt = op->V * op->N;
+  const size_t n = (size_t)J * Nt;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  for (int q = 0; q < Nt; q++) {
+    if (op->precision == BOSSS_FP64) hipLaunchKernelGGL(k_probe_all<double>, dim3(nb), dim3(256), 0, s, J, Nt, q, (double*)op->hX);
+    else hipLaunchKernelGGL(k_probe_all<float>, dim3(nb), dim3(256), 0, s, J, Nt, q, (float*)op->hX);
+    if ((rc = launch(op, KernelSet::JVP, 1.0, 0.0, U, op->hX, nullptr, op->hY, s, 1))) return rc;
+    if (op->precision == BOSSS_FP64)
+      hipLaunchKernelGGL(k_extract_diag<double>, dim3(nb), dim3(256), 0, s, J, Nt, q, (const double*)op->hY, alpha, shift, blocks);
+    else
+      hipLaunchKernelGGL(k_extract_diag<float>, dim3(nb), dim3(256), 0, s, J, Nt, q, (const float*)op->hY, alpha, shift, blocks);
+  }
   HIPCHK(hipGetLastError());
   return BOSSS_OK;
 }

@@ -94,6 +94,7 @@ struct KArgs {
   const Real* dU;   // tangent (JVP) or cotangent W (VJP)
   Real* R;          // residual output (beta R + alpha Op(U))
   Real* out;        // JVP / VJP output
+  int own_only;     // JVP: neighbour tangents taken as zero (J(U)_jj X_j, the diagonal blocks)
 };
 
 enum { MODE_RES = 0, MODE_JVP = 1, MODE_RES_JVP = 2 };
@@ -274,7 +275,7 @@ __global__ void __launch_bounds__(256) k_forward(KArgs<Real> a) {
     }
     Real cnb[V][NK], tnb[V][NK];
     load_coeffs<Real, V, N, NK>(a.U, cn, has_nb, scn, grp, cnb);
-    if (TAN) load_coeffs<Real, V, N, NK>(a.dU, cn, has_nb, scn, grp, tnb);
+    if (TAN) load_coeffs<Real, V, N, NK>(a.dU, cn, has_nb && !a.own_only, scn, grp, tnb);
 
     const Real* tro = tr + f * S::KEP * N;
     const Real* trn = tr + fo * S::KEP * N;

@@ -224,6 +224,7 @@ struct TArgs {
   Real* R;
   Real* out;
   int nxcd;             // XCDs the workgroups are dealt over (1: no remapping)
+  int own_only;         // JVP: neighbour tangents taken as zero (J(U)_jj X_j, the diagonal blocks)
 };
 
 // Workgroup -> cell-block mapping.  The dispatcher deals consecutive workgroups
@@ -379,7 +380,7 @@ __global__ void __launch_bounds__(64 * TWAVES, (tensor_min_waves<Real, P>())) kt
     const Real* nbU = a.U + (size_t)cn * NT;
     const Real* nbX = TAN ? a.X + (size_t)cn * NT : nullptr;
     auto Bnu = [&](int v, int kk) { int m = 4 * kk + grp; return nbU[v * N + (m < N ? m : N - 1)] * ((has_nb && m < N) ? scn : Real(0)); };
-    auto Bnx = [&](int v, int kk) { int m = 4 * kk + grp; return nbX[v * N + (m < N ? m : N - 1)] * ((has_nb && m < N) ? scn : Real(0)); };
+    auto Bnx = [&](int v, int kk) { int m = 4 * kk + grp; return nbX[v * N + (m < N ? m : N - 1)] * ((has_nb && !a.own_only && m < N) ? scn : Real(0)); };
 
     for (int t = 0; t < TS::TE; t++) {
       const int nodeE = t * 16 + col;
@@ -1226,7 +1227,7 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
     const Real* nbU = a.U + (size_t)cn * NT;
     const Real* nbX = a.X + (size_t)cn * NT;
     auto Bnu = [&](int v, int kk) { int m = 4 * kk + grp; return nbU[v * N + (m < N ? m : N - 1)] * ((has_nb && m < N) ? scn : Real(0)); };
-    auto Bnx = [&](int v, int kk) { int m = 4 * kk + grp; return nbX[v * N + (m < N ? m : N - 1)] * ((has_nb && m < N) ? scn : Real(0)); };
+    auto Bnx = [&](int v, int kk) { int m = 4 * kk + grp; return nbX[v * N + (m < N ? m : N - 1)] * ((has_nb && !a.own_only && m < N) ? scn : Real(0)); };
 
     for (int t = 0; t < TS::TE; t++) {
       const int nodeE = t * 16 + col;

@@ -415,17 +415,17 @@ __device__ inline double wave_sum(double v) {
 // partial sums of a row are added through LDS in a fixed order.  With DOT, the workgroup's
 // sum of y_i x_i over its rows (p.q of CG, q as stored) goes to partial[2 b] (partial[2 b + 1] = 0).
 constexpr int XPANEL = 7 * 2 * XN;   // panel columns: at most 7 slots of at most 2 N entries
-template <typename Real, bool DOT>
-__global__ void __launch_bounds__(256) k_xdg_spmv_panel(int J, int S, const int32_t* __restrict__ nrows, const int32_t* __restrict__ mcol,
-                                                        const int64_t* __restrict__ xoff, const int64_t* __restrict__ boff,
-                                                        const int64_t* __restrict__ dof, const Real* __restrict__ A, double alpha,
-                                                        const Real* __restrict__ x, double beta, Real* __restrict__ y,
-                                                        double* __restrict__ partial) {
-  __shared__ double xs[4][XPANEL];
-  __shared__ double red[4][64];
-  __shared__ double wsum[4];
+template <typename Real, bool DOT, int T>
+__global__ void __launch_bounds__(256 * T) k_xdg_spmv_panel(int J, int S, const int32_t* __restrict__ nrows, const int32_t* __restrict__ mcol,
+                                                            const int64_t* __restrict__ xoff, const int64_t* __restrict__ boff,
+                                                            const int64_t* __restrict__ dof, const Real* __restrict__ A, double alpha,
+                                                            const Real* __restrict__ x, double beta, Real* __restrict__ y,
+                                                            double* __restrict__ partial, int ngroups) {
+  __shared__ double xs[4 * T][XPANEL];
+  __shared__ double red[4 * T][64];
+  __shared__ double wsum[4 * T];
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;   // wave-uniform: scalar index loads
-  const int j0 = blockIdx.x * 4 + w;
+  const int j0 = blockIdx.x * 4 * T + w;
   const bool valid = j0 < J;
   const int j = valid ? j0 : J - 1;
   const int n = nrows[j];
@@ -462,12 +462,14 @@ __global__ void __launch_bounds__(256) k_xdg_spmv_panel(int J, int S, const int3
     t = valid ? (double)yi * xs[w][lane] : 0.0;   // slot 0 is the diagonal: xs[w][0..n) = own x
   }
   if (DOT) {
+    // one partial per group of four cell rows, whatever T: the same sums in the same order
     t = wave_sum(t);
     if (lane == 0) wsum[w] = t;
     __syncthreads();
-    if (threadIdx.x == 0) {
-      partial[2 * blockIdx.x] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
-      partial[2 * blockIdx.x + 1] = 0.0;
+    const int rg = blockIdx.x * T + (w >> 2);
+    if (lane == 0 && (w & 3) == 0 && rg < ngroups) {
+      partial[2 * rg] = ((wsum[w] + wsum[w + 1]) + wsum[w + 2]) + wsum[w + 3];
+      partial[2 * rg + 1] = 0.0;
     }
   }
 }
@@ -658,22 +660,22 @@ __global__ void __launch_bounds__(1024) k_dot_final(int nb, const double* __rest
 // PCG scalar slots
 enum { S_RZ = 0, S_PQ = 1, S_RZN = 2, S_RR = 3, S_BB = 4, S_NSLOT = 8 };
 
-// Fused CG step on one cell row per wavefront (four per workgroup): with STEP, x += alpha p and
+// Fused CG step on one cell row per wavefront (4 T per workgroup): with STEP, x += alpha p and
 // r -= alpha q (alpha = rz / pq, 0 if pq == 0); then z = Dinv r with the column-major inverse
 // diagonal block (lane l < G n reads entries l + G n t, as the panel apply), or z = r when IDENT
 // (the FP32 solve runs on the equilibrated matrix, whose diagonal blocks are the identity); the
 // workgroup's sums of r.z and r.r go to partial[2 b], partial[2 b + 1].
-template <typename Real, bool IDENT>
-__global__ void __launch_bounds__(256) k_pcg_update(int J, int step, const int32_t* __restrict__ nrows, const int64_t* __restrict__ dof,
-                                                    const int64_t* __restrict__ doff, const double* __restrict__ scal,
-                                                    const Real* __restrict__ Dinv, const Real* __restrict__ p, const Real* __restrict__ q,
-                                                    Real* __restrict__ x, Real* __restrict__ r, Real* __restrict__ z,
-                                                    double* __restrict__ partial) {
-  __shared__ double rs[4][64];
-  __shared__ double red[4][64];
-  __shared__ double wsum[4][2];
+template <typename Real, bool IDENT, int T>
+__global__ void __launch_bounds__(256 * T) k_pcg_update(int J, int step, const int32_t* __restrict__ nrows, const int64_t* __restrict__ dof,
+                                                        const int64_t* __restrict__ doff, const double* __restrict__ scal,
+                                                        const Real* __restrict__ Dinv, const Real* __restrict__ p, const Real* __restrict__ q,
+                                                        Real* __restrict__ x, Real* __restrict__ r, Real* __restrict__ z,
+                                                        double* __restrict__ partial, int ngroups) {
+  __shared__ double rs[4 * T][64];
+  __shared__ double red[4 * T][64];
+  __shared__ double wsum[4 * T][2];
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;   // wave-uniform: scalar index loads
-  const int j0 = blockIdx.x * 4 + w;
+  const int j0 = blockIdx.x * 4 * T + w;
   const bool valid = j0 < J;
   const int j = valid ? j0 : J - 1;
   const int n = nrows[j];
@@ -717,9 +719,10 @@ __global__ void __launch_bounds__(256) k_pcg_update(int J, int step, const int32
   t2 = wave_sum(t2);
   if (lane == 0) { wsum[w][0] = t1; wsum[w][1] = t2; }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = ((wsum[0][0] + wsum[1][0]) + wsum[2][0]) + wsum[3][0];
-    partial[2 * blockIdx.x + 1] = ((wsum[0][1] + wsum[1][1]) + wsum[2][1]) + wsum[3][1];
+  const int rg = blockIdx.x * T + (w >> 2);
+  if (lane == 0 && (w & 3) == 0 && rg < ngroups) {
+    partial[2 * rg] = ((wsum[w][0] + wsum[w + 1][0]) + wsum[w + 2][0]) + wsum[w + 3][0];
+    partial[2 * rg + 1] = ((wsum[w][1] + wsum[w + 1][1]) + wsum[w + 2][1]) + wsum[w + 3][1];
   }
 }
 // p = z + beta p with beta = rz_new / rz (0 if rz == 0)
@@ -765,6 +768,11 @@ struct bosss_xdg {
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   const void* graph_key[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   int graph_k[2] = {0, 0};
+  int graph_tile[2] = {0, 0};
+  // tile shape of the solver's cell-row kernels (T groups of four rows per workgroup), per
+  // precision: 0 = tune at the next solve; tile_ms = timed ms per apply + update of each candidate
+  int tile[2] = {0, 0};
+  float tile_ms[2][3] = {{-1.f, -1.f, -1.f}, {-1.f, -1.f, -1.f}};
   ~bosss_xdg() {
     for (int i = 0; i < 2; i++) if (graph[i]) (void)hipGraphExecDestroy(graph[i]);
     for (void* p : allocs) (void)hipFree(p);
@@ -817,21 +825,41 @@ void launch_spmv(bosss_xdg* X, const Real* A, double alpha, const Real* x, doubl
                      X->d_nrows, X->d_cols, X->d_boff, X->d_dof, A, alpha, x, beta, y);
 }
 
-// workgroups of the cell-row kernels (four cell rows each)
-inline unsigned row_groups(const bosss_xdg* X) { return (unsigned)((X->a.J + 3) / 4); }
+// groups of four cell rows: one dot-product partial each, whatever the tile shape
+inline int row_groups(const bosss_xdg* X) { return (X->a.J + 3) / 4; }
+
+// tile shapes of the cell-row kernels: T groups of four rows (4 T wavefronts) per workgroup
+constexpr int XDG_TILES[3] = {1, 2, 4};
+inline int tile_of(const bosss_xdg* X, int prec) { return X->tile[prec] > 0 ? X->tile[prec] : 1; }
+
+template <typename Real, bool DOT, int T>
+void spmv_panel_T(bosss_xdg* X, const Real* A, double alpha, const Real* x, double beta, Real* y, hipStream_t s) {
+  const int ng = row_groups(X);
+  hipLaunchKernelGGL((k_xdg_spmv_panel<Real, DOT, T>), dim3((unsigned)((ng + T - 1) / T)), dim3(256 * T), 0, s, X->a.J, X->a.S, X->d_nrows,
+                     X->d_mcol, X->d_xoff, X->d_boff, X->d_dof, A, alpha, x, beta, y, X->partial, ng);
+}
+template <typename Real, bool IDENT, int T>
+void update_T(bosss_xdg* X, int step, const Real* Dinv, const Real* p, const Real* q, Real* x, Real* r, Real* z, hipStream_t s) {
+  const int ng = row_groups(X);
+  hipLaunchKernelGGL((k_pcg_update<Real, IDENT, T>), dim3((unsigned)((ng + T - 1) / T)), dim3(256 * T), 0, s, X->a.J, step, X->d_nrows,
+                     X->d_dof, X->d_doff, X->scal, Dinv, p, q, x, r, z, X->partial, ng);
+}
 
 // y = alpha A x + beta y on the column-major copy; with dot_slot >= 0 also scal[dot_slot] = x.y
 // (and first scal[cdst] = scal[csrc] when cdst >= 0)
 template <typename Real>
 void launch_spmv_cm(bosss_xdg* X, const Real* A, double alpha, const Real* x, double beta, Real* y, hipStream_t s, int dot_slot = -1,
                     int cdst = -1, int csrc = 0) {
+  const int T = tile_of(X, sizeof(Real) == 4);
   if (dot_slot < 0) {
-    hipLaunchKernelGGL((k_xdg_spmv_panel<Real, false>), dim3(row_groups(X)), dim3(256), 0, s, X->a.J, X->a.S, X->d_nrows, X->d_mcol,
-                       X->d_xoff, X->d_boff, X->d_dof, A, alpha, x, beta, y, X->partial);
+    if (T == 4) spmv_panel_T<Real, false, 4>(X, A, alpha, x, beta, y, s);
+    else if (T == 2) spmv_panel_T<Real, false, 2>(X, A, alpha, x, beta, y, s);
+    else spmv_panel_T<Real, false, 1>(X, A, alpha, x, beta, y, s);
   } else {
-    hipLaunchKernelGGL((k_xdg_spmv_panel<Real, true>), dim3(row_groups(X)), dim3(256), 0, s, X->a.J, X->a.S, X->d_nrows, X->d_mcol,
-                       X->d_xoff, X->d_boff, X->d_dof, A, alpha, x, beta, y, X->partial);
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(1024), 0, s, (int)row_groups(X), X->partial, X->scal, dot_slot, -1, cdst, csrc);
+    if (T == 4) spmv_panel_T<Real, true, 4>(X, A, alpha, x, beta, y, s);
+    else if (T == 2) spmv_panel_T<Real, true, 2>(X, A, alpha, x, beta, y, s);
+    else spmv_panel_T<Real, true, 1>(X, A, alpha, x, beta, y, s);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(1024), 0, s, row_groups(X), X->partial, X->scal, dot_slot, -1, cdst, csrc);
   }
 }
 
@@ -839,13 +867,59 @@ void launch_spmv_cm(bosss_xdg* X, const Real* A, double alpha, const Real* x, do
 // scal[S_RZN] = r.z, scal[S_RR] = r.r.
 template <typename Real>
 void launch_update(bosss_xdg* X, int step, const Real* Dinv, const Real* p, const Real* q, Real* x, Real* r, Real* z, hipStream_t s) {
-  if (Dinv)
-    hipLaunchKernelGGL((k_pcg_update<Real, false>), dim3(row_groups(X)), dim3(256), 0, s, X->a.J, step, X->d_nrows, X->d_dof, X->d_doff,
-                       X->scal, Dinv, p, q, x, r, z, X->partial);
-  else
-    hipLaunchKernelGGL((k_pcg_update<Real, true>), dim3(row_groups(X)), dim3(256), 0, s, X->a.J, step, X->d_nrows, X->d_dof, X->d_doff,
-                       X->scal, Dinv, p, q, x, r, z, X->partial);
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(1024), 0, s, (int)row_groups(X), X->partial, X->scal, (int)S_RZN, (int)S_RR, -1, 0);
+  const int T = tile_of(X, sizeof(Real) == 4);
+  if (Dinv) {
+    if (T == 4) update_T<Real, false, 4>(X, step, Dinv, p, q, x, r, z, s);
+    else if (T == 2) update_T<Real, false, 2>(X, step, Dinv, p, q, x, r, z, s);
+    else update_T<Real, false, 1>(X, step, Dinv, p, q, x, r, z, s);
+  } else {
+    if (T == 4) update_T<Real, true, 4>(X, step, Dinv, p, q, x, r, z, s);
+    else if (T == 2) update_T<Real, true, 2>(X, step, Dinv, p, q, x, r, z, s);
+    else update_T<Real, true, 1>(X, step, Dinv, p, q, x, r, z, s);
+  }
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(1024), 0, s, row_groups(X), X->partial, X->scal, (int)S_RZN, (int)S_RR, -1, 0);
+}
+
+// Autotuner of the tile shape (once per precision, not under stream capture): times five rounds of
+// the iteration's apply (with its p.q partials) and update (step 0: z = Dinv r and its partials,
+// x and r untouched) for each candidate on the solver's own matrix and keeps the fastest.  Every
+// candidate forms the same partial sums per group of four rows in the same order, so the CG
+// iterates do not depend on the choice.  The buffers it runs on are set again by pcg_start.
+template <typename Real>
+int tune_tiles(bosss_xdg* X, const Real* A, const Real* Dinv, Real* p, Real* q, Real* r, Real* z, hipStream_t s) {
+  const int prec = sizeof(Real) == 4;
+  if (X->tile[prec] > 0) return BOSSS_OK;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  XHIP(hipStreamIsCapturing(s, &cs));
+  if (cs != hipStreamCaptureStatusNone) return BOSSS_OK;     // keep the default shape for this solve
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  XHIP(hipEventCreate(&e0));
+  hipError_t e = hipEventCreate(&e1);
+  float best = 1e30f;
+  int bestT = 1;
+  for (int c = 0; c < 3 && e == hipSuccess; c++) {
+    X->tile[prec] = XDG_TILES[c];
+    launch_spmv_cm<Real>(X, A, 1.0, p, 0.0, q, s, S_PQ, S_RZ, S_RZN);
+    launch_update<Real>(X, 0, Dinv, p, q, nullptr, r, z, s);
+    e = hipEventRecord(e0, s);
+    for (int k = 0; k < 5 && e == hipSuccess; k++) {
+      launch_spmv_cm<Real>(X, A, 1.0, p, 0.0, q, s, S_PQ, S_RZ, S_RZN);
+      launch_update<Real>(X, 0, Dinv, p, q, nullptr, r, z, s);
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess) {
+      X->tile_ms[prec][c] = ms / 5;
+      if (ms < best) { best = ms; bestT = XDG_TILES[c]; }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (e != hipSuccess) { X->tile[prec] = 0; return xfail(BOSSS_ERR_HIP, "tile autotuner: %s", hipGetErrorString(e)); }
+  X->tile[prec] = bestT;
+  return BOSSS_OK;
 }
 
 // One PCG iteration (Real = double or float), all scalars on the device; A column-major, Dinv
@@ -875,6 +949,7 @@ template <typename Real>
 int pcg_solve(bosss_xdg* X, int gslot, const Real* A, const Real* Dinv, const Real* b, Real* x, Real* r, Real* z, Real* p, Real* q,
               double tol, int maxit, int K, int use_graph, bool stop_on_stall, int* iters, double* relres) {
   hipStream_t s = X->stream;
+  if (int rc = tune_tiles<Real>(X, A, Dinv, p, q, r, z, s)) return rc;
   pcg_start<Real>(X, A, Dinv, b, x, r, z, p, s);
   double h[S_NSLOT];
   XHIP(hipMemcpyAsync(h, X->scal, sizeof h, hipMemcpyDeviceToHost, s));
@@ -885,7 +960,8 @@ int pcg_solve(bosss_xdg* X, int gslot, const Real* A, const Real* Dinv, const Re
     // the graph bakes in the matrix and solution pointers: rebuild when they change
     const void* key0 = (const void*)A;
     const void* key1 = (const void*)x;
-    if (!X->graph[gslot] || X->graph_key[gslot][0] != key0 || X->graph_key[gslot][1] != key1 || X->graph_k[gslot] != K) {
+    if (!X->graph[gslot] || X->graph_key[gslot][0] != key0 || X->graph_key[gslot][1] != key1 || X->graph_k[gslot] != K ||
+        X->graph_tile[gslot] != tile_of(X, sizeof(Real) == 4)) {
       if (X->graph[gslot]) { XHIP(hipGraphExecDestroy(X->graph[gslot])); X->graph[gslot] = nullptr; }
       hipGraph_t g;
       XHIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -897,6 +973,7 @@ int pcg_solve(bosss_xdg* X, int gslot, const Real* A, const Real* Dinv, const Re
       X->graph_key[gslot][0] = key0;
       X->graph_key[gslot][1] = key1;
       X->graph_k[gslot] = K;
+      X->graph_tile[gslot] = tile_of(X, sizeof(Real) == 4);
     }
   }
   // The recurrence residual drifts from b - A x in finite precision: on convergence the true
@@ -1195,6 +1272,27 @@ int bosss_hip_xdg_solve_dev(bosss_xdg* x, const double* blocks, const double* b,
   *outer = k;
   *relres = rel;
   XHIP(hipGetLastError());
+  return BOSSS_OK;
+}
+
+int bosss_hip_xdg_set_tile(bosss_xdg* x, int32_t precision, int32_t t) {
+  if (!x) return xfail(BOSSS_ERR_ARG, "NULL operator");
+  if (precision != BOSSS_FP64 && precision != BOSSS_FP32) return xfail(BOSSS_ERR_ARG, "precision must be BOSSS_FP64 or BOSSS_FP32");
+  if (t != 0 && t != 1 && t != 2 && t != 4) return xfail(BOSSS_ERR_ARG, "tile must be 0 (tune), 1, 2 or 4 groups of four rows");
+  const int prec = precision == BOSSS_FP32;
+  x->tile[prec] = t;
+  if (t == 0)
+    for (int c = 0; c < 3; c++) x->tile_ms[prec][c] = -1.0f;
+  return BOSSS_OK;
+}
+
+int bosss_hip_xdg_tiles(const bosss_xdg* x, int32_t precision, int32_t* chosen, float* ms3) {
+  if (!x) return xfail(BOSSS_ERR_ARG, "NULL operator");
+  if (precision != BOSSS_FP64 && precision != BOSSS_FP32) return xfail(BOSSS_ERR_ARG, "precision must be BOSSS_FP64 or BOSSS_FP32");
+  const int prec = precision == BOSSS_FP32;
+  if (chosen) *chosen = x->tile[prec];
+  if (ms3)
+    for (int c = 0; c < 3; c++) ms3[c] = x->tile_ms[prec][c];
   return BOSSS_OK;
 }
 

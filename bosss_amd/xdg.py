@@ -26,6 +26,9 @@ the length-scale assembly of MultiphaseAgglomerator.LengthScaleAgg (MultiphaseAg
 """
 import numpy as np
 
+from . import levelset
+from .levelset import PolyLevelSet
+
 SPECIES = ("A", "B")
 
 
@@ -258,6 +261,38 @@ def tensor_nodes(lo, hi, n):
 
 
 # ------------------------------------------------------------- XDG problem
+def _cell_bounds(psi, lo, hi):
+    """Lower and upper bounds of psi on the boxes [lo, hi] (exact for QuadLevelSet)."""
+    if isinstance(psi, PolyLevelSet):
+        return psi.bounds(lo, hi)
+    a, b, c = psi.as_tuple()
+    return _bounds_many(a, b, c, lo, hi)
+
+
+def _cell_rule(psi, lo, hi, n):
+    if isinstance(psi, PolyLevelSet):
+        return levelset.box_nodes([psi], lo, hi, n)
+    return box_nodes([psi.as_tuple()], lo, hi, n)
+
+
+def _face_rule(psi, d, x, tl, th, n):
+    """Rule of the face x_d = x over the tangential box [tl, th], split at the interface, and the
+    species (0: psi < 0, 1: psi > 0) of each node."""
+    if isinstance(psi, PolyLevelSet):
+        ff = psi.restrict(d, x)
+        Y, W = levelset.box_nodes([ff], tl, th, n)
+        return Y, W, (ff(Y) > 0).astype(int)
+    ff = _restrict(psi.as_tuple(), d, x)
+    Y, W = box_nodes([ff], tl, th, n)
+    return Y, W, (((ff[0] * Y * Y + ff[1] * Y).sum(1) + ff[2]) > 0).astype(int)
+
+
+def _interface_rule(psi, lo, hi, n):
+    if isinstance(psi, PolyLevelSet):
+        return levelset.surface_nodes(psi, lo, hi, n)
+    return surface_nodes(psi.as_tuple(), lo, hi, n)
+
+
 class XdgRules:
     """Species layout, quadrature items and metrics of a two-species XDG discretisation.
 
@@ -273,23 +308,39 @@ class XdgRules:
         self.grid, self.psi, self.n = grid, psi, n
         D, J = grid.D, grid.J
         self.D, self.J = D, J
-        f = psi.as_tuple()
         lo, hi = grid.cell_lo, grid.cell_hi
         ctr, hc = grid.centers, grid.hc
-        # ---- cell classification (exact bounds of the separable quadratic)
-        mn, mx = _bounds_many(f[0], f[1], f[2], lo, hi)
-        self.cut = (mn < 0) & (mx > 0)
-        has_A = mn < 0
-        has_B = mx > 0
-        self.has = np.stack([has_A, has_B], 1)                      # [J, 2]
-        self.nspec = self.has.sum(1).astype(np.int32)
-        self.spec0 = np.where(has_A, 0, 1).astype(np.int32)        # species of the first block
+        # ---- cell classification: bounds of psi on every cell (exact for the separable
+        # quadratics, Bernstein bounds for polynomials), then the candidates' rules decide
+        mn, mx = _cell_bounds(psi, lo, hi)
+        cand = np.where((mn < 0) & (mx > 0))[0]
 
         def to_ref(X, j):
             return 2.0 * (X - ctr[j]) / hc[j]
 
+        cut_items, single = [], {}
+        for j in cand:
+            X, W = _cell_rule(psi, lo[j], hi[j], n)
+            s = (psi(X) > 0).astype(int)
+            present = [sp for sp in (0, 1) if (s == sp).any()]
+            if len(present) == 1:
+                single[j] = present[0]                 # bounds straddled zero, psi does not
+                continue
+            for sp in present:
+                sel = s == sp
+                cut_items.append((j, sp, to_ref(X[sel], j), W[sel]))
+        self.cut = np.zeros(J, bool)
+        self.cut[[it[0] for it in cut_items]] = True
+        noncand = ~((mn < 0) & (mx > 0))                 # one species: A where psi <= 0 there, else B
+        has_A = (noncand & (mn < 0)) | self.cut
+        has_B = (noncand & ~(mn < 0)) | self.cut
+        for j, sp in single.items():
+            (has_A if sp == 0 else has_B)[j] = True
+        self.has = np.stack([has_A, has_B], 1)                      # [J, 2]
+        self.nspec = self.has.sum(1).astype(np.int32)
+        self.spec0 = np.where(has_A, 0, 1).astype(np.int32)        # species of the first block
+
         # ---- volume items
-        vcell, vspec, vnodes, vw = [], [], [], []
         xr, wr = tensor_nodes(-np.ones(D), np.ones(D), n)           # reference rule for uncut cells
         unc = np.where(~self.cut)[0]
         vol = np.zeros((J, 2))
@@ -297,15 +348,8 @@ class XdgRules:
         vol[unc, u_spec] = grid.volume[unc]
         self._uncut_cells = unc
         cut_cells = np.where(self.cut)[0]
-        cut_items = []
-        for j in cut_cells:
-            X, W = box_nodes([f], lo[j], hi[j], n)
-            s = (psi(X) > 0).astype(int)
-            for sp in (0, 1):
-                sel = s == sp
-                if sel.any():
-                    cut_items.append((j, sp, to_ref(X[sel], j), W[sel]))
-                    vol[j, sp] = W[sel].sum()
+        for j, sp, _, W in cut_items:
+            vol[j, sp] = W.sum()
         # assemble: uncut (vectorised) then cut
         nu, K0 = len(unc), len(wr)
         self.vol_cell = np.concatenate([unc, [it[0] for it in cut_items]]).astype(np.int32)
@@ -323,11 +367,10 @@ class XdgRules:
         eplus = grid.normals[np.arange(E), edir] > 0
         exf = np.where(eplus, hi[ein, edir], lo[ein, edir])           # face coordinate
         # bounds of psi on each face: psi restricted to x_d = exf over the other dims
-        a_, b_ = f[0], f[1]
         flo, fhi = lo[ein].copy(), hi[ein].copy()
         flo[np.arange(E), edir] = exf
         fhi[np.arange(E), edir] = exf
-        fmn, fmx = _bounds_many(a_, b_, f[2], flo, fhi)
+        fmn, fmx = _cell_bounds(psi, flo, fhi)
         ecut = (fmn < 0) & (fmx > 0)
         espec = np.where(fmn < 0, 0, 1)
         edge_area = np.zeros((E, 2))
@@ -349,10 +392,8 @@ class XdgRules:
                 items.append((e, espec[e], X[i], W[i]))
         for e in np.where(ecut)[0]:
             d = edir[e]
-            ff = _restrict(f, d, exf[e])
             tl, th = np.delete(lo[ein[e]], d), np.delete(hi[ein[e]], d)
-            Y, W = box_nodes([ff], tl, th, n)
-            sv = (((ff[0] * Y * Y + ff[1] * Y).sum(1) + ff[2]) > 0).astype(int)
+            Y, W, sv = _face_rule(psi, d, exf[e], tl, th, n)
             for sp in (0, 1):
                 m_ = sv == sp
                 if m_.any():
@@ -388,7 +429,7 @@ class XdgRules:
         icell, inodes, iw, inorm = [], [], [], []
         iarea = np.zeros(J)
         for j in cut_cells:
-            X, W, N = surface_nodes(f, lo[j], hi[j], n)
+            X, W, N = _interface_rule(psi, lo[j], hi[j], n)
             if len(W):
                 icell.append(j); inodes.append(to_ref(X, j)); iw.append(W); inorm.append(N)
                 iarea[j] = W.sum()
@@ -587,3 +628,17 @@ class XdgOperator:
                                                   int(bool(use_graph)), check_every, ctypes.byref(it), ctypes.byref(outer),
                                                   ctypes.byref(rel), self._stream(stream)))
         return it.value, outer.value, rel.value
+
+    def set_tile(self, t, precision="fp64"):
+        """Fix the solver's tile shape (1, 2 or 4 groups of four cell rows per workgroup) or ask for
+        a new tuning at the next solve (0)."""
+        from ._lib import lib, BOSSS_FP64, BOSSS_FP32
+        self._check(lib().bosss_hip_xdg_set_tile(self.h, BOSSS_FP64 if precision == "fp64" else BOSSS_FP32, int(t)))
+
+    def tiles(self, precision="fp64"):
+        """(chosen T or 0, {T: timed ms per apply + update}) of the solver's tile autotuner."""
+        import ctypes
+        from ._lib import lib, BOSSS_FP64, BOSSS_FP32
+        c, ms = ctypes.c_int32(), (ctypes.c_float * 3)()
+        self._check(lib().bosss_hip_xdg_tiles(self.h, BOSSS_FP64 if precision == "fp64" else BOSSS_FP32, ctypes.byref(c), ms))
+        return c.value, {t: round(float(m), 5) for t, m in zip((1, 2, 4), ms) if m >= 0}

@@ -152,6 +152,13 @@ int bosss_hip_evaluate_vjp_dev(bosss_op* op, double alpha, double beta, const do
 int bosss_hip_op_matrix_layout(bosss_op* op, int32_t* blocks_per_row, int64_t* entries, int32_t* ncolors, int32_t* cols);
 /* blocks <- beta*blocks + alpha*dR/dU(U)  (device pointers; the Jacobian of a linear form is its matrix) */
 int bosss_hip_op_assemble_dev(bosss_op* op, const double* U, double alpha, double beta, double* blocks, void* stream);
+/* Diagonal cell blocks of the Jacobian, for cell-block preconditioners (the blocks BlockJacobi.InitImpl
+ * reads from the assembled matrix, BlockJacobi.cs:63-120, without assembling the off-diagonal ones):
+ * blocks[j] <- alpha * dR_j/dU_j (U) + shift * I, row-major Nt x Nt (Nt = V*N), J blocks of FP64.
+ * Nt probing JVPs with the neighbour tangents taken as zero, in the operator's precision (U is
+ * double for BOSSS_FP64, float for BOSSS_FP32; an FP32 operator gives FP32-accurate blocks, widened).
+ * Works for every form, linear or not. */
+int bosss_hip_op_diag_blocks_dev(bosss_op* op, const void* U, double alpha, double shift, double* blocks, void* stream);
 /* y <- alpha*A x + beta*y with A in the operator's block-ELL layout (BlockMsrMatrix.SpMV, BlockMsrMatrix.cs:2554) */
 int bosss_hip_bell_spmv_dev(bosss_op* op, const double* blocks, double alpha, const double* x, double beta, double* y, void* stream);
 /* Host-pointer forms of the two calls above (blocks, U, x, y host arrays of entries / ndof
@@ -282,6 +289,15 @@ int bosss_hip_xdg_spmv_dev(bosss_xdg* xdg, const double* blocks, double alpha, c
 int bosss_hip_xdg_solve_dev(bosss_xdg* xdg, const double* blocks, const double* b, double* sol, double tol, int32_t maxit,
                             int32_t precision, int32_t use_graph, int32_t check_every, int32_t* iterations, int32_t* outer,
                             double* relres, void* stream);
+/* Tile shape of the solver's cell-row kernels (apply and update, one wavefront per cell row): T
+ * groups of four rows per workgroup, T = 1, 2 or 4.  The first solve in each precision times the
+ * three shapes on its own matrix (an autotuner; not while the stream is being captured) and keeps the
+ * fastest; t = 0 asks for a new tuning, 1, 2 or 4 fixes the shape.  Every shape forms the same
+ * dot-product partial sums per group of four rows in the same order, so the CG iterates do not
+ * depend on it.  bosss_hip_xdg_tiles reports the chosen T (0: not tuned yet) and the timed ms per
+ * apply + update of T = 1, 2, 4 (< 0: not timed). */
+int bosss_hip_xdg_set_tile(bosss_xdg* xdg, int32_t precision, int32_t t);
+int bosss_hip_xdg_tiles(const bosss_xdg* xdg, int32_t precision, int32_t* chosen, float* ms3);
 /* Host-pointer forms (blocks: nentries doubles, vectors: ndof doubles, host memory; copies on the
  * operator's stream, synchronous).  bosss_hip_xdg_solve runs the solver above with use_graph = 1
  * and check_every = 16. */
@@ -290,6 +306,35 @@ int bosss_hip_xdg_spmv(bosss_xdg* xdg, const double* blocks, double alpha, const
 int bosss_hip_xdg_solve(bosss_xdg* xdg, const double* blocks, const double* b, double* sol, double tol, int32_t maxit, int32_t precision,
                         int32_t* iterations, int32_t* outer, double* relres);
 const char* bosss_hip_xdg_last_error(void);
+
+/* ---- device-resident GMRES (Newton.MatrixFreeGMRES, Newton.cs:1128-1500, with the Arnoldi
+ *      process, the Hessenberg matrix and the Givens rotations on the GPU; the reference keeps
+ *      them on the host).  Restart length m <= 256; V holds m+1 basis vectors of n entries at a
+ *      row stride ldv >= n, double (BOSSS_FP64) or float (BOSSS_FP32); H is the (m+1) x m
+ *      column-major Hessenberg matrix, cs, sn the m rotations, g the m+1 rotated right-hand side
+ *      (g[0] = |r0| before step 0, the rest zero), all double; work holds the doubles that
+ *      bosss_hip_gmres_workspace reports.  Both calls only queue kernels on `stream` (no
+ *      allocation, no memset, no synchronisation), so a run of them can be captured into a graph.
+ *
+ *      arnoldi step k (0 <= k < m): on entry V[0..k] and w = A M^-1 V[k] (right preconditioning;
+ *      w = A V[k] without); classical Gram-Schmidt twice against V[0..k], column k of H, V[k+1] =
+ *      w / |w| (zero after a breakdown), the previous rotations applied to column k, rotation k,
+ *      g updated; res[k] (m doubles) receives |g[k+1]|, the residual estimate of the k+1-step
+ *      solution (g[k+1] itself changes in later steps).  w is overwritten.
+ *      update: y = R^-1 g[0..k1) (R the leading k1 x k1 triangle of H, y: k1 doubles) and
+ *      x += sum_{i<k1} y_i Z[i] (Z = V, or the preconditioned vectors M^-1 V[i], stride ldz). ---- */
+int bosss_hip_gmres_workspace(int32_t m, int64_t* doubles);
+int bosss_hip_gmres_arnoldi_dev(bosss_ctx* ctx, int64_t n, int32_t k, int32_t m, int32_t precision, void* V, int64_t ldv, void* w,
+                                double* H, double* cs, double* sn, double* g, double* res, double* work, void* stream);
+int bosss_hip_gmres_update_dev(bosss_ctx* ctx, int64_t n, int32_t k1, int32_t m, int32_t precision, const void* Z, int64_t ldz,
+                               const double* H, const double* g, double* y, void* x, void* stream);
+/* Cell-block preconditioner apply: y_b <- alpha A_b x_b + beta y_b for nblk blocks of uniform size
+ * n <= 128 (x_b, y_b at b*n), A_b stored TRANSPOSED (AinvT[b][k][i] = A_b[i][k]: each block column
+ * is contiguous, read coalesced), double or float like x and y.  For the inverted diagonal blocks
+ * of a Newton step (BlockJacobi.Solve, BlockJacobi.cs:148-197), in FP32 for the mixed solver. */
+int bosss_hip_block_gemv_uniform_dev(bosss_ctx* ctx, int nblk, int32_t n, int32_t precision, const void* AinvT, double alpha,
+                                     const void* x, double beta, void* y, void* stream);
+const char* bosss_hip_krylov_last_error(void);
 
 /* ---- measured arithmetic ceilings (no counterpart in the reference: the rates the roofline
  *      fractions are quoted against, measured on the context's device) ----

@@ -174,3 +174,75 @@ def test_large_block_zero_diagonal_swaps_in_the_pivot(sizes, col):
     assert all(pivref[poff[b] + col] != col for b in range(len(sizes)))   # a real swap at column col
     assert (ipiv.cpu().numpy() == pivref + 1).all()
     assert np.abs(A.cpu().numpy() - LUref).max() < 1e-12 * np.abs(LUref).max()
+
+
+@pytest.mark.parametrize("sizes", [list(range(1, 17)) * 3, [10] * 61 + [20] * 9, [20, 19, 3, 17, 7, 12] * 5])
+def test_register_kernel_small_blocks(sizes):
+    """Blocks up to 20 DOF factor with their rows in registers, 64 / G blocks per wavefront (batches
+    whose largest block is <= 16 and <= 20): pivots and factors equal the oracle's getf2,
+    zero pivots (a zero column, even and odd) set INFO and the factorisation continues, a zero
+    diagonal entry with a non-zero pivot below is swapped in, a NaN is never chosen as pivot; the
+    inverses satisfy A inv(A) = I."""
+    import torch
+    from bosss_amd.blocks import BlockBatch
+    say("case register-kernel blocks", len(sizes), "blocks, max", max(sizes))
+    blocks = make_blocks(sizes, 21)
+    big = [b for b, n in enumerate(sizes) if n >= 6]
+    blocks[big[0]][:, 2] = 0.0                       # zero pivot at an even column
+    blocks[big[1]][:, 5] = 0.0                       # and at an odd one
+    blocks[big[2]] = zero_diagonal_block(sizes[big[2]], 3, 5)
+    blocks[big[3]][4, 0] = np.nan                    # NaN below the diagonal in column 0
+    bb = BlockBatch(sizes)
+    A0 = np.concatenate([b.ravel() for b in blocks])
+    A = torch.from_numpy(A0.copy()).to(bb.dev)
+    ipiv = torch.zeros(bb.ndof, dtype=torch.int32, device=bb.dev)
+    info = torch.zeros(bb.nblk, dtype=torch.int32, device=bb.dev)
+    bb.factor(A, ipiv, info)
+    torch.cuda.synchronize()
+    LUref, pivref, inforef, off, poff = ref_block_lu(sizes, blocks)
+    assert (info.cpu().numpy() == inforef).all()
+    assert inforef[big[0]] == 3 and inforef[big[1]] == 6 and inforef[big[2]] == 0
+    assert (ipiv.cpu().numpy() == pivref + 1).all()
+    LU = A.cpu().numpy()
+    fin = np.isfinite(LUref)
+    assert (np.isfinite(LU) == fin).all()
+    assert np.abs(LU[fin] - LUref[fin]).max() < 1e-12 * np.abs(LUref[fin]).max()
+    # inverses of the regular blocks
+    ok = [b for b in range(len(sizes)) if inforef[b] == 0 and b != big[3]]
+    Ai = torch.from_numpy(np.concatenate([blocks[b].ravel() for b in ok])).to(bb.dev)
+    bi = BlockBatch([sizes[b] for b in ok])
+    infoi = torch.zeros(bi.nblk, dtype=torch.int32, device=bi.dev)
+    bi.factor(Ai, None, infoi, inverse=True)
+    torch.cuda.synchronize()
+    inv = Ai.cpu().numpy()
+    assert (infoi.cpu().numpy() == 0).all()
+    for k, b in enumerate(ok):
+        n = sizes[b]
+        Ainv = inv[bi.off[k]:bi.off[k] + n * n].reshape(n, n)
+        assert np.abs(blocks[b] @ Ainv - np.eye(n)).max() < 1e-12 * n * max(1.0, np.abs(Ainv).max())
+
+
+@pytest.mark.parametrize("sizes", [[9, 12], [40, 33], [150, 140], [600, 520]])   # registers, LDS, one workgroup, split
+def test_nan_row_follows_idamax(sizes):
+    """A NaN below the diagonal in column 0 turns that row into NaNs; when the elimination reaches
+    the row's own column its NaN diagonal entry is the pivot (IDAMAX never moves past a NaN first
+    element) and INFO stays 0 (NaN is not zero): the oracle's pivots in every kernel form."""
+    import torch
+    from bosss_amd.blocks import BlockBatch
+    say("case NaN pivots", sizes)
+    blocks = make_blocks(sizes, 31)
+    blocks[0][4, 0] = np.nan
+    bb = BlockBatch(sizes)
+    A = torch.from_numpy(np.concatenate([b.ravel() for b in blocks])).to(bb.dev)
+    ipiv = torch.zeros(bb.ndof, dtype=torch.int32, device=bb.dev)
+    info = torch.zeros(bb.nblk, dtype=torch.int32, device=bb.dev)
+    bb.factor(A, ipiv, info)
+    torch.cuda.synchronize()
+    LUref, pivref, inforef, off, poff = ref_block_lu(sizes, blocks)
+    assert (info.cpu().numpy() == inforef).all() and (inforef == 0).all()
+    assert pivref[4] == 4                                    # the NaN row is its column's pivot
+    assert (ipiv.cpu().numpy() == pivref + 1).all()
+    LU = A.cpu().numpy()
+    n1 = sizes[1]
+    b1 = slice(off[1], off[1] + n1 * n1)                    # the second block is untouched by the NaN
+    assert np.abs(LU[b1] - LUref[b1]).max() < 1e-12 * np.abs(LUref[b1]).max()
