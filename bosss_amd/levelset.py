@@ -9,8 +9,8 @@ surfaces and volumes in hyperrectangles", SIAM J. Sci. Comput. 37(2), A993-A1019
 Quadrature/SayeGaussRule_*.cs, ``CutCellQuadratureType = Saye``, AppControl.cs:1102).  The
 functions here follow that algorithm for general polynomials:
 
-  * constraints that keep one sign on a box are dropped (Bernstein bounds, refined by
-    subdivision when they straddle zero);
+  * constraints that keep one sign on a box are dropped (Bernstein bounds of the box and, when
+    they straddle zero, of its 2^D and 4^D sub-boxes);
   * a height direction k is one in which every remaining constraint is monotone (the Bernstein
     bounds of d psi / d x_k exclude zero), chosen with the largest |d psi / d x_k| at the centre;
     the base constraints are the restrictions to the bottom and top faces, the base box is
@@ -33,6 +33,21 @@ MAX_DEPTH = 12
 
 def _binom(n, k):
     return math.comb(n, k)
+
+
+_BERN = {}
+
+
+def _bern_tables(q):
+    """binom(i, k), the exponents i - k and k, the mask k <= i, and M[k, m] = binom(m, k) /
+    binom(q, k) (t^k in the degree-q Bernstein basis), for degree q."""
+    if q not in _BERN:
+        i = np.arange(q + 1)[:, None]
+        k = np.arange(q + 1)[None, :]
+        binom = np.array([[_binom(a, b) if b <= a else 0 for b in range(q + 1)] for a in range(q + 1)], float)
+        M = np.array([[_binom(m, kk) / _binom(q, kk) if m >= kk else 0.0 for m in range(q + 1)] for kk in range(q + 1)])
+        _BERN[q] = (binom, np.maximum(i - k, 0), np.broadcast_to(k, (q + 1, q + 1)), k <= i, M)
+    return _BERN[q]
 
 
 class PolyLevelSet:
@@ -151,21 +166,24 @@ class PolyLevelSet:
         lo = np.atleast_2d(np.asarray(lo, float))
         hi = np.atleast_2d(np.asarray(hi, float))
         B = lo.shape[0]
-        v = np.broadcast_to(self.C, (B,) + self.C.shape).copy()
+        if B == 1:
+            # one box: contract the leading axis and append the result; after D steps the axes are
+            # back in order
+            v = self.C
+            for d in range(self.D):
+                binom, ex_lo, ex_h, tri, M = _bern_tables(self.C.shape[d] - 1)
+                T = np.where(tri, binom * lo[0, d] ** ex_lo * (hi[0, d] - lo[0, d]) ** ex_h, 0.0)
+                v = np.tensordot(v, T @ M, axes=([0], [0]))
+            return v[None]
+        v = np.broadcast_to(self.C, (B,) + self.C.shape)
         for d in range(self.D):
             q = self.C.shape[d] - 1
+            binom, ex_lo, ex_h, tri, M = _bern_tables(q)
             h = hi[:, d] - lo[:, d]
-            # monomial x^i with x = lo + h t:  sum_k binom(i, k) lo^(i-k) h^k t^k
-            T = np.zeros((B, q + 1, q + 1))
-            for i in range(q + 1):
-                for k in range(i + 1):
-                    T[:, i, k] = _binom(i, k) * lo[:, d] ** (i - k) * h ** k
-            # t^k in the Bernstein basis of degree q: t^k = sum_{m >= k} binom(m, k)/binom(q, k) B_m
-            M = np.zeros((q + 1, q + 1))
-            for k in range(q + 1):
-                for m in range(k, q + 1):
-                    M[k, m] = _binom(m, k) / _binom(q, k)
-            TM = np.einsum("bik,km->bim", T, M)
+            # monomial x^i with x = lo + h t:  sum_{k <= i} binom(i, k) lo^(i-k) h^k t^k, then t^k in
+            # the Bernstein basis of degree q (M)
+            T = np.where(tri, binom * lo[:, d, None, None] ** ex_lo * h[:, None, None] ** ex_h, 0.0)
+            TM = T @ M
             v = np.moveaxis(np.einsum("b...i,bim->b...m", np.moveaxis(v, d + 1, -1), TM), -1, d + 1)
         return v
 
@@ -173,24 +191,23 @@ class PolyLevelSet:
         b = self.bernstein(lo, hi).reshape(np.atleast_2d(lo).shape[0], -1)
         return b.min(1), b.max(1)
 
-    def sign_on(self, lo, hi, depth=3):
-        """+1 / -1 if psi keeps that strict sign on the box (Bernstein bounds, refined by halving
-        up to `depth` times), 0 if it may change sign there."""
-        mn, mx = self.bounds(lo, hi)
-        if mn[0] > 0:
-            return 1
-        if mx[0] < 0:
-            return -1
-        if depth == 0:
-            return 0
+    def sign_on(self, lo, hi, levels=2):
+        """+1 / -1 if psi keeps that strict sign on the box, 0 if it may change sign there: the
+        Bernstein bounds of the box, then of its 2^D and 4^D equal sub-boxes (one vectorised call
+        per level)."""
         lo = np.asarray(lo, float); hi = np.asarray(hi, float)
-        d = int(np.argmax(hi - lo))
-        mid = 0.5 * (lo[d] + hi[d])
-        h1 = hi.copy(); h1[d] = mid
-        l2 = lo.copy(); l2[d] = mid
-        s1 = self.sign_on(lo, h1, depth - 1)
-        s2 = self.sign_on(l2, hi, depth - 1)
-        return s1 if (s1 == s2 and s1 != 0) else 0
+        for lev in range(levels + 1):
+            m = 1 << lev
+            idx = np.stack(np.meshgrid(*[np.arange(m)] * len(lo), indexing="ij"), -1).reshape(-1, len(lo))
+            h = (hi - lo) / m
+            mn, mx = self.bounds(lo + idx * h, lo + (idx + 1) * h)
+            if (mn > 0).all():
+                return 1
+            if (mx < 0).all():
+                return -1
+            if (mn > 0).any() and (mx < 0).any():
+                return 0                                  # both signs certainly present
+        return 0
 
     def monotone(self, k, lo, hi):
         """d psi / d x_k keeps a strict sign on the box (Bernstein bounds)."""

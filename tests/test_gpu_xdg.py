@@ -1,8 +1,9 @@
 """GPU parity of the XDG Poisson path against the oracle (bosss_ref XDG assembly).
 
 Cases: the reference's ParabolaTest setting (2D planar interface, p = 2), a sphere in a
-3D Cartesian box (p = 2, and p = 3 for 40 x 40 cut-cell blocks) and an off-centre sphere on
-a grid graded in every direction.  Matrix and affine vector
+3D Cartesian box (p = 2, and p = 3 for 40 x 40 cut-cell blocks), an off-centre sphere on
+a grid graded in every direction, and a torus (a degree-4 polynomial level set, cut-cell
+rules from the general Saye algorithm).  Matrix and affine vector
 to 1e-12 (relative to the largest entry), the block-sparse apply, and the block-Jacobi PCG
 solves: FP64 and FP32-with-refinement both to ||b - M x|| <= 1e-10 ||b|| (5e-12 for the
 parabola, whose exact-solution check needs the algebraic error far below 1e-10), with and without
@@ -48,7 +49,16 @@ def graded_sphere_case(n, p):
     return R, (-1.0, -10.0), (1.0, 0.5), g_D
 
 
-CASES = [("parabola", 0.0, 2), ("parabola", 0.3, 2), ("sphere", 8, 2), ("sphere", 6, 3), ("sphere_graded", 6, 2)]
+def torus_case(n, p):
+    """A torus (degree-4 polynomial level set, general Saye rules of bosss_amd.levelset)."""
+    from bosss_amd.levelset import PolyLevelSet
+    g = CartesianGrid((n,) * 3)
+    R = XdgRules(g, PolyLevelSet.torus([0.5, 0.48, 0.52], 0.27, 0.12), n=p + 1)
+    g_D = lambda X: 0.1 * X[:, 0] - 0.2 * X[:, 1] * X[:, 2]
+    return R, (-1.0, -10.0), (1.0, 0.5), g_D
+
+
+CASES = [("parabola", 0.0, 2), ("parabola", 0.3, 2), ("sphere", 8, 2), ("sphere", 6, 3), ("sphere_graded", 6, 2), ("torus", 8, 2)]
 
 
 def build(case, arg, p):
@@ -57,6 +67,9 @@ def build(case, arg, p):
         src = (1.0, 1.0)
     elif case == "sphere_graded":
         R, mu, src, g_D = graded_sphere_case(arg, p)
+        ex = None
+    elif case == "torus":
+        R, mu, src, g_D = torus_case(arg, p)
         ex = None
     else:
         R, mu, src, g_D = sphere_case(arg, p)
