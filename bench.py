@@ -109,6 +109,29 @@ def hbm_traffic(kernel, n):
     return None
 
 
+def fp64_issue(kernel, n, kernel_ms):
+    """FP64 work the kernel issues per launch on the n^3 mesh (committed rocprofv3 counts,
+    profiles/fp64_issue.json: MFMA flops = SQ_INSTS_VALU_MFMA_MOPS_F64 x 512, VALU flops =
+    64 x (ADD + MUL + TRANS + 2 FMA)_F64) over this run's kernel time: the MFMA, VALU and combined
+    rates as fractions of the FP64 peak, or None if no counts were taken for the kernel.  The
+    dense-contraction roofline counts only the algorithm's contractions; this counts what the
+    kernel issues, padding of the MFMA tiles and the pointwise flux and adjoint included."""
+    path = os.path.join(ROOT, "profiles", "fp64_issue.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for r in rec.get("kernels", []):
+        if r.get("kernel") == kernel and r.get("mesh") == n:
+            t = kernel_ms * 1e-3 * 1e12
+            m, v = r["mfma_flops"] / t, r["valu_flops"] / t
+            return {"mfma_tflops": round(m, 2), "valu_tflops": round(v, 2), "mfma_frac": round(m / FP64_PEAK_TFLOPS, 4),
+                    "valu_frac": round(v / FP64_PEAK_TFLOPS, 4), "combined_frac": round((m + v) / FP64_PEAK_TFLOPS, 4),
+                    "mfma_flops_per_launch": r["mfma_flops"], "valu_flops_per_launch": r["valu_flops"]}
+    return None
+
+
 def copy_calibration(dev, nbytes=1 << 30):
     """One device-to-device copy of nbytes (torch copy kernel): the known-size reference that
     rocprofv3 FETCH_SIZE / WRITE_SIZE readings of the DG kernel are scaled by."""
@@ -482,7 +505,8 @@ def main():
                      "kernel": kernel_label(form, p, args.split),
                      "kernel_ms": round(kernel_ms, 4), "alg_flops_per_launch": flops,
                      "kernel_ms_median": round(float(np.median(per_launch)), 4),
-                     "kernel_ms_min_max": [round(min(per_launch), 4), round(max(per_launch), 4)]},
+                     "kernel_ms_min_max": [round(min(per_launch), 4), round(max(per_launch), 4)],
+                     "fp64_issue": fp64_issue(kernel_label(form, p, args.split), n, kernel_ms)},
         "clock_settle": {"seconds": args.settle_s, "steps": settle_steps},
     }
     print("bench: %s kernels, %s, %.3f ms/step, %.3f ms kernel" % (

@@ -1,11 +1,12 @@
 // Batched uneven cell-block LU, inverse and block-Jacobi sweep (BlockJacobi.cs:63-197,
 // IMatrixExtensions.InvertInPlace:2128-2230 -> LAPACK DGETRF/DGETRI semantics).
 //
-// One wavefront owns one block.  The block is staged in LDS with an odd row
-// stride (no bank conflicts on column access); lane i owns rows i and i+64.
-// Partial pivoting picks the first row with the largest |a| in the column,
-// as LAPACK's idamax does; a zero pivot sets INFO = j+1 (1-based) and the
-// factorisation continues, like DGETF2.  Pivots are stored 1-based (ipiv).
+// Every form follows DGETF2: partial pivoting picks the first row with the largest
+// |a| in the column, as LAPACK's idamax does (a NaN in the diagonal position is never
+// beaten); a zero pivot sets INFO = j+1 (1-based) and the factorisation continues.
+// Pivots are stored 1-based (ipiv).  Blocks up to 20 DOF keep their rows in registers
+// (several blocks per wavefront), 21..128 DOF one block per workgroup in registers,
+// larger ones (Schwarz blocks) go through global memory.
 //
 // All blocks of a batch are processed by one launch (blocks of different size
 // share the launch; each wave reads its own size), so a whole block-Jacobi set
@@ -36,94 +37,215 @@ __device__ __forceinline__ void wave_argmax(Real& v, int& idx) {
   }
 }
 
-// In-place LU of the block in LDS (row stride ld).  Returns INFO (0 or first zero pivot, 1-based).
-template <typename Real>
-__device__ int lds_getrf(Real* A, int n, int ld, int* piv) {
-  const int lane = threadIdx.x & 63;
-  int info = 0;
-  for (int j = 0; j < n; j++) {
-    Real best = Real(-1);
-    int bi = n;
-    for (int i = lane; i < n; i += 64)
-      if (i >= j) {
-        Real v = fabs(A[i * ld + j]);
-        if (i == j && v != v) v = Real(INFINITY);   // idamax: a NaN first element is never beaten
-        if (v > best || (v == best && i < bi)) { best = v; bi = i; }
-      }
-    wave_argmax(best, bi);
-    if (bi >= n) bi = j;   // no comparable candidate (all NaN): the diagonal, as idamax
-    if (lane == 0) piv[j] = bi;
-    __syncthreads();
-    // one wavefront: every lane reads the pivot before any lane swaps it.  NaN passes the test
-    // and leaves INFO unset, as in DGETF2 (A(JP,J).NE.ZERO).
-    if (!(A[bi * ld + j] != Real(0))) {
-      if (!info) info = j + 1;
-      __syncthreads();
-      continue;
+template <int RA>
+struct TileLDS {
+  double col[2][16 * RA], row[2][16 * RA], pv[2];
+  int pos[16 * RA], lrow[16 * RA], piv[16 * RA], p[2];
+};
+
+// Loads block g (n x n, row-major) into the register tile and resets the row bookkeeping.
+template <int RA>
+__device__ __forceinline__ void tile_load(double (&a)[RA][RA], const double* __restrict__ g, int n, TileLDS<RA>& L) {
+  const int tid = threadIdx.x, tc = tid >> 4, tr = tid & 15;
+#pragma unroll
+  for (int x = 0; x < RA; x++)
+#pragma unroll
+    for (int y = 0; y < RA; y++) {
+      const int r = tr + 16 * x, c = tc + 16 * y;
+      a[x][y] = 0.0;
+      if (r < n && c < n) a[x][y] = g[(int64_t)r * n + c];
     }
-    if (bi != j)
-      for (int c = lane; c < n; c += 64) { Real t = A[j * ld + c]; A[j * ld + c] = A[bi * ld + c]; A[bi * ld + c] = t; }
-    __syncthreads();
-    const Real inv = Real(1) / A[j * ld + j];
-    for (int i = lane; i < n; i += 64)
-      if (i > j) {
-        Real l = A[i * ld + j] * inv;
-        A[i * ld + j] = l;
-        for (int c = j + 1; c < n; c++) A[i * ld + c] -= l * A[j * ld + c];
-      }
-    __syncthreads();
-  }
-  return info;
+  for (int i = tid; i < 16 * RA; i += 256) { L.pos[i] = i; L.lrow[i] = i; }
+  __syncthreads();
 }
 
-template <typename Real>
-__global__ void __launch_bounds__(64) k_block_lu(int nblk, int nmax, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
-                                                 Real* __restrict__ A, int32_t* __restrict__ ipiv, const int64_t* __restrict__ pivoff,
-                                                 int32_t* __restrict__ info, int want_inverse) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
+// Column steps 0..n-1 of getf2 (GJ = false) or of in-place Gauss-Jordan (GJ = true) on the register
+// tile.  Returns INFO (first zero pivot, 1-based; Gauss-Jordan stops there).  L.piv[j] receives the
+// position of step j's pivot row, L.pos / L.lrow the final row placement.
+template <int RA, bool GJ>
+__device__ __forceinline__ int tile_eliminate(double (&a)[RA][RA], int n, TileLDS<RA>& L) {
+  const int tid = threadIdx.x, tc = tid >> 4, tr = tid & 15;
+  unsigned used = 0;   // bit x: row tr + 16 x is in a position <= j (LU form only)
+  int inf = 0;
+  for (int j = 0; j < n; j++) {
+    const int k = j & 1, jc = j & 15, jy = j >> 4;
+    if (tc == jc) {
+      // the 16 threads holding column j: publish it, find the pivot among the unused rows
+      double best = -1.0, bval = 0.0;
+      int bpos = 1 << 30, bp = -1;
+#pragma unroll
+      for (int yy = 0; yy < RA; yy++)
+        if (yy == jy) {
+#pragma unroll
+          for (int x = 0; x < RA; x++) {
+            const int r = tr + 16 * x;
+            const double v0 = a[x][yy];
+            if (r < n) {
+              L.col[k][r] = v0;
+              const int ps = L.pos[r];
+              if (ps >= j) {
+                double v = fabs(v0);
+                if (ps == j && v != v) v = (double)INFINITY;   // idamax: a NaN first element is never beaten
+                if (v > best || (v == best && ps < bpos)) { best = v; bpos = ps; bp = r; bval = v0; }
+              }
+            }
+            if (GJ) a[x][yy] = 0.0;   // becomes -a[r][j] / pivot through the update
+          }
+        }
+#pragma unroll
+      for (int o = 8; o >= 1; o >>= 1) {
+        const double ov = __shfl_xor(best, o, 16), oval = __shfl_xor(bval, o, 16);
+        const int opos = __shfl_xor(bpos, o, 16), op = __shfl_xor(bp, o, 16);
+        if (ov > best || (ov == best && opos < bpos)) { best = ov; bpos = opos; bp = op; bval = oval; }
+      }
+      if (tr == 0) {
+        if (bp < 0) { bp = L.lrow[j]; bval = L.col[k][bp]; }   // no comparable candidate (all NaN): position j
+        L.p[k] = bp;
+        L.pv[k] = bval;
+      }
+    }
+    __syncthreads();
+    const int p = __builtin_amdgcn_readfirstlane(L.p[k]);
+    const double pv = L.pv[k];
+    if (tid == 0) L.piv[j] = L.pos[p];
+    if (!(pv != 0.0)) {   // zero pivot (NaN passes, as in DGETF2)
+      if (!inf) inf = j + 1;
+      if (GJ) return inf;
+      const int r0 = L.lrow[j];   // no exchange: the row in position j is done all the same
+      if (tr == (r0 & 15)) used |= 1u << (r0 >> 4);
+      continue;
+    }
+    if (tid == 0) {
+      const int q = L.pos[p], rj = L.lrow[j];
+      L.pos[p] = j; L.pos[rj] = q; L.lrow[j] = p; L.lrow[q] = rj;
+    }
+    const int pc = p & 15, px = __builtin_amdgcn_readfirstlane(p >> 4);
+    const double pinv = 1.0 / pv;
+    if (tr == pc) {
+#pragma unroll
+      for (int xx = 0; xx < RA; xx++)
+        if (xx == px) {
+#pragma unroll
+          for (int y = 0; y < RA; y++) {
+            const int c = tc + 16 * y;
+            L.row[k][c] = GJ ? (c == j ? pinv : a[xx][y] * pinv) : a[xx][y];
+          }
+        }
+    }
+    __syncthreads();
+    double u[RA];
+#pragma unroll
+    for (int y = 0; y < RA; y++) u[y] = L.row[k][tc + 16 * y];
+    if (GJ) {
+#pragma unroll
+      for (int x = 0; x < RA; x++) {
+        const int r = tr + 16 * x;
+        const double f = r < n ? L.col[k][r] : 0.0;
+#pragma unroll
+        for (int y = 0; y < RA; y++) a[x][y] -= f * u[y];
+      }
+      if (tr == pc) {
+#pragma unroll
+        for (int xx = 0; xx < RA; xx++)
+          if (xx == px) {
+#pragma unroll
+            for (int y = 0; y < RA; y++) a[xx][y] = u[y];
+          }
+      }
+    } else {
+      // rows after the pivot in factor order: multiplier into column j, columns > j updated
+#pragma unroll
+      for (int x = 0; x < RA; x++) {
+        const int r = tr + 16 * x;
+        if (r < n && r != p && !((used >> x) & 1u)) {
+          const double l = L.col[k][r] * pinv;
+#pragma unroll
+          for (int y = 0; y < RA; y++)
+            if (tc + 16 * y > j) a[x][y] -= l * u[y];
+          if (tc == jc) {
+#pragma unroll
+            for (int yy = 0; yy < RA; yy++)
+              if (yy == jy) a[x][yy] = l;
+          }
+        }
+      }
+      if (tr == pc) used |= 1u << px;
+    }
+  }
+  return inf;
+}
+
+// Writes the tile back: factors in position order (row pos[r]), or the inverse with the column
+// permutation too (X[pos[r]][lrow[c]] = a[r][c]).
+template <int RA>
+__device__ __forceinline__ void tile_store(const double (&a)[RA][RA], double* __restrict__ g, int n, const TileLDS<RA>& L, bool inverse) {
+  const int tid = threadIdx.x, tc = tid >> 4, tr = tid & 15;
+#pragma unroll
+  for (int x = 0; x < RA; x++) {
+    const int r = tr + 16 * x;
+    if (r < n) {
+      const int64_t row = (int64_t)L.pos[r] * n;
+#pragma unroll
+      for (int y = 0; y < RA; y++) {
+        const int c = tc + 16 * y;
+        if (c < n) g[row + (inverse ? L.lrow[c] : c)] = a[x][y];
+      }
+    }
+  }
+}
+
+// Blocks of 21..16 RA DOF (the 100-DOF cell blocks of p = 3 Euler, up to 128): one workgroup of
+// 256 threads per block, the whole block in registers.  Thread (tc, tr) = (tid >> 4, tid & 15)
+// holds rows tr + 16 x and columns tc + 16 y (x, y < RA), in the block's original row order:
+// row exchanges are bookkeeping only, pos[r] is the position (row of the factors) that physical
+// row r has been given and lrow[] its inverse.  Each column step: the 16 threads holding column j
+// find the pivot among the rows not yet used (largest |a|, ties to the smaller position, a NaN in
+// position j never beaten: getf2's IDAMAX in getf2's row order) with shuffles inside their 16-lane
+// group and publish the column; the threads holding the pivot row publish it; every thread updates
+// its entries.  Factors, pivots and INFO are getf2's, operation for operation (multiplier a * (1 / pivot),
+// then the rank-one update of the columns after j).
+//
+// With INV the elimination is Gauss-Jordan in place (every row other than the pivot row updated at
+// every step: uniform work for all threads) and the inverse is written out with the row and column
+// permutation applied.  A zero pivot stops it; the block is then factored from its original values
+// instead (LU factors and INFO, no inverse, as DGETRF leaves them when DGETRI cannot proceed).  Register columns and rows are
+// picked with wavefront-uniform branches, so the block never leaves the register file.
+template <int RA, bool INV>
+__global__ void __launch_bounds__(256) k_block_lu_tile(int nblk, int nmax, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
+                                                       double* __restrict__ A, int32_t* __restrict__ ipiv,
+                                                       const int64_t* __restrict__ pivoff, int32_t* __restrict__ info) {
+  __shared__ TileLDS<RA> L;
   const int b = blockIdx.x;
   if (b >= nblk) return;
   const int n = nsz[b];
-  if (n <= 0 || n > nmax) {          // size outside what the launch was sized for: report, touch nothing
+  if (n <= 0 || n > nmax || n > 16 * RA) {   // size outside what the launch was sized for: report, touch nothing
     if (threadIdx.x == 0 && info) info[b] = -1;
     return;
   }
-  const int ld = n | 1;
-  const int cap = nmax * (nmax | 1);
-  Real* L = reinterpret_cast<Real*>(smem_raw);
-  Real* X = L + cap;                                        // inverse workspace (only if want_inverse)
-  int* piv = reinterpret_cast<int*>(L + (want_inverse ? 2 : 1) * cap);
-  Real* g = A + off[b];
-  const int lane = threadIdx.x;
-  for (int i = lane; i < n * n; i += 64) L[(i / n) * ld + (i % n)] = g[i];
-  __syncthreads();
-  int inf = lds_getrf(L, n, ld, piv);
-  if (want_inverse && inf == 0) {
-    // DGETRI result: column c of inv(A) solves A x = e_c; lane c owns column c of X.
-    for (int i = lane; i < n * n; i += 64) X[(i / n) * ld + (i % n)] = Real((i / n) == (i % n) ? 1 : 0);
-    __syncthreads();
-    for (int c = lane; c < n; c += 64) {
-      for (int j = 0; j < n; j++) { int p = piv[j]; if (p != j) { Real t = X[j * ld + c]; X[j * ld + c] = X[p * ld + c]; X[p * ld + c] = t; } }
-      for (int i = 0; i < n; i++) { Real a = X[i * ld + c]; for (int k = 0; k < i; k++) a -= L[i * ld + k] * X[k * ld + c]; X[i * ld + c] = a; }
-      for (int i = n - 1; i >= 0; i--) {
-        Real a = X[i * ld + c];
-        for (int k = i + 1; k < n; k++) a -= L[i * ld + k] * X[k * ld + c];
-        X[i * ld + c] = a / L[i * ld + i];
-      }
+  double* g = A + off[b];
+  double a[RA][RA];
+  tile_load<RA>(a, g, n, L);
+  int inf = 0;
+  bool inverse = false;
+  if (INV) {
+    inf = tile_eliminate<RA, true>(a, n, L);
+    inverse = inf == 0;
+    if (!inverse) {   // singular: factor the original block (LU and INFO, no inverse)
+      __syncthreads();
+      tile_load<RA>(a, g, n, L);
     }
-    __syncthreads();
-    for (int i = lane; i < n * n; i += 64) g[i] = X[(i / n) * ld + (i % n)];
-  } else {
-    for (int i = lane; i < n * n; i += 64) g[i] = L[(i / n) * ld + (i % n)];
   }
+  if (!inverse) inf = tile_eliminate<RA, false>(a, n, L);
+  __syncthreads();
+  tile_store<RA>(a, g, n, L, inverse);
   if (ipiv)
-    for (int j = lane; j < n; j += 64) ipiv[pivoff[b] + j] = piv[j] + 1;
-  if (lane == 0 && info) info[b] = inf;
+    for (int i = threadIdx.x; i < n; i += 256) ipiv[pivoff[b] + i] = L.piv[i] + 1;
+  if (threadIdx.x == 0 && info) info[b] = inf;
 }
 
 // Small blocks (n <= 20: the 10- and 20-DOF cell blocks of XDG and p <= 2 scalar operators): G lanes
 // per block (G = 16 or 32, so 64 / G blocks per wavefront), lane r of a group holds row r of its
-// block in registers.  DGETF2 with the operation order of lds_getrf -- pivot = first row with the
+// block in registers.  DGETF2 with the operation order of the tile kernel's LU form -- pivot = first row with the
 // largest |a| in the column (group argmax over the lanes' rows, ties to the smaller row; as in
 // IDAMAX a NaN on the diagonal is kept and any other NaN is never chosen), row swap (register exchange through lane shuffles), column scaled by the reciprocal
 // pivot, rank-1 update a_rc -= l_r u_jc with u_jc broadcast from the pivot row's lane -- so the
@@ -445,9 +567,6 @@ __global__ void k_lu_info(int nblk, const int32_t* __restrict__ st, int32_t* __r
   if (b < nblk) info[b] = st[b];
 }
 
-size_t lu_lds_bytes(int nmax, int want_inverse) {
-  return (size_t)(want_inverse ? 2 : 1) * nmax * (nmax | 1) * sizeof(double) + (size_t)nmax * sizeof(int);
-}
 
 // x_b <- LU_b \ x_b   (getrs with the 1-based pivots), one wavefront per block, any size up to
 // 4096.  Lane l owns the rows i = l (mod 64) of x_b and is the only one to write them: the
@@ -578,8 +697,8 @@ static int lu_check_args(bosss_ctx* ctx, int32_t nmax, const int32_t* n, const i
   return BOSSS_OK;
 }
 
-// Blocks up to LU_MAXN: one wavefront per block, the block staged in LDS.
-static int lu_launch_lds(int nblk, int nmax, const int32_t* n, const int64_t* off, double* A, int32_t* ipiv, const int64_t* pivoff,
+// Blocks up to LU_MAXN, in registers: up to 20 DOF several blocks per wavefront, 21..128 one workgroup per block.
+static int lu_launch_reg(int nblk, int nmax, const int32_t* n, const int64_t* off, double* A, int32_t* ipiv, const int64_t* pivoff,
                          int32_t* info, int want_inverse, hipStream_t s) {
   if (nmax <= 20) {
     // rows in registers, 64 / G blocks per wavefront (larger blocks would no longer unroll into
@@ -594,12 +713,23 @@ static int lu_launch_lds(int nblk, int nmax, const int32_t* n, const int64_t* of
     if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
     return BOSSS_OK;
   }
-  size_t lds = lu_lds_bytes(nmax, want_inverse);
-  if (lds > 160 * 1024) { g_lu_err = "block too large for an in-LDS inverse (n <= 100)"; return BOSSS_ERR_UNSUPPORTED; }
-  if (hipFuncSetAttribute((const void*)k_block_lu<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
-    g_lu_err = "hipFuncSetAttribute failed"; return BOSSS_ERR_HIP;
+  // 21..128: one workgroup per block, the block in registers
+  const dim3 grid((unsigned)nblk), blk(256);
+#define LU_TILE(RA)                                                                                                     \
+  if (want_inverse)                                                                                                     \
+    hipLaunchKernelGGL((k_block_lu_tile<RA, true>), grid, blk, 0, s, nblk, nmax, n, off, A, ipiv, pivoff, info);        \
+  else                                                                                                                  \
+    hipLaunchKernelGGL((k_block_lu_tile<RA, false>), grid, blk, 0, s, nblk, nmax, n, off, A, ipiv, pivoff, info);
+  if (nmax <= 32) {
+    LU_TILE(2)
+  } else if (nmax <= 64) {
+    LU_TILE(4)
+  } else if (nmax <= 112) {
+    LU_TILE(7)
+  } else {
+    LU_TILE(8)
   }
-  hipLaunchKernelGGL(k_block_lu<double>, dim3(nblk), dim3(64), lds, s, nblk, nmax, n, off, A, ipiv, pivoff, info, want_inverse);
+#undef LU_TILE
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
   return BOSSS_OK;
@@ -627,8 +757,8 @@ int bosss_hip_block_lu_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t
                            int32_t* ipiv, const int64_t* pivoff, int32_t* info, int32_t want_inverse, void* stream) {
   if (nblk <= 0) return BOSSS_OK;
   if (int rc = lu_check_args(ctx, nmax, n, off, A, ipiv, pivoff)) return rc;
-  if (nmax <= LU_MAXN) return lu_launch_lds(nblk, nmax, n, off, A, ipiv, pivoff, info, want_inverse, (hipStream_t)stream);
-  if (want_inverse) { g_lu_err = "block too large for an in-LDS inverse (n <= 100)"; return BOSSS_ERR_UNSUPPORTED; }
+  if (nmax <= LU_MAXN) return lu_launch_reg(nblk, nmax, n, off, A, ipiv, pivoff, info, want_inverse, (hipStream_t)stream);
+  if (want_inverse) { g_lu_err = "inverses are limited to blocks of 128 DOF"; return BOSSS_ERR_UNSUPPORTED; }
   hipStream_t s = (hipStream_t)stream;
   if (nmax >= LU_SPLIT_MINN && nblk < LU_SPLIT_MAXBLK) {
     // few large blocks: two launches per pair of columns, the trailing update over many workgroups
@@ -651,8 +781,8 @@ int bosss_hip_block_lu_noalloc_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const
                                    int32_t* ipiv, const int64_t* pivoff, int32_t* info, int32_t want_inverse, void* stream) {
   if (nblk <= 0) return BOSSS_OK;
   if (int rc = lu_check_args(ctx, nmax, n, off, A, ipiv, pivoff)) return rc;
-  if (nmax <= LU_MAXN) return lu_launch_lds(nblk, nmax, n, off, A, ipiv, pivoff, info, want_inverse, (hipStream_t)stream);
-  if (want_inverse) { g_lu_err = "block too large for an in-LDS inverse (n <= 100)"; return BOSSS_ERR_UNSUPPORTED; }
+  if (nmax <= LU_MAXN) return lu_launch_reg(nblk, nmax, n, off, A, ipiv, pivoff, info, want_inverse, (hipStream_t)stream);
+  if (want_inverse) { g_lu_err = "inverses are limited to blocks of 128 DOF"; return BOSSS_ERR_UNSUPPORTED; }
   if (nblk > LU_NOALLOC_MAXBLK) {
     g_lu_err = "bosss_hip_block_lu_noalloc_dev: " + std::to_string(nblk) + " blocks above " + std::to_string(LU_MAXN) +
                " DOF exceed its scratch (at most " + std::to_string(LU_NOALLOC_MAXBLK) + " per call)";

@@ -29,14 +29,15 @@ int ctx_device_stream(bosss_ctx* ctx, int* device, hipStream_t* stream);
 namespace {
 
 constexpr int KR_THREADS = 256;
-constexpr int KR_MAXBLK = 1024;     // workgroups of the streaming kernels (fewer for short vectors)
+constexpr int KR_MAXBLK = 2048;     // workgroups of the streaming kernels (fewer for short vectors)
 constexpr int KR_MAXM = 256;        // largest restart length
-constexpr int KR_CHUNK = 8;         // basis vectors per pass of the dot kernel
+constexpr int KR_CHUNK = 16;        // basis vectors per pass of the dot kernel
+constexpr int KR_FUSED = 32;        // longest basis whose second-pass dots ride on the first update
 
 thread_local std::string g_kr_err;
 
 int kr_blocks(int64_t n) {
-  int64_t b = (n + KR_THREADS - 1) / KR_THREADS;
+  int64_t b = (n + 4 * KR_THREADS - 1) / (4 * KR_THREADS);
   return (int)(b < 1 ? 1 : (b > KR_MAXBLK ? KR_MAXBLK : b));
 }
 
@@ -54,12 +55,42 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Workgroup sum in a fixed order (wavefront trees, then the wavefronts in index order); the
+// result is valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int r = 0; r < KR_THREADS / 64; r++) s += red[r];
+  return s;
+}
+
+// Stores the workgroup sums of acc[0..k1-i0) as part[i0 + q][blockIdx.x].
+template <int C>
+__device__ __forceinline__ void store_partials(const double (&acc)[C], int i0, int k1, double (*red)[C], double* part) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < C; q++) {
+    const double s = wave_sum(acc[q]);
+    if (lane == 0) red[wv][q] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < C && i0 + (int)threadIdx.x < k1) {
+    double s = 0.0;
+    for (int r = 0; r < KR_THREADS / 64; r++) s += red[r][threadIdx.x];
+    part[(int64_t)(i0 + threadIdx.x) * KR_MAXBLK + blockIdx.x] = s;
+  }
+  __syncthreads();
+}
+
 // part[i][blk] = sum over the workgroup's share of t of V[i][t] w[t], i < k1
 template <typename Real>
 __global__ void __launch_bounds__(KR_THREADS) k_mdot(int64_t n, int k1, const Real* __restrict__ V, int64_t ldv,
                                                      const Real* __restrict__ w, double* __restrict__ part) {
   __shared__ double red[KR_THREADS / 64][KR_CHUNK];
-  const int nb = gridDim.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nb = gridDim.x;
   for (int i0 = 0; i0 < k1; i0 += KR_CHUNK) {
     double acc[KR_CHUNK];
 #pragma unroll
@@ -70,30 +101,39 @@ __global__ void __launch_bounds__(KR_THREADS) k_mdot(int64_t n, int k1, const Re
       for (int q = 0; q < KR_CHUNK; q++)
         if (i0 + q < k1) acc[q] += (double)V[(int64_t)(i0 + q) * ldv + t] * wt;
     }
-#pragma unroll
-    for (int q = 0; q < KR_CHUNK; q++) {
-      const double s = wave_sum(acc[q]);
-      if (lane == 0) red[wv][q] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < KR_CHUNK && i0 + (int)threadIdx.x < k1) {
-      double s = 0.0;
-      for (int r = 0; r < KR_THREADS / 64; r++) s += red[r][threadIdx.x];
-      part[(int64_t)(i0 + threadIdx.x) * KR_MAXBLK + blockIdx.x] = s;
-    }
-    __syncthreads();
+    store_partials<KR_CHUNK>(acc, i0, k1, red, part);
   }
 }
 
-// h[i] = sum_blk part[i][blk]; H[i + k ldh] = h[i] (pass 0) or += h[i] (pass 1)
-__global__ void __launch_bounds__(KR_THREADS) k_hreduce(int nb, int k1, const double* __restrict__ part, double* __restrict__ h,
+// h[i] = sum_blk part[i][blk]; H[i + k ldh] = h[i] (pass 0) or += h[i] (pass 1).  One workgroup per i.
+__global__ void __launch_bounds__(KR_THREADS) k_hreduce(int nb, const double* __restrict__ part, double* __restrict__ h,
                                                         double* __restrict__ Hk, int accumulate) {
-  const int i = threadIdx.x;
-  if (i >= k1) return;
+  __shared__ double red[KR_THREADS / 64];
+  const int i = blockIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nb; b++) s += part[(int64_t)i * KR_MAXBLK + b];
-  h[i] = s;
-  Hk[i] = accumulate ? Hk[i] + s : s;
+#pragma unroll 8
+  for (int b = threadIdx.x; b < nb; b += KR_THREADS) s += part[(int64_t)i * KR_MAXBLK + b];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) {
+    h[i] = s;
+    Hk[i] = accumulate ? Hk[i] + s : s;
+  }
+}
+
+// s = sum_{i<k1} V[i][t] h[i], in index order, with the loads of eight basis vectors issued together
+template <typename Real>
+__device__ __forceinline__ double basis_combination(int k1, const Real* __restrict__ V, int64_t ldv, int64_t t, const double* hs) {
+  double s = 0.0;
+  int i = 0;
+  for (; i + 8 <= k1; i += 8) {
+    Real v[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) v[q] = V[(int64_t)(i + q) * ldv + t];
+#pragma unroll
+    for (int q = 0; q < 8; q++) s += (double)v[q] * hs[i + q];
+  }
+  for (; i < k1; i++) s += (double)V[(int64_t)i * ldv + t] * hs[i];
+  return s;
 }
 
 // w[t] -= sum_{i<k1} V[i][t] h[i]; with norm: npart[blk] = workgroup's sum of the new w[t]^2
@@ -106,35 +146,60 @@ __global__ void __launch_bounds__(KR_THREADS) k_mupdate(int64_t n, int k1, const
   __syncthreads();
   double nrm = 0.0;
   for (int64_t t = (int64_t)blockIdx.x * KR_THREADS + threadIdx.x; t < n; t += (int64_t)gridDim.x * KR_THREADS) {
-    double s = 0.0;
-    for (int i = 0; i < k1; i++) s += (double)V[(int64_t)i * ldv + t] * hs[i];
-    const Real x = (Real)((double)w[t] - s);
+    const Real x = (Real)((double)w[t] - basis_combination(k1, V, ldv, t, hs));
     w[t] = x;
     nrm += (double)x * (double)x;
   }
   if (npart) {
-    nrm = wave_sum(nrm);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = nrm;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int r = 0; r < KR_THREADS / 64; r++) s += red[r];
-      npart[blockIdx.x] = s;
-    }
+    nrm = block_sum(nrm, red);
+    if (threadIdx.x == 0) npart[blockIdx.x] = nrm;
   }
+}
+
+// The first Gram-Schmidt update fused with the second pass's dot products: w[t] -= sum_i V[i][t] h[i],
+// then part[i][blk] = workgroup's sum of V[i][t] w[t] over the updated w, with the basis values kept
+// in registers between the two (k1 <= C).  Same sums, in the same order, as k_mupdate followed by
+// k_mdot on the same grid.
+template <typename Real, int C>
+__global__ void __launch_bounds__(KR_THREADS) k_mupdate_dot(int64_t n, int k1, const Real* __restrict__ V, int64_t ldv,
+                                                            const double* __restrict__ h, Real* __restrict__ w,
+                                                            double* __restrict__ part) {
+  __shared__ double hs[C];
+  __shared__ double red[KR_THREADS / 64][C];
+  for (int i = threadIdx.x; i < C; i += KR_THREADS) hs[i] = i < k1 ? h[i] : 0.0;
+  __syncthreads();
+  double acc[C];
+#pragma unroll
+  for (int q = 0; q < C; q++) acc[q] = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * KR_THREADS + threadIdx.x; t < n; t += (int64_t)gridDim.x * KR_THREADS) {
+    Real v[C];
+#pragma unroll
+    for (int q = 0; q < C; q++) v[q] = q < k1 ? V[(int64_t)q * ldv + t] : Real(0);
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < C; q++)
+      if (q < k1) s += (double)v[q] * hs[q];
+    const Real x = (Real)((double)w[t] - s);
+    w[t] = x;
+    const double xd = (double)x;
+#pragma unroll
+    for (int q = 0; q < C; q++) acc[q] += (double)v[q] * xd;
+  }
+  store_partials<C>(acc, 0, k1, red, part);
 }
 
 // |w| from the norm partials, column k of H completed and rotated, rotation k formed, g updated,
 // res[k] = |g[k+1]| (the residual estimate after step k), and 1/|w| (0 on breakdown) for the new
-// basis vector.  One wavefront.
-__global__ void __launch_bounds__(64) k_finish(int nb, int k, int ldh, const double* __restrict__ npart, double* __restrict__ H,
-                                               double* __restrict__ cs, double* __restrict__ sn, double* __restrict__ g,
-                                               double* __restrict__ res, double* __restrict__ scal) {
-  const int lane = threadIdx.x;
+// basis vector.  One workgroup; thread 0 does the rotations.
+__global__ void __launch_bounds__(KR_THREADS) k_finish(int nb, int k, int ldh, const double* __restrict__ npart, double* __restrict__ H,
+                                                       double* __restrict__ cs, double* __restrict__ sn, double* __restrict__ g,
+                                                       double* __restrict__ res, double* __restrict__ scal) {
+  __shared__ double red[KR_THREADS / 64];
   double s = 0.0;
-  for (int b = lane; b < nb; b += 64) s += npart[b];
-  s = wave_sum(s);
-  if (lane != 0) return;
+#pragma unroll 8
+  for (int b = threadIdx.x; b < nb; b += KR_THREADS) s += npart[b];
+  s = block_sum(s, red);
+  if (threadIdx.x != 0) return;
   const double hn = sqrt(s);
   double* Hk = H + (int64_t)k * ldh;
   Hk[k + 1] = hn;
@@ -182,9 +247,7 @@ __global__ void __launch_bounds__(KR_THREADS) k_xupdate(int64_t n, int k1, const
   for (int i = threadIdx.x; i < k1; i += KR_THREADS) ys[i] = y[i];
   __syncthreads();
   for (int64_t t = (int64_t)blockIdx.x * KR_THREADS + threadIdx.x; t < n; t += (int64_t)gridDim.x * KR_THREADS) {
-    double s = 0.0;
-    for (int i = 0; i < k1; i++) s += (double)Z[(int64_t)i * ldz + t] * ys[i];
-    x[t] = (Real)((double)x[t] + s);
+    x[t] = (Real)((double)x[t] + basis_combination(k1, Z, ldz, t, ys));
   }
 }
 
@@ -225,14 +288,26 @@ void arnoldi(int64_t n, int k, int ldh, Real* V, int64_t ldv, Real* w, double* H
   double* h = work + WS_H;
   double* scal = work + WS_S;
   double* Hk = H + (int64_t)k * ldh;
-  for (int pass = 0; pass < 2; pass++) {
-    hipLaunchKernelGGL(k_mdot<Real>, dim3(nb), dim3(KR_THREADS), 0, s, n, k1, (const Real*)V, ldv, (const Real*)w, part);
-    hipLaunchKernelGGL(k_hreduce, dim3(1), dim3(KR_THREADS), 0, s, nb, k1, (const double*)part, h, Hk, pass);
-    hipLaunchKernelGGL(k_mupdate<Real>, dim3(nb), dim3(KR_THREADS), 0, s, n, k1, (const Real*)V, ldv, (const double*)h, w,
-                       pass == 1 ? npart : nullptr);
+  const dim3 grid(nb), blk(KR_THREADS);
+  // pass 0: h = V^T w, H[:, k] = h
+  hipLaunchKernelGGL(k_mdot<Real>, grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const Real*)w, part);
+  hipLaunchKernelGGL(k_hreduce, dim3(k1), blk, 0, s, nb, (const double*)part, h, Hk, 0);
+  // w -= V h, and the pass-1 dots V^T w of the updated w
+  if (k1 <= 8)
+    hipLaunchKernelGGL((k_mupdate_dot<Real, 8>), grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const double*)h, w, part);
+  else if (k1 <= 16)
+    hipLaunchKernelGGL((k_mupdate_dot<Real, 16>), grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const double*)h, w, part);
+  else if (k1 <= KR_FUSED)
+    hipLaunchKernelGGL((k_mupdate_dot<Real, KR_FUSED>), grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const double*)h, w, part);
+  else {
+    hipLaunchKernelGGL(k_mupdate<Real>, grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const double*)h, w, nullptr);
+    hipLaunchKernelGGL(k_mdot<Real>, grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const Real*)w, part);
   }
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, s, nb, k, ldh, (const double*)npart, H, cs, sn, g, res, scal);
-  hipLaunchKernelGGL(k_scale_to<Real>, dim3(nb), dim3(KR_THREADS), 0, s, n, (const Real*)w, (const double*)scal,
+  // pass 1: H[:, k] += h, w -= V h with the norm partials of the result
+  hipLaunchKernelGGL(k_hreduce, dim3(k1), blk, 0, s, nb, (const double*)part, h, Hk, 1);
+  hipLaunchKernelGGL(k_mupdate<Real>, grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const double*)h, w, npart);
+  hipLaunchKernelGGL(k_finish, dim3(1), blk, 0, s, nb, k, ldh, (const double*)npart, H, cs, sn, g, res, scal);
+  hipLaunchKernelGGL(k_scale_to<Real>, grid, blk, 0, s, n, (const Real*)w, (const double*)scal,
                      V + (int64_t)(k + 1) * ldv);
 }
 

@@ -181,17 +181,20 @@ int bosss_hip_op_spmv_tiles(const bosss_op* op, int32_t* chosen, int32_t* ncand,
  *      IMatrixExtensions.cs:2128-2230, and Schwarz's LAPACK block solver for blocks up to
  *      4096 DOF, Schwarz.cs:827-915; above that the reference switches to PARDISO).
  *      Blocks are dense row-major at element offsets off[b], sizes n[b] <= nmax <= 4096:
- *      nmax <= 128 stages each block in LDS (one wavefront per block); larger nmax
+ *      nmax <= 128 holds each block in registers (up to 20 DOF several blocks per
+ *      wavefront, above that one 256-thread workgroup per block); larger nmax
  *      factors in place in global memory with the same pivoting and operation order
  *      (one 256-thread workgroup per block, or, for fewer than 256 blocks with nmax >= 512,
  *      two launches per pair of columns with the trailing update over many workgroups;
  *      that form synchronises the stream before it returns).
  *      ipiv (1-based, LAPACK convention) at pivoff[b]; info[b] as DGETRF's INFO
  *      (0, or j+1 for the first zero pivot; -1 if n[b] exceeds nmax).  The inverse
- *      (want_inverse) needs nmax <= 100.  In
+ *      (want_inverse) needs nmax <= 128.  In
  *      bosss_hip_block_lu_dev, ipiv (with pivoff) and info may be NULL.  ctx selects the
  *      device (NULL: the calling thread's current device); stream must belong to it.
- *      want_inverse != 0 overwrites each block with its inverse (DGETRI), n <= 100.
+ *      want_inverse != 0 overwrites each block with its inverse (DGETRI's result, above
+ *      20 DOF by in-place Gauss-Jordan with the same pivot rows), n <= 128; a singular
+ *      block keeps its LU factors instead, with INFO set.
  *      All arrays are device memory; one launch covers the whole batch. ---- */
 int bosss_hip_block_lu_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t* n, const int64_t* off, double* A,
                            int32_t* ipiv, const int64_t* pivoff, int32_t* info, int32_t want_inverse, void* stream);

@@ -76,7 +76,7 @@ def test_host_grid_on_graded_nodes_matches_oracle(D, per):
 
 def test_block_lu_refuses_sizes_before_any_launch():
     """The batched LU checks its sizes on the host before touching the device: nmax above
-    4096 is an argument error, an inverse of blocks above 100 DOF is unsupported; the error
+    4096 is an argument error, an inverse of blocks above 128 DOF is unsupported; the error
     text names the limit (host memory stands in for the device arrays: nothing is read)."""
     L = _lib.lib()
     n = np.array([5000], np.int32)
@@ -87,7 +87,7 @@ def test_block_lu_refuses_sizes_before_any_launch():
     rc = L.bosss_hip_block_lu_dev(None, 1, 5000, ptr(n), ptr(off), ptr(A), None, None, ptr(info), 0, None)
     assert rc == -1 and "4096" in L.bosss_hip_block_last_error().decode()
     rc = L.bosss_hip_block_lu_dev(None, 1, 200, ptr(n), ptr(off), ptr(A), None, None, ptr(info), 1, None)
-    assert rc == -3 and "100" in L.bosss_hip_block_last_error().decode()
+    assert rc == -3 and "128" in L.bosss_hip_block_last_error().decode()
     rc = L.bosss_hip_block_lu_dev(None, 1, 20, None, ptr(off), ptr(A), None, None, ptr(info), 0, None)
     assert rc == -1
 

@@ -139,7 +139,7 @@ def test_large_block_zero_pivot_and_inverse_limit(sizes, col):
     # getf2 skips the zero column and goes on: the factors after it match too
     assert np.abs(A.cpu().numpy() - LUref).max() < 1e-12 * np.abs(LUref).max()
     with pytest.raises(RuntimeError):
-        bb.factor(A, None, info, inverse=True)   # inverses are limited to 100 DOF
+        bb.factor(A, None, info, inverse=True)   # inverses are limited to 128 DOF
 
 
 def zero_diagonal_block(n, j, seed):
@@ -246,3 +246,49 @@ def test_nan_row_follows_idamax(sizes):
     n1 = sizes[1]
     b1 = slice(off[1], off[1] + n1 * n1)                    # the second block is untouched by the NaN
     assert np.abs(LU[b1] - LUref[b1]).max() < 1e-12 * np.abs(LUref[b1]).max()
+
+
+@pytest.mark.parametrize("sizes", [[21, 32, 33, 64, 65, 100, 112, 113, 128], [100] * 300])
+def test_workgroup_tile_blocks(sizes):
+    """Blocks of 21..128 DOF (one workgroup per block, the block in registers, row exchanges as
+    bookkeeping): pivots and INFO equal the oracle's getf2 and the factors agree to rounding, with a
+    zero column, a zero diagonal entry with a pivot below, exactly tied pivot candidates in column 0
+    and a NaN below the diagonal; the inverses (in-place Gauss-Jordan) satisfy A inv(A) = I; a
+    singular block asked for its inverse keeps getf2's factors and INFO instead."""
+    import torch
+    from bosss_amd.blocks import BlockBatch
+    say("case workgroup-tile blocks", len(sizes), "blocks, max", max(sizes))
+    blocks = make_blocks(sizes, 41)
+    blocks[0][:, 7] = 0.0                                  # zero pivot at column 7
+    blocks[1] = zero_diagonal_block(sizes[1], 5, 3)        # zero diagonal, pivot swapped in
+    blocks[2][:, 0] = np.where(np.arange(sizes[2]) % 3 == 1, 4.0, -4.0) * (np.arange(sizes[2]) % 2 * 2 - 1)
+    blocks[3][9, 0] = np.nan                               # NaN row
+    bb = BlockBatch(sizes)
+    A0 = np.concatenate([b.ravel() for b in blocks])
+    A = torch.from_numpy(A0.copy()).to(bb.dev)
+    ipiv = torch.zeros(bb.ndof, dtype=torch.int32, device=bb.dev)
+    info = torch.zeros(bb.nblk, dtype=torch.int32, device=bb.dev)
+    bb.factor(A, ipiv, info)
+    torch.cuda.synchronize()
+    LUref, pivref, inforef, off, poff = ref_block_lu(sizes, blocks)
+    assert inforef[0] == 8 and pivref[poff[2]] == 0 and pivref[poff[1] + 5] != 5
+    assert (info.cpu().numpy() == inforef).all()
+    assert (ipiv.cpu().numpy() == pivref + 1).all()
+    LU = A.cpu().numpy()
+    fin = np.isfinite(LUref)
+    assert (np.isfinite(LU) == fin).all()
+    assert np.abs(LU[fin] - LUref[fin]).max() < 1e-12 * np.abs(LUref[fin]).max()
+    # inverses: every block; the singular one keeps its factors and INFO
+    A = torch.from_numpy(A0.copy()).to(bb.dev)
+    info.zero_()
+    bb.factor(A, None, info, inverse=True)
+    torch.cuda.synchronize()
+    inv = A.cpu().numpy()
+    assert (info.cpu().numpy() == inforef).all()
+    s0 = slice(off[0], off[0] + sizes[0] ** 2)
+    assert np.abs(inv[s0] - LUref[s0]).max() < 1e-12 * np.abs(LUref[s0]).max()
+    for b, n in enumerate(sizes):
+        if b in (0, 3):
+            continue
+        Ainv = inv[off[b]:off[b] + n * n].reshape(n, n)
+        assert np.abs(blocks[b] @ Ainv - np.eye(n)).max() < 1e-12 * n * max(1.0, np.abs(Ainv).max()), (b, n)
