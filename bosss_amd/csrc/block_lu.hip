@@ -245,25 +245,83 @@ __global__ void __launch_bounds__(256) k_block_lu_tile(int nblk, int nmax, const
 
 // Small blocks (n <= 20: the 10- and 20-DOF cell blocks of XDG and p <= 2 scalar operators): G lanes
 // per block (G = 16 or 32, so 64 / G blocks per wavefront), lane r of a group holds row r of its
-// block in registers.  DGETF2 with the operation order of the tile kernel's LU form -- pivot = first row with the
-// largest |a| in the column (group argmax over the lanes' rows, ties to the smaller row; as in
-// IDAMAX a NaN on the diagonal is kept and any other NaN is never chosen), row swap (register exchange through lane shuffles), column scaled by the reciprocal
-// pivot, rank-1 update a_rc -= l_r u_jc with u_jc broadcast from the pivot row's lane -- so the
-// factors and pivots are those of the LDS kernel.  The inverse (want_inverse) solves L U X = P for
-// all columns at once with the rows of X in registers: the same result as DGETRI up to rounding.
-// No lane returns early: every shuffle has all lanes of the wavefront taking part, and column
-// loops that no block of the launch needs (c >= nmax) are skipped uniformly.
-template <int G, typename Real>
-__device__ __forceinline__ Real grp_shfl(Real v, int src_lane) {
-  return __shfl(v, src_lane, 64);
+// block in registers for the whole factorisation.  Row exchanges are bookkeeping, as in the
+// workgroup-tile form: each lane carries the position its row has been given.  The pivot is the
+// unused row with the largest |a| (ties to the smaller position, a NaN in position j never beaten,
+// any other NaN never chosen: IDAMAX in getf2's row order), found by a shuffle argmax inside the
+// group; the pivot row goes to the group's LDS slot and every lane reads it back (a group is part
+// of one wavefront, whose LDS operations complete in order: no barrier).  LU: getf2's operations
+// in getf2's order (multiplier a * (1 / pivot), rank-one update of the columns after j) on the
+// rows after the pivot in position order.  Inverse: Gauss-Jordan in place, written out with both
+// permutations applied; a zero pivot makes the group factor its block from the original values
+// instead (LU and INFO).  The step loop is unrolled, so column j is a register index known at
+// compile time.  No lane returns early: every shuffle has all lanes of its group taking part.
+template <typename Real, int G, int NM, bool GJ>
+__device__ __forceinline__ int rows_eliminate(Real (&a)[NM], int r, int n, int& pos, Real* row_slot, int* pr, int* pp, int base) {
+  int inf = 0;
+  bool dead = false;   // Gauss-Jordan stops at a zero pivot
+#pragma unroll
+  for (int j = 0; j < NM; j++) {
+    if (j < n && !dead) {   // uniform over the group
+      Real v = Real(-1);
+      if (r < n && pos >= j) {
+        v = fabs(a[j]);
+        if (pos == j && v != v) v = Real(INFINITY);   // idamax: a NaN first element is never beaten
+        if (!(v >= Real(0))) v = Real(-1);              // any other NaN is never a candidate
+      }
+      int key = v >= Real(0) ? (pos << 6) | r : 0x7fffffff;   // position first (unique), then the lane
+#pragma unroll
+      for (int o = G / 2; o >= 1; o >>= 1) {
+        const Real ov = __shfl_xor(v, o, 64);
+        const int ok = __shfl_xor(key, o, 64);
+        if (ov > v || (ov == v && ok < key)) { v = ov; key = ok; }
+      }
+      // the row in position j is always a candidate (|a| >= 0, or a NaN made infinite), so key is set
+      const int p = key & 63, ps = key >> 6;
+      const Real apj = __shfl(a[j], base + p, 64);
+      if (r == 0) { pr[j] = p; pp[j] = ps; }
+      if (!(apj != Real(0))) {   // zero pivot (NaN passes, as in DGETF2)
+        if (!inf) inf = j + 1;
+        if (GJ) dead = true;
+        continue;                // no exchange: the row in position j is done all the same
+      }
+      if (r == p) pos = j;
+      else if (pos == j) pos = ps;
+      const Real pinv = Real(1) / apj;
+      if (r == p) {
+#pragma unroll
+        for (int c = 0; c < NM; c++) row_slot[c] = GJ ? (c == j ? pinv : a[c] * pinv) : a[c];
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (GJ) {
+        const Real f = a[j];
+        a[j] = Real(0);          // becomes -f / pivot through the update
+#pragma unroll
+        for (int c = 0; c < NM; c++) a[c] -= f * row_slot[c];
+        if (r == p) {
+#pragma unroll
+          for (int c = 0; c < NM; c++) a[c] = row_slot[c];
+        }
+      } else if (pos > j) {      // rows after the pivot in position order
+        const Real l = a[j] * pinv;
+        a[j] = l;
+#pragma unroll
+        for (int c = j + 1; c < NM; c++) a[c] -= l * row_slot[c];
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  return inf;
 }
 
 template <typename Real, int G, int NM>
 __global__ void __launch_bounds__(256) k_block_lu_rows(int nblk, int nmax, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
                                                        Real* __restrict__ A, int32_t* __restrict__ ipiv, const int64_t* __restrict__ pivoff,
                                                        int32_t* __restrict__ info, int want_inverse) {
-  const int r = threadIdx.x % G;                       // row of the block this lane holds
-  const int b = blockIdx.x * (256 / G) + threadIdx.x / G;
+  __shared__ Real s_row[256 / G][NM];
+  __shared__ int s_pr[256 / G][NM], s_pp[256 / G][NM];   // per step: pivot row (physical), its position before the exchange
+  const int r = threadIdx.x % G, grp = threadIdx.x / G;   // row of the block this lane holds, group in the workgroup
+  const int b = blockIdx.x * (256 / G) + grp;
   const int base = (threadIdx.x & 63) - r;             // first lane of the group in the wavefront
   const bool inb = b < nblk;
   const int n0 = inb ? nsz[b] : 0;
@@ -271,102 +329,41 @@ __global__ void __launch_bounds__(256) k_block_lu_rows(int nblk, int nmax, const
   const bool ok = inb && n0 > 0 && n0 <= nmax;
   const int n = ok ? n0 : 0;
   Real* g = ok ? A + off[b] : A;
-  Real row[NM];
+  Real a[NM];
 #pragma unroll
-  for (int c = 0; c < NM; c++) row[c] = (r < n && c < n) ? g[(size_t)r * n + c] : Real(0);
-  int inf = 0, mypiv = r;
-  int pv[NM];                                          // the group's pivot rows (same in every lane)
+  for (int c = 0; c < NM; c++) a[c] = (r < n && c < n) ? g[(size_t)r * n + c] : Real(0);
+  int pos = r, inf = 0;
+  bool inverse = false;
+  if (want_inverse) {
+    inf = rows_eliminate<Real, G, NM, true>(a, r, n, pos, s_row[grp], s_pr[grp], s_pp[grp], base);
+    inverse = inf == 0;
+    if (!inverse) {   // singular (uniform over the group): factor the original block, LU and INFO
 #pragma unroll
-  for (int j = 0; j < NM; j++) {
-    pv[j] = j;
-    const bool act = j < n;
-    Real v = (act && r >= j && r < n) ? fabs(row[j]) : Real(-1);
-    if (act && r == j && v != v) v = Real(INFINITY);   // idamax: a NaN first element is never beaten
-    int bi = G;
-    if (v >= Real(0)) bi = r; else v = Real(-1);       // any other NaN is never a candidate
-#pragma unroll
-    for (int o = G / 2; o >= 1; o >>= 1) {
-      const Real ov = __shfl_xor(v, o, 64);
-      const int oi = __shfl_xor(bi, o, 64);
-      if (ov > v || (ov == v && oi < bi)) { v = ov; bi = oi; }
-    }
-    const int p = bi < G ? bi : j;                     // no comparable candidate (all NaN): the diagonal
-    pv[j] = p;
-    if (r == j) mypiv = p;
-    const Real apj = grp_shfl<G>(row[j], base + p);
-    const bool elim = act && (apj != Real(0));          // NaN passes, as DGETF2's A(JP,J).NE.ZERO
-    if (act && !elim && !inf) inf = j + 1;
-    const int src = (elim && r == j) ? p : ((elim && r == p) ? j : r);
-    if (__any(src != r)) {
-#pragma unroll
-      for (int c = 0; c < NM; c++) row[c] = grp_shfl<G>(row[c], base + src);
-    }
-    const Real inv = Real(1) / grp_shfl<G>(row[j], base + j);
-    const bool mine = elim && r > j && r < n;
-    const Real l = row[j] * inv;
-    if (mine) row[j] = l;
-#pragma unroll
-    for (int c = j + 1; c < NM; c++) {
-      const Real u = grp_shfl<G>(row[c], base + j);
-      if (mine) row[c] -= l * u;
+      for (int c = 0; c < NM; c++) a[c] = (r < n && c < n) ? g[(size_t)r * n + c] : Real(0);
+      pos = r;
     }
   }
-  if (want_inverse && __any(ok)) {
-    // X = (LU)^-1 P: rows of the identity permuted by the pivots, then L and U solved for all columns.
-    Real x[NM];
-#pragma unroll
-    for (int c = 0; c < NM; c++) x[c] = (r == c) ? Real(1) : Real(0);
-#pragma unroll
-    for (int j = 0; j < NM; j++) {
-      const int p = pv[j];
-      const int src = (j < n && r == j) ? p : ((j < n && r == p) ? j : r);
-      if (__any(src != r)) {
-#pragma unroll
-        for (int c = 0; c < NM; c++) x[c] = grp_shfl<G>(x[c], base + src);
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < NM; j++) {                     // forward: unit lower L
-      const bool mine = r > j && r < n;
-      const Real lrj = row[j];
-#pragma unroll
-      for (int c = 0; c < NM; c++) {
-        const Real xj = grp_shfl<G>(x[c], base + j);
-        if (mine) x[c] -= lrj * xj;
-      }
-    }
-#pragma unroll
-    for (int j = NM - 1; j >= 0; j--) {                // backward: U
-      const Real ujj = grp_shfl<G>(row[j], base + j);
-      if (r == j && r < n) {
-#pragma unroll
-        for (int c = 0; c < NM; c++) x[c] = x[c] / ujj;
-      }
-      const bool mine = r < j && j < n;
-      const Real urj = row[j];
-#pragma unroll
-      for (int c = 0; c < NM; c++) {
-        const Real xj = grp_shfl<G>(x[c], base + j);
-        if (mine) x[c] -= urj * xj;
-      }
-    }
-    if (inf == 0) {
-#pragma unroll
-      for (int c = 0; c < NM; c++) row[c] = x[c];
-    }
-  }
+  if (!inverse) inf = rows_eliminate<Real, G, NM, false>(a, r, n, pos, s_row[grp], s_pr[grp], s_pp[grp], base);
+  __builtin_amdgcn_wave_barrier();
   if (ok && r < n) {
+    Real* out = g + (size_t)pos * n;
+    if (inverse) {
 #pragma unroll
-    for (int c = 0; c < NM; c++)
-      if (c < n) g[(size_t)r * n + c] = row[c];
-    if (ipiv) ipiv[pivoff[b] + r] = mypiv + 1;
+      for (int c = 0; c < NM; c++)
+        if (c < n) out[s_pr[grp][c]] = a[c];   // X[pos[r]][row placed in position c] = a[r][c]
+    } else {
+#pragma unroll
+      for (int c = 0; c < NM; c++)
+        if (c < n) out[c] = a[c];
+    }
+    if (ipiv) ipiv[pivoff[b] + r] = s_pp[grp][r] + 1;
   }
   if (ok && r == 0 && info) info[b] = inf;
 }
 
 // Blocks above LU_MAXN (the multi-cell blocks of Schwarz smoothers, up to LU_BIG_MAXN): one
-// workgroup of 256 threads per block, DGETF2 in place in global memory with the operation order
-// of the small-block kernel -- pivot = first row with the largest |a| in the column (workgroup
+// workgroup of 256 threads per block, DGETF2 in place in global memory with getf2's operation order
+// -- pivot = first row with the largest |a| in the column (workgroup
 // argmax, ties to the smaller row), row swap, column scaled by the reciprocal pivot, rank-1
 // update of the trailing rows (one row per wavefront, lanes along the row: coalesced).  A zero
 // pivot sets INFO and the factorisation continues.  O(n^3) global-memory traffic per block,

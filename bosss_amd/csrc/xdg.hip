@@ -16,9 +16,11 @@
 //
 // One wavefront per quadrature item.  The basis (orthonormal Legendre products, the
 // reference's mode order given by exponents[N][D]) and its physical gradients are staged
-// for 64 nodes at a time in LDS; lanes own (test, trial) mode pairs.  Face items add into
-// blocks that other faces of the same cell also touch, so they use FP64 atomic adds; volume
-// and interface items own their sub-blocks.
+// for 64 nodes at a time in LDS; lanes own (test, trial) mode pairs.  Volume and interface
+// items own their sub-blocks, and so does a face item for its two off-diagonal sub-blocks.  The
+// diagonal sub-blocks (and boundary right-hand sides) that all faces of a cell touch are staged
+// per face item and added by one workgroup per cell in face-item order, so the assembled
+// matrix is the same bits on every run (no atomics).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -236,7 +238,7 @@ __global__ void __launch_bounds__(64) k_xdg_face(XArgs a, const int32_t* __restr
                                                  const double* __restrict__ nin, const double* __restrict__ nout,
                                                  const double* __restrict__ w, const double* __restrict__ normal,
                                                  const double* __restrict__ gD, double* __restrict__ blocks,
-                                                 double* __restrict__ affine) {
+                                                 double* __restrict__ stage, double* __restrict__ astage) {
   __shared__ NStage si, so;
   __shared__ double sw[XCHUNK], sg[XCHUNK];
   const int it = blockIdx.x, lane = threadIdx.x;
@@ -295,22 +297,38 @@ __global__ void __launch_bounds__(64) k_xdg_face(XArgs a, const int32_t* __restr
     __syncthreads();
   }
   const int oi = spec_off(a, ji, sp), oo = inner ? spec_off(a, jo, sp) : 0;
+  double* st = stage + (size_t)it * 2 * NP;
 #pragma unroll
   for (int i = 0; i < XPAIRS; i++) {
     const int pr = lane + 64 * i;
     if (pr < NP) {
       const int r = pr / N, c = pr % N;
+      st[pr] = acc[i][0];
       if (inner) {
-        unsafeAtomicAdd(block_at(a, blocks, ji, 0, oi + r, oi + c), acc[i][0]);
-        unsafeAtomicAdd(block_at(a, blocks, ji, fslot[2 * it], oi + r, oo + c), acc[i][1]);
-        unsafeAtomicAdd(block_at(a, blocks, jo, fslot[2 * it + 1], oo + r, oi + c), acc[i][2]);
-        unsafeAtomicAdd(block_at(a, blocks, jo, 0, oo + r, oo + c), acc[i][3]);
-      } else {
-        unsafeAtomicAdd(block_at(a, blocks, ji, 0, oi + r, oi + c), acc[i][0]);
+        // the off-diagonal sub-blocks belong to this item alone (one item per face and species)
+        *block_at(a, blocks, ji, fslot[2 * it], oi + r, oo + c) = acc[i][1];
+        *block_at(a, blocks, jo, fslot[2 * it + 1], oo + r, oi + c) = acc[i][2];
+        st[NP + pr] = acc[i][3];
       }
     }
   }
-  if (!inner && lane < N) unsafeAtomicAdd(affine + a.dof[ji] + oi + lane, aff);
+  if (!inner && lane < N) astage[(size_t)it * N + lane] = aff;
+}
+
+// Adds the staged diagonal contributions of every face item touching cell j (side 0: the item's
+// IN cell, side 1: its OUT cell) in item order, and the boundary items' right-hand sides.
+__global__ void __launch_bounds__(256) k_xdg_face_gather(XArgs a, const int32_t* __restrict__ goff, const int32_t* __restrict__ gent,
+                                                         const int32_t* __restrict__ cells, const int32_t* __restrict__ spec,
+                                                         const double* __restrict__ stage, const double* __restrict__ astage,
+                                                         double* __restrict__ blocks, double* __restrict__ affine) {
+  const int j = blockIdx.x, N = a.N, NP = N * N;
+  for (int e = goff[j]; e < goff[j + 1]; e++) {
+    const int it = gent[e] >> 1, side = gent[e] & 1;
+    const int o = spec_off(a, j, spec[it]);
+    const double* st = stage + ((size_t)it * 2 + side) * NP;
+    for (int pr = threadIdx.x; pr < NP; pr += blockDim.x) *block_at(a, blocks, j, 0, o + pr / N, o + pr % N) += st[pr];
+    if (cells[2 * it + 1] < 0 && (int)threadIdx.x < N) affine[a.dof[j] + o + threadIdx.x] += astage[(size_t)it * N + threadIdx.x];
+  }
 }
 
 __global__ void __launch_bounds__(64) k_xdg_interface(XArgs a, const int32_t* __restrict__ cell, const int64_t* __restrict__ off,
@@ -759,6 +777,10 @@ struct bosss_xdg {
   int32_t *v_cell, *v_spec, *f_cells, *f_slot, *f_spec, *i_cell;
   int64_t *v_off, *f_off, *i_off;
   double *v_nodes, *v_w, *f_nin, *f_nout, *f_w, *f_normal, *f_gD, *i_nodes, *i_w, *i_normal;
+  // face items touching each cell (CSR, entries item * 2 + side, in item order) and the staged
+  // diagonal sub-blocks [nface][2][N N] and boundary right-hand sides [nface][N] of the items
+  int32_t *g_off = nullptr, *g_ent = nullptr;
+  double *f_stage = nullptr, *f_astage = nullptr;
   // solver workspace (allocated on first solve)
   bool ws = false;
   double *Dinv = nullptr, *Lchol = nullptr, *At = nullptr, *scal = nullptr, *partial = nullptr;
@@ -1092,6 +1114,17 @@ int bosss_hip_xdg_create(bosss_ctx* ctx, const bosss_xdg_desc* d, bosss_xdg** ou
       }
     }
   }
+  {
+    // each (cell, slot, species) pair of off-diagonal sub-blocks is written by one face item only
+    std::vector<int32_t> seen((size_t)J * S * 2, -1);
+    for (int i = 0; i < d->nface; i++) {
+      const int ji = d->face_cells[2 * i], jo = d->face_cells[2 * i + 1];
+      if (jo < 0) continue;
+      int32_t& w = seen[((size_t)ji * S + d->face_slot[2 * i]) * 2 + d->face_spec[i]];
+      if (w >= 0) { delete x; return xfail(BOSSS_ERR_ARG, "face items %d and %d: same face and species", w, i); }
+      w = i;
+    }
+  }
   for (int i = 0; i < d->nifc; i++)
     if (d->ifc_cell[i] < 0 || d->ifc_cell[i] >= J) { delete x; return xfail(BOSSS_ERR_ARG, "interface item %d: bad cell", i); }
   const size_t KV = d->nvol ? (size_t)d->vol_off[d->nvol] : 0, KF = d->nface ? (size_t)d->face_off[d->nface] : 0,
@@ -1120,6 +1153,24 @@ int bosss_hip_xdg_create(bosss_ctx* ctx, const bosss_xdg_desc* d, bosss_xdg** ou
   if (d->face_gD) { UP(f_gD, d->face_gD, KF); } else { x->f_gD = nullptr; }
   UP(i_cell, d->ifc_cell, d->nifc); UP(i_off, d->ifc_off, d->nifc + 1); UP(i_nodes, d->ifc_nodes, KI * D);
   UP(i_w, d->ifc_w, KI); UP(i_normal, d->ifc_normal, KI * D);
+  {
+    std::vector<int32_t> goff(J + 1, 0), gent;
+    for (int i = 0; i < d->nface; i++) {
+      goff[d->face_cells[2 * i] + 1]++;
+      if (d->face_cells[2 * i + 1] >= 0) goff[d->face_cells[2 * i + 1] + 1]++;
+    }
+    for (int j = 0; j < J; j++) goff[j + 1] += goff[j];
+    gent.resize(goff[J]);
+    std::vector<int32_t> fill(goff.begin(), goff.end() - 1);
+    for (int i = 0; i < d->nface; i++) {
+      gent[fill[d->face_cells[2 * i]]++] = 2 * i;
+      if (d->face_cells[2 * i + 1] >= 0) gent[fill[d->face_cells[2 * i + 1]]++] = 2 * i + 1;
+    }
+    UP(g_off, goff.data(), goff.size());
+    UP(g_ent, gent.data(), gent.size());
+    if (rc == BOSSS_OK && d->nface) rc = alloc(x, &x->f_stage, (size_t)d->nface * 2 * N * N);
+    if (rc == BOSSS_OK && d->nface) rc = alloc(x, &x->f_astage, (size_t)d->nface * N);
+  }
 #undef UP
   if (rc != BOSSS_OK) { delete x; return rc; }
   XArgs& a = x->a;
@@ -1154,9 +1205,12 @@ int bosss_hip_xdg_assemble_dev(bosss_xdg* x, double* blocks, double* affine, voi
   hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(x->ndof)), dim3(256), 0, s, x->ndof, affine, 0.0);
   if (x->nvol)
     hipLaunchKernelGGL(k_xdg_volume, dim3(x->nvol), dim3(64), 0, s, x->a, x->v_cell, x->v_spec, x->v_off, x->v_nodes, x->v_w, blocks, affine);
-  if (x->nface)
+  if (x->nface) {
     hipLaunchKernelGGL(k_xdg_face, dim3(x->nface), dim3(64), 0, s, x->a, x->f_cells, x->f_slot, x->f_spec, x->f_off, x->f_nin, x->f_nout,
-                       x->f_w, x->f_normal, x->f_gD, blocks, affine);
+                       x->f_w, x->f_normal, x->f_gD, blocks, x->f_stage, x->f_astage);
+    hipLaunchKernelGGL(k_xdg_face_gather, dim3(x->a.J), dim3(256), 0, s, x->a, x->g_off, x->g_ent, x->f_cells, x->f_spec, x->f_stage,
+                       x->f_astage, blocks, affine);
+  }
   if (x->nifc)
     hipLaunchKernelGGL(k_xdg_interface, dim3(x->nifc), dim3(64), 0, s, x->a, x->i_cell, x->i_off, x->i_nodes, x->i_w, x->i_normal, blocks);
   XHIP(hipGetLastError());

@@ -181,8 +181,8 @@ def test_register_kernel_small_blocks(sizes):
     """Blocks up to 20 DOF factor with their rows in registers, 64 / G blocks per wavefront (batches
     whose largest block is <= 16 and <= 20): pivots and factors equal the oracle's getf2,
     zero pivots (a zero column, even and odd) set INFO and the factorisation continues, a zero
-    diagonal entry with a non-zero pivot below is swapped in, a NaN is never chosen as pivot; the
-    inverses satisfy A inv(A) = I."""
+    diagonal entry with a non-zero pivot below is swapped in, exactly tied candidates go to the
+    first row, a NaN is never chosen as pivot; the inverses satisfy A inv(A) = I."""
     import torch
     from bosss_amd.blocks import BlockBatch
     say("case register-kernel blocks", len(sizes), "blocks, max", max(sizes))
@@ -192,6 +192,8 @@ def test_register_kernel_small_blocks(sizes):
     blocks[big[1]][:, 5] = 0.0                       # and at an odd one
     blocks[big[2]] = zero_diagonal_block(sizes[big[2]], 3, 5)
     blocks[big[3]][4, 0] = np.nan                    # NaN below the diagonal in column 0
+    nt = sizes[big[4]]                               # exactly tied pivot candidates in column 0
+    blocks[big[4]][:, 0] = np.where(np.arange(nt) % 3 == 1, 4.0, -4.0) * (np.arange(nt) % 2 * 2 - 1)
     bb = BlockBatch(sizes)
     A0 = np.concatenate([b.ravel() for b in blocks])
     A = torch.from_numpy(A0.copy()).to(bb.dev)
@@ -201,7 +203,7 @@ def test_register_kernel_small_blocks(sizes):
     torch.cuda.synchronize()
     LUref, pivref, inforef, off, poff = ref_block_lu(sizes, blocks)
     assert (info.cpu().numpy() == inforef).all()
-    assert inforef[big[0]] == 3 and inforef[big[1]] == 6 and inforef[big[2]] == 0
+    assert inforef[big[0]] == 3 and inforef[big[1]] == 6 and inforef[big[2]] == 0 and pivref[poff[big[4]]] == 0
     assert (ipiv.cpu().numpy() == pivref + 1).all()
     LU = A.cpu().numpy()
     fin = np.isfinite(LUref)

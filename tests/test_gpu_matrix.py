@@ -17,7 +17,8 @@ def say(*a):
 
 
 @pytest.mark.parametrize("form,D,n,p,per", [("sip", 2, 6, 2, False), ("advdiff", 3, 4, 3, True), ("sip", 3, 3, 2, False),
-                                            ("euler_hllc", 3, 3, 2, True)])
+                                            ("euler_hllc", 3, 3, 2, True),
+                                            ("sip", 2, 64, 2, False)])   # the 64 x 64 p = 2 Dirichlet Poisson problem
 def test_assembled_blocks_and_spmv(form, D, n, p, per):
     import torch
     from bosss_amd.grid import CartesianGrid

@@ -93,6 +93,11 @@ def test_xdg_assembly_and_spmv_match_oracle(case, arg, p):
     ea = np.abs(ah - aref).max() / max(np.abs(aref).max(), 1e-300)
     say("  blocks %.2e affine %.2e" % (eb, ea))
     assert eb < 1e-12 and ea < 1e-12
+    # no atomics: a second assembly gives the same bits
+    B2 = torch.full_like(B, float("nan"))
+    a2 = torch.full_like(a, float("nan"))
+    op.assemble_dev(B2, a2)
+    assert torch.equal(B2, B) and torch.equal(a2, a)
     # apply
     x = np.random.default_rng(7).uniform(-1, 1, op.ndof)
     y = torch.zeros_like(a)
@@ -170,3 +175,32 @@ def test_xdg_create_rejects_scattered_block_offsets(monkeypatch):
     monkeypatch.setattr(R, "block_layout", scattered)
     with pytest.raises(BosssError, match="slot order"):
         XdgOperator(R, 2, mu, source=src, penalty=2.0, g_D=g_D)
+
+
+def test_xdg_create_rejects_two_items_for_one_face(monkeypatch):
+    """A face item writes its two off-diagonal sub-blocks without atomics, so each face may have
+    one item per species: bosss_hip_xdg_create refuses a second one (before any device allocation)."""
+    import bosss_amd.xdg as X
+    from bosss_amd._lib import BosssError
+    say("case xdg duplicate face item")
+    R, mu, src, g_D, _ = build("sphere", 6, 2)
+    real = X.desc_arrays
+
+    def duplicated(rules, p, gD):
+        k = real(rules, p, gD)
+        fc, fsl, fsp = k["fc"].reshape(-1, 2), k["fsl"].reshape(-1, 2), k["fsp"].copy()
+        first = {}
+        for i in range(len(fsp)):
+            if fc[i, 1] < 0:
+                continue
+            key = (int(fc[i, 0]), int(fsl[i, 0]))
+            if key in first and fsp[first[key]] != fsp[i]:
+                fsp[i] = fsp[first[key]]        # the cut face's second item now claims the first one's species
+                k["fsp"] = fsp
+                return k
+            first[key] = i
+        raise AssertionError("no cut face with an item per species")
+
+    monkeypatch.setattr(X, "desc_arrays", duplicated)
+    with pytest.raises(BosssError, match="same face and species"):
+        X.XdgOperator(R, 2, mu, source=src, penalty=2.0, g_D=g_D)
