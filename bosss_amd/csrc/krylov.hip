@@ -18,6 +18,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <initializer_list>
 #include <string>
 
 #include "../../include/bosss_hip.h"
@@ -32,11 +33,11 @@ constexpr int KR_THREADS = 256;
 constexpr int KR_MAXBLK = 2048;     // workgroups of the streaming kernels (fewer for short vectors)
 constexpr int KR_MAXM = 256;        // largest restart length
 constexpr int KR_CHUNK = 16;        // basis vectors per pass of the dot kernel
-constexpr int KR_FUSED = 32;        // longest basis whose second-pass dots ride on the first update
+constexpr int KR_FUSED = 32;        // longest basis whose second-pass dots ride on the first update (16 with 16-byte packs)
 
 thread_local std::string g_kr_err;
 
-int kr_blocks(int64_t n) {
+int kr_blocks(int64_t n) {   // n: elements, or VEC-element packs
   int64_t b = (n + 4 * KR_THREADS - 1) / (4 * KR_THREADS);
   return (int)(b < 1 ? 1 : (b > KR_MAXBLK ? KR_MAXBLK : b));
 }
@@ -85,21 +86,58 @@ __device__ __forceinline__ void store_partials(const double (&acc)[C], int i0, i
   __syncthreads();
 }
 
+// VEC consecutive elements per thread (16-byte loads: 2 doubles or 4 floats, or 1 where the
+// vector length, the row stride or a pointer does not allow it).  Per element every sum runs in
+// the same index order as with VEC = 1; only the partial sums per thread group VEC elements.
+template <typename Real, int VEC>
+struct Pack {
+  Real v[VEC];
+};
+template <typename Real, int VEC>
+__device__ __forceinline__ Pack<Real, VEC> load_pack(const Real* __restrict__ p) {
+  Pack<Real, VEC> r;
+  if constexpr (VEC == 1) {
+    r.v[0] = *p;
+  } else {
+    typedef Real vt __attribute__((ext_vector_type(VEC)));
+    const vt x = *reinterpret_cast<const vt*>(p);
+#pragma unroll
+    for (int e = 0; e < VEC; e++) r.v[e] = x[e];
+  }
+  return r;
+}
+template <typename Real, int VEC>
+__device__ __forceinline__ void store_pack(Real* __restrict__ p, const Pack<Real, VEC>& r) {
+  if constexpr (VEC == 1) {
+    *p = r.v[0];
+  } else {
+    typedef Real vt __attribute__((ext_vector_type(VEC)));
+    vt x;
+#pragma unroll
+    for (int e = 0; e < VEC; e++) x[e] = r.v[e];
+    *reinterpret_cast<vt*>(p) = x;
+  }
+}
+
 // part[i][blk] = sum over the workgroup's share of t of V[i][t] w[t], i < k1
-template <typename Real>
+template <typename Real, int VEC>
 __global__ void __launch_bounds__(KR_THREADS) k_mdot(int64_t n, int k1, const Real* __restrict__ V, int64_t ldv,
                                                      const Real* __restrict__ w, double* __restrict__ part) {
   __shared__ double red[KR_THREADS / 64][KR_CHUNK];
-  const int nb = gridDim.x;
+  const int64_t nv = n / VEC;
   for (int i0 = 0; i0 < k1; i0 += KR_CHUNK) {
     double acc[KR_CHUNK];
 #pragma unroll
     for (int q = 0; q < KR_CHUNK; q++) acc[q] = 0.0;
-    for (int64_t t = (int64_t)blockIdx.x * KR_THREADS + threadIdx.x; t < n; t += (int64_t)nb * KR_THREADS) {
-      const double wt = (double)w[t];
+    for (int64_t tv = (int64_t)blockIdx.x * KR_THREADS + threadIdx.x; tv < nv; tv += (int64_t)gridDim.x * KR_THREADS) {
+      const Pack<Real, VEC> wt = load_pack<Real, VEC>(w + tv * VEC);
 #pragma unroll
       for (int q = 0; q < KR_CHUNK; q++)
-        if (i0 + q < k1) acc[q] += (double)V[(int64_t)(i0 + q) * ldv + t] * wt;
+        if (i0 + q < k1) {
+          const Pack<Real, VEC> vq = load_pack<Real, VEC>(V + (int64_t)(i0 + q) * ldv + tv * VEC);
+#pragma unroll
+          for (int e = 0; e < VEC; e++) acc[q] += (double)vq.v[e] * (double)wt.v[e];
+        }
     }
     store_partials<KR_CHUNK>(acc, i0, k1, red, part);
   }
@@ -120,35 +158,49 @@ __global__ void __launch_bounds__(KR_THREADS) k_hreduce(int nb, const double* __
   }
 }
 
-// s = sum_{i<k1} V[i][t] h[i], in index order, with the loads of eight basis vectors issued together
-template <typename Real>
-__device__ __forceinline__ double basis_combination(int k1, const Real* __restrict__ V, int64_t ldv, int64_t t, const double* hs) {
-  double s = 0.0;
+// s[e] = sum_{i<k1} V[i][t + e] h[i], in index order, with the loads of eight basis vectors issued together
+template <typename Real, int VEC>
+__device__ __forceinline__ void basis_combination(int k1, const Real* __restrict__ V, int64_t ldv, int64_t t, const double* hs,
+                                                  double (&s)[VEC]) {
+#pragma unroll
+  for (int e = 0; e < VEC; e++) s[e] = 0.0;
   int i = 0;
   for (; i + 8 <= k1; i += 8) {
-    Real v[8];
+    Pack<Real, VEC> v[8];
 #pragma unroll
-    for (int q = 0; q < 8; q++) v[q] = V[(int64_t)(i + q) * ldv + t];
+    for (int q = 0; q < 8; q++) v[q] = load_pack<Real, VEC>(V + (int64_t)(i + q) * ldv + t);
 #pragma unroll
-    for (int q = 0; q < 8; q++) s += (double)v[q] * hs[i + q];
+    for (int q = 0; q < 8; q++)
+#pragma unroll
+      for (int e = 0; e < VEC; e++) s[e] += (double)v[q].v[e] * hs[i + q];
   }
-  for (; i < k1; i++) s += (double)V[(int64_t)i * ldv + t] * hs[i];
-  return s;
+  for (; i < k1; i++) {
+    const Pack<Real, VEC> v = load_pack<Real, VEC>(V + (int64_t)i * ldv + t);
+#pragma unroll
+    for (int e = 0; e < VEC; e++) s[e] += (double)v.v[e] * hs[i];
+  }
 }
 
 // w[t] -= sum_{i<k1} V[i][t] h[i]; with norm: npart[blk] = workgroup's sum of the new w[t]^2
-template <typename Real>
+template <typename Real, int VEC>
 __global__ void __launch_bounds__(KR_THREADS) k_mupdate(int64_t n, int k1, const Real* __restrict__ V, int64_t ldv,
                                                         const double* __restrict__ h, Real* __restrict__ w, double* __restrict__ npart) {
   __shared__ double hs[KR_MAXM + 1];
   __shared__ double red[KR_THREADS / 64];
   for (int i = threadIdx.x; i < k1; i += KR_THREADS) hs[i] = h[i];
   __syncthreads();
+  const int64_t nv = n / VEC;
   double nrm = 0.0;
-  for (int64_t t = (int64_t)blockIdx.x * KR_THREADS + threadIdx.x; t < n; t += (int64_t)gridDim.x * KR_THREADS) {
-    const Real x = (Real)((double)w[t] - basis_combination(k1, V, ldv, t, hs));
-    w[t] = x;
-    nrm += (double)x * (double)x;
+  for (int64_t tv = (int64_t)blockIdx.x * KR_THREADS + threadIdx.x; tv < nv; tv += (int64_t)gridDim.x * KR_THREADS) {
+    double s[VEC];
+    basis_combination<Real, VEC>(k1, V, ldv, tv * VEC, hs, s);
+    Pack<Real, VEC> x = load_pack<Real, VEC>(w + tv * VEC);
+#pragma unroll
+    for (int e = 0; e < VEC; e++) {
+      x.v[e] = (Real)((double)x.v[e] - s[e]);
+      nrm += (double)x.v[e] * (double)x.v[e];
+    }
+    store_pack<Real, VEC>(w + tv * VEC, x);
   }
   if (npart) {
     nrm = block_sum(nrm, red);
@@ -160,7 +212,7 @@ __global__ void __launch_bounds__(KR_THREADS) k_mupdate(int64_t n, int k1, const
 // then part[i][blk] = workgroup's sum of V[i][t] w[t] over the updated w, with the basis values kept
 // in registers between the two (k1 <= C).  Same sums, in the same order, as k_mupdate followed by
 // k_mdot on the same grid.
-template <typename Real, int C>
+template <typename Real, int C, int VEC>
 __global__ void __launch_bounds__(KR_THREADS) k_mupdate_dot(int64_t n, int k1, const Real* __restrict__ V, int64_t ldv,
                                                             const double* __restrict__ h, Real* __restrict__ w,
                                                             double* __restrict__ part) {
@@ -168,22 +220,36 @@ __global__ void __launch_bounds__(KR_THREADS) k_mupdate_dot(int64_t n, int k1, c
   __shared__ double red[KR_THREADS / 64][C];
   for (int i = threadIdx.x; i < C; i += KR_THREADS) hs[i] = i < k1 ? h[i] : 0.0;
   __syncthreads();
+  const int64_t nv = n / VEC;
   double acc[C];
 #pragma unroll
   for (int q = 0; q < C; q++) acc[q] = 0.0;
-  for (int64_t t = (int64_t)blockIdx.x * KR_THREADS + threadIdx.x; t < n; t += (int64_t)gridDim.x * KR_THREADS) {
-    Real v[C];
+  for (int64_t tv = (int64_t)blockIdx.x * KR_THREADS + threadIdx.x; tv < nv; tv += (int64_t)gridDim.x * KR_THREADS) {
+    Pack<Real, VEC> v[C];
 #pragma unroll
-    for (int q = 0; q < C; q++) v[q] = q < k1 ? V[(int64_t)q * ldv + t] : Real(0);
-    double s = 0.0;
+    for (int q = 0; q < C; q++) {
+      if (q < k1) v[q] = load_pack<Real, VEC>(V + (int64_t)q * ldv + tv * VEC);
+      else {
+#pragma unroll
+        for (int e = 0; e < VEC; e++) v[q].v[e] = Real(0);
+      }
+    }
+    Pack<Real, VEC> x = load_pack<Real, VEC>(w + tv * VEC);
+    double xd[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; e++) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < C; q++)
+        if (q < k1) s += (double)v[q].v[e] * hs[q];
+      x.v[e] = (Real)((double)x.v[e] - s);
+      xd[e] = (double)x.v[e];
+    }
+    store_pack<Real, VEC>(w + tv * VEC, x);
 #pragma unroll
     for (int q = 0; q < C; q++)
-      if (q < k1) s += (double)v[q] * hs[q];
-    const Real x = (Real)((double)w[t] - s);
-    w[t] = x;
-    const double xd = (double)x;
 #pragma unroll
-    for (int q = 0; q < C; q++) acc[q] += (double)v[q] * xd;
+      for (int e = 0; e < VEC; e++) acc[q] += (double)v[q].v[e] * xd[e];
   }
   store_partials<C>(acc, 0, k1, red, part);
 }
@@ -220,12 +286,17 @@ __global__ void __launch_bounds__(KR_THREADS) k_finish(int nb, int k, int ldh, c
   res[k] = fabs(g[k + 1]);   // later rotations change g[k+1]; the estimate of step k is kept here
 }
 
-template <typename Real>
+template <typename Real, int VEC>
 __global__ void __launch_bounds__(KR_THREADS) k_scale_to(int64_t n, const Real* __restrict__ w, const double* __restrict__ scal,
                                                          Real* __restrict__ out) {
   const double inv = scal[0];
-  for (int64_t t = (int64_t)blockIdx.x * KR_THREADS + threadIdx.x; t < n; t += (int64_t)gridDim.x * KR_THREADS)
-    out[t] = (Real)((double)w[t] * inv);
+  const int64_t nv = n / VEC;
+  for (int64_t tv = (int64_t)blockIdx.x * KR_THREADS + threadIdx.x; tv < nv; tv += (int64_t)gridDim.x * KR_THREADS) {
+    Pack<Real, VEC> x = load_pack<Real, VEC>(w + tv * VEC);
+#pragma unroll
+    for (int e = 0; e < VEC; e++) x.v[e] = (Real)((double)x.v[e] * inv);
+    store_pack<Real, VEC>(out + tv * VEC, x);
+  }
 }
 
 // y = R^-1 g[0..k1) with R the leading k1 x k1 upper triangle of H (back substitution)
@@ -240,14 +311,20 @@ __global__ void k_trisolve(int k1, int ldh, const double* __restrict__ H, const 
 }
 
 // x[t] += sum_{i<k1} Z[i][t] y[i]
-template <typename Real>
+template <typename Real, int VEC>
 __global__ void __launch_bounds__(KR_THREADS) k_xupdate(int64_t n, int k1, const Real* __restrict__ Z, int64_t ldz,
                                                         const double* __restrict__ y, Real* __restrict__ x) {
   __shared__ double ys[KR_MAXM];
   for (int i = threadIdx.x; i < k1; i += KR_THREADS) ys[i] = y[i];
   __syncthreads();
-  for (int64_t t = (int64_t)blockIdx.x * KR_THREADS + threadIdx.x; t < n; t += (int64_t)gridDim.x * KR_THREADS) {
-    x[t] = (Real)((double)x[t] + basis_combination(k1, Z, ldz, t, ys));
+  const int64_t nv = n / VEC;
+  for (int64_t tv = (int64_t)blockIdx.x * KR_THREADS + threadIdx.x; tv < nv; tv += (int64_t)gridDim.x * KR_THREADS) {
+    double s[VEC];
+    basis_combination<Real, VEC>(k1, Z, ldz, tv * VEC, ys, s);
+    Pack<Real, VEC> xv = load_pack<Real, VEC>(x + tv * VEC);
+#pragma unroll
+    for (int e = 0; e < VEC; e++) xv.v[e] = (Real)((double)xv.v[e] + s[e]);
+    store_pack<Real, VEC>(x + tv * VEC, xv);
   }
 }
 
@@ -279,10 +356,20 @@ int kr_select(bosss_ctx* ctx) {
   return BOSSS_OK;
 }
 
+// 16-byte packs when the vector length, the row stride and every pointer allow them
 template <typename Real>
-void arnoldi(int64_t n, int k, int ldh, Real* V, int64_t ldv, Real* w, double* H, double* cs, double* sn, double* g, double* res,
-             double* work, hipStream_t s) {
-  const int nb = kr_blocks(n), k1 = k + 1;
+bool kr_vec_ok(int64_t n, int64_t ld, std::initializer_list<const void*> ptrs) {
+  constexpr int VEC = 16 / sizeof(Real);
+  if (n % VEC || ld % VEC) return false;
+  for (const void* p : ptrs)
+    if (reinterpret_cast<uintptr_t>(p) % 16) return false;
+  return true;
+}
+
+template <typename Real, int VEC>
+void arnoldi_v(int64_t n, int k, int ldh, Real* V, int64_t ldv, Real* w, double* H, double* cs, double* sn, double* g, double* res,
+               double* work, hipStream_t s) {
+  const int nb = kr_blocks(n / VEC), k1 = k + 1;
   double* part = work + WS_PART;
   double* npart = work + WS_NORM;
   double* h = work + WS_H;
@@ -290,25 +377,42 @@ void arnoldi(int64_t n, int k, int ldh, Real* V, int64_t ldv, Real* w, double* H
   double* Hk = H + (int64_t)k * ldh;
   const dim3 grid(nb), blk(KR_THREADS);
   // pass 0: h = V^T w, H[:, k] = h
-  hipLaunchKernelGGL(k_mdot<Real>, grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const Real*)w, part);
+  hipLaunchKernelGGL((k_mdot<Real, VEC>), grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const Real*)w, part);
   hipLaunchKernelGGL(k_hreduce, dim3(k1), blk, 0, s, nb, (const double*)part, h, Hk, 0);
   // w -= V h, and the pass-1 dots V^T w of the updated w
   if (k1 <= 8)
-    hipLaunchKernelGGL((k_mupdate_dot<Real, 8>), grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const double*)h, w, part);
+    hipLaunchKernelGGL((k_mupdate_dot<Real, 8, VEC>), grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const double*)h, w, part);
   else if (k1 <= 16)
-    hipLaunchKernelGGL((k_mupdate_dot<Real, 16>), grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const double*)h, w, part);
-  else if (k1 <= KR_FUSED)
-    hipLaunchKernelGGL((k_mupdate_dot<Real, KR_FUSED>), grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const double*)h, w, part);
+    hipLaunchKernelGGL((k_mupdate_dot<Real, 16, VEC>), grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const double*)h, w, part);
+  else if (VEC == 1 && k1 <= KR_FUSED)   // 32 packs of 16 bytes would not leave two wavefronts per SIMD
+    hipLaunchKernelGGL((k_mupdate_dot<Real, KR_FUSED, 1>), grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const double*)h, w, part);
   else {
-    hipLaunchKernelGGL(k_mupdate<Real>, grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const double*)h, w, nullptr);
-    hipLaunchKernelGGL(k_mdot<Real>, grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const Real*)w, part);
+    hipLaunchKernelGGL((k_mupdate<Real, VEC>), grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const double*)h, w, nullptr);
+    hipLaunchKernelGGL((k_mdot<Real, VEC>), grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const Real*)w, part);
   }
   // pass 1: H[:, k] += h, w -= V h with the norm partials of the result
   hipLaunchKernelGGL(k_hreduce, dim3(k1), blk, 0, s, nb, (const double*)part, h, Hk, 1);
-  hipLaunchKernelGGL(k_mupdate<Real>, grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const double*)h, w, npart);
+  hipLaunchKernelGGL((k_mupdate<Real, VEC>), grid, blk, 0, s, n, k1, (const Real*)V, ldv, (const double*)h, w, npart);
   hipLaunchKernelGGL(k_finish, dim3(1), blk, 0, s, nb, k, ldh, (const double*)npart, H, cs, sn, g, res, scal);
-  hipLaunchKernelGGL(k_scale_to<Real>, grid, blk, 0, s, n, (const Real*)w, (const double*)scal,
-                     V + (int64_t)(k + 1) * ldv);
+  hipLaunchKernelGGL((k_scale_to<Real, VEC>), grid, blk, 0, s, n, (const Real*)w, (const double*)scal, V + (int64_t)(k + 1) * ldv);
+}
+
+template <typename Real>
+void arnoldi(int64_t n, int k, int ldh, Real* V, int64_t ldv, Real* w, double* H, double* cs, double* sn, double* g, double* res,
+             double* work, hipStream_t s) {
+  if (kr_vec_ok<Real>(n, ldv, {V, w}))
+    arnoldi_v<Real, 16 / sizeof(Real)>(n, k, ldh, V, ldv, w, H, cs, sn, g, res, work, s);
+  else
+    arnoldi_v<Real, 1>(n, k, ldh, V, ldv, w, H, cs, sn, g, res, work, s);
+}
+
+template <typename Real>
+void xupdate(int64_t n, int k1, const Real* Z, int64_t ldz, const double* y, Real* x, hipStream_t s) {
+  constexpr int VEC = 16 / sizeof(Real);
+  if (kr_vec_ok<Real>(n, ldz, {Z, x}))
+    hipLaunchKernelGGL((k_xupdate<Real, VEC>), dim3(kr_blocks(n / VEC)), dim3(KR_THREADS), 0, s, n, k1, Z, ldz, y, x);
+  else
+    hipLaunchKernelGGL((k_xupdate<Real, 1>), dim3(kr_blocks(n)), dim3(KR_THREADS), 0, s, n, k1, Z, ldz, y, x);
 }
 
 }  // namespace
@@ -350,11 +454,10 @@ int bosss_hip_gmres_update_dev(bosss_ctx* ctx, int64_t n, int32_t k1, int32_t m,
   if (k1 == 0) return BOSSS_OK;
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_trisolve, dim3(1), dim3(64), 0, s, k1, m + 1, H, g, y);
-  const int nb = kr_blocks(n);
   if (precision == BOSSS_FP64)
-    hipLaunchKernelGGL(k_xupdate<double>, dim3(nb), dim3(KR_THREADS), 0, s, n, k1, (const double*)Z, ldz, (const double*)y, (double*)x);
+    xupdate<double>(n, k1, (const double*)Z, ldz, (const double*)y, (double*)x, s);
   else if (precision == BOSSS_FP32)
-    hipLaunchKernelGGL(k_xupdate<float>, dim3(nb), dim3(KR_THREADS), 0, s, n, k1, (const float*)Z, ldz, (const double*)y, (float*)x);
+    xupdate<float>(n, k1, (const float*)Z, ldz, (const double*)y, (float*)x, s);
   else { g_kr_err = "precision must be BOSSS_FP64 or BOSSS_FP32"; return BOSSS_ERR_ARG; }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) { g_kr_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }

@@ -1,8 +1,9 @@
 """Device-resident GMRES (bosss_hip_gmres_arnoldi_dev / _update_dev, bosss_amd.newton.DeviceGmres)
 against the host-Givens restatement (newton.gmres) on the same operators:
 
-  * a dense, non-symmetric, well-conditioned matrix applied with torch (restarts included):
-    same iteration counts, solutions within 1e-12 of the host solver and of numpy's solve;
+  * a dense, non-symmetric, well-conditioned matrix applied with torch (restarts included;
+    an even and an odd length, i.e. the 16-byte-pack and the one-element kernels): same
+    iteration counts, solutions within 1e-12 of the host solver and of numpy's solve;
   * graph-captured steps against eager steps: three solves in a row, with eager work and a
     synchronisation between them, each bitwise equal to the eager solver's result;
   * the Euler HLLC implicit step (p = 2, 6^3) through the Newton driver with host, device-eager
@@ -36,12 +37,13 @@ def dense_case(n=300, seed=3):
     return A, b, At, bt, apply_A
 
 
+@pytest.mark.parametrize("n", [300, 301])          # 16-byte packs; one element per thread (odd length)
 @pytest.mark.parametrize("restart", [10, 40])
-def test_device_gmres_matches_host_gmres(restart):
+def test_device_gmres_matches_host_gmres(restart, n):
     import torch
     from bosss_amd.newton import DeviceGmres, gmres
-    say("case device gmres dense restart", restart)
-    A, b, At, bt, apply_A = dense_case()
+    say("case device gmres dense restart", restart, "n", n)
+    A, b, At, bt, apply_A = dense_case(n)
     xh, ith, rh = gmres(apply_A, bt, 1e-12, restart, 400)
     dg = DeviceGmres(bt.numel(), torch.float64, bt.device, restart)
     for every in (1, 8):
