@@ -73,7 +73,7 @@ SIGNATURES = {
     "bosss_hip_op_matrix_layout": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
                                                   ctypes.POINTER(ctypes.c_int32), _vp]),
     "bosss_hip_op_assemble_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.c_double, _vp, _vp]),
-    "bosss_hip_op_diag_blocks_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.c_double, _vp, _vp]),
+    "bosss_hip_op_diag_blocks_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.c_double, ctypes.c_int32, _vp, _vp]),
     "bosss_hip_bell_spmv_dev": (ctypes.c_int, [_vp, _vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp]),
     "bosss_hip_op_set_spmv_tile": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "bosss_hip_op_spmv_tiles": (ctypes.c_int, [_vp, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_float), ctypes.c_int32]),

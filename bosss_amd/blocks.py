@@ -88,9 +88,10 @@ class CellBlockPreconditioner:
     diagonal blocks of the linearised operator), without assembling the off-diagonal blocks.
 
     The blocks come from own-cell JVP probes of ``op`` (``bosss_hip_op_diag_blocks_dev``; an FP32
-    operator probes in FP32), are inverted in FP64 by the batched LU (pivots and inverse in FP64),
-    and are stored transposed in the apply precision ("fp32": rounded once to FP32), so that the
-    apply (``bosss_hip_block_gemv_uniform_dev``) reads each block column by column, coalesced."""
+    operator probes in FP32) in transposed storage, which the probes write contiguously; inverting
+    D^T in place in FP64 (batched LU pivots, Gauss-Jordan inverse) gives (D^-1)^T, the layout in
+    which the apply (``bosss_hip_block_gemv_uniform_dev``) reads each block column by column,
+    coalesced; it is kept in the apply precision ("fp32": rounded once to FP32)."""
 
     def __init__(self, op, U, shift, precision="fp32"):
         import time
@@ -100,14 +101,14 @@ class CellBlockPreconditioner:
         self.op, self.J, self.Nt = op, J, Nt
         self.dtype = torch.float32 if precision == "fp32" else torch.float64
         B = torch.empty(J * Nt * Nt, dtype=torch.float64, device=U.device)
-        op.diag_blocks_dev(U, B, alpha=1.0, shift=shift)
+        op.diag_blocks_dev(U, B, alpha=1.0, shift=shift, transposed=True)       # D_j^T
         batch = BlockBatch([Nt] * J, device=U.device)
         info = torch.zeros(J, dtype=torch.int32, device=U.device)
-        batch.factor(B, None, info, inverse=True)
+        batch.factor(B, None, info, inverse=True)                               # (D_j^T)^-1 = (D_j^-1)^T
         bad = int((info != 0).sum())
         if bad:
             raise ArithmeticException("%d singular diagonal blocks" % bad)
-        self.invT = B.view(J, Nt, Nt).transpose(1, 2).to(self.dtype).contiguous().view(-1)
+        self.invT = B if self.dtype == torch.float64 else B.to(self.dtype)
         del B
         torch.cuda.synchronize(U.device)
         self.setup_seconds = time.perf_counter() - t0

@@ -693,7 +693,8 @@ static int ensure_matrix_setup(bosss_op* op) {
 }
 
 // Diagonal cell blocks by probing the own-cell JVP: X[c][e] = (e == q) in every cell, then
-// blocks[j][r][q] = alpha * (J(U)_jj e_q)[r] + shift * (r == q).
+// blocks[j][r][q] = alpha * (J(U)_jj e_q)[r] + shift * (r == q)  (transposed: blocks[j][q][r], the
+// probe's column written contiguously; the row-major form writes it with a stride of Nt).
 template <typename Real>
 __global__ void k_probe_all(int J, int Nt, int q, Real* __restrict__ X) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -701,12 +702,14 @@ __global__ void k_probe_all(int J, int Nt, int q, Real* __restrict__ X) {
   X[i] = (int)(i % Nt) == q ? Real(1) : Real(0);
 }
 template <typename Real>
-__global__ void k_extract_diag(int J, int Nt, int q, const Real* __restrict__ Y, double alpha, double shift, double* __restrict__ blocks) {
+__global__ void k_extract_diag(int J, int Nt, int q, int transposed, const Real* __restrict__ Y, double alpha, double shift,
+                               double* __restrict__ blocks) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (size_t)J * Nt) return;
   const size_t j = i / Nt;
   const int r = (int)(i - j * Nt);
-  blocks[(j * Nt + r) * Nt + q] = alpha * (double)Y[i] + (r == q ? shift : 0.0);
+  const size_t at = transposed ? (j * Nt + q) * Nt + r : (j * Nt + r) * Nt + q;
+  blocks[at] = alpha * (double)Y[i] + (r == q ? shift : 0.0);
 }
 
 // Autotuner of the block-ELL apply's panel size: times every candidate on this operator's
@@ -793,7 +796,8 @@ int bosss_hip_op_assemble_dev(bosss_op* op, const double* U, double alpha, doubl
   return BOSSS_OK;
 }
 
-int bosss_hip_op_diag_blocks_dev(bosss_op* op, const void* U, double alpha, double shift, double* blocks, void* stream) {
+int bosss_hip_op_diag_blocks_dev(bosss_op* op, const void* U, double alpha, double shift, int32_t transposed, double* blocks,
+                                 void* stream) {
   if (!op || !U || !blocks) return fail(BOSSS_ERR_ARG, "NULL argument");
   int rc;
   HIPCHK(hipSetDevice(op->ctx->device));
@@ -807,9 +811,11 @@ int bosss_hip_op_diag_blocks_dev(bosss_op* op, const void* U, double alpha, doub
     else hipLaunchKernelGGL(k_probe_all<float>, dim3(nb), dim3(256), 0, s, J, Nt, q, (float*)op->hX);
     if ((rc = launch(op, KernelSet::JVP, 1.0, 0.0, U, op->hX, nullptr, op->hY, s, 1))) return rc;
     if (op->precision == BOSSS_FP64)
-      hipLaunchKernelGGL(k_extract_diag<double>, dim3(nb), dim3(256), 0, s, J, Nt, q, (const double*)op->hY, alpha, shift, blocks);
+      hipLaunchKernelGGL(k_extract_diag<double>, dim3(nb), dim3(256), 0, s, J, Nt, q, (int)(transposed != 0), (const double*)op->hY,
+                         alpha, shift, blocks);
     else
-      hipLaunchKernelGGL(k_extract_diag<float>, dim3(nb), dim3(256), 0, s, J, Nt, q, (const float*)op->hY, alpha, shift, blocks);
+      hipLaunchKernelGGL(k_extract_diag<float>, dim3(nb), dim3(256), 0, s, J, Nt, q, (int)(transposed != 0), (const float*)op->hY, alpha,
+                         shift, blocks);
   }
   HIPCHK(hipGetLastError());
   return BOSSS_OK;

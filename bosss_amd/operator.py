@@ -203,14 +203,16 @@ class DifferentialOperator:
         self._check_matrix_args(blocks, U)
         check(lib().bosss_hip_op_assemble_dev(self.h, self._ptr(U), alpha, beta, self._ptr(blocks), self._stream(stream)))
 
-    def diag_blocks_dev(self, U, blocks, alpha=1.0, shift=0.0, stream=None):
+    def diag_blocks_dev(self, U, blocks, alpha=1.0, shift=0.0, transposed=False, stream=None):
         """blocks <- alpha * dR_j/dU_j(U) + shift * I for every cell j: J row-major Nt x Nt FP64
-        blocks (own-cell JVP probes in the operator's precision; U is a state vector of it)."""
+        blocks, or each block transposed (own-cell JVP probes in the operator's precision; U is a
+        state vector of it)."""
         import torch
         self._check_dev(U)
         Nt = self.nvars * self.N
         device_array(blocks, torch.float64, self.grid.J * Nt * Nt, self.ctx.device, "blocks")
-        check(lib().bosss_hip_op_diag_blocks_dev(self.h, self._ptr(U), alpha, shift, self._ptr(blocks), self._stream(stream)))
+        check(lib().bosss_hip_op_diag_blocks_dev(self.h, self._ptr(U), alpha, shift, int(bool(transposed)), self._ptr(blocks),
+                                                 self._stream(stream)))
 
     def spmv_dev(self, blocks, x, y, alpha=1.0, beta=0.0, stream=None):
         self._check_matrix_args(blocks, x, y)

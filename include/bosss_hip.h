@@ -154,11 +154,15 @@ int bosss_hip_op_matrix_layout(bosss_op* op, int32_t* blocks_per_row, int64_t* e
 int bosss_hip_op_assemble_dev(bosss_op* op, const double* U, double alpha, double beta, double* blocks, void* stream);
 /* Diagonal cell blocks of the Jacobian, for cell-block preconditioners (the blocks BlockJacobi.InitImpl
  * reads from the assembled matrix, BlockJacobi.cs:63-120, without assembling the off-diagonal ones):
- * blocks[j] <- alpha * dR_j/dU_j (U) + shift * I, row-major Nt x Nt (Nt = V*N), J blocks of FP64.
+ * blocks[j] <- alpha * dR_j/dU_j (U) + shift * I, row-major Nt x Nt (Nt = V*N), J blocks of FP64;
+ * with transposed != 0 each block is stored transposed (column-major), which is how the probes
+ * produce it (one column per probe, written contiguously): inverting those blocks in place gives
+ * the transposed inverses that bosss_hip_block_gemv_uniform_dev applies.
  * Nt probing JVPs with the neighbour tangents taken as zero, in the operator's precision (U is
  * double for BOSSS_FP64, float for BOSSS_FP32; an FP32 operator gives FP32-accurate blocks, widened).
  * Works for every form, linear or not. */
-int bosss_hip_op_diag_blocks_dev(bosss_op* op, const void* U, double alpha, double shift, double* blocks, void* stream);
+int bosss_hip_op_diag_blocks_dev(bosss_op* op, const void* U, double alpha, double shift, int32_t transposed, double* blocks,
+                                 void* stream);
 /* y <- alpha*A x + beta*y with A in the operator's block-ELL layout (BlockMsrMatrix.SpMV, BlockMsrMatrix.cs:2554) */
 int bosss_hip_bell_spmv_dev(bosss_op* op, const double* blocks, double alpha, const double* x, double beta, double* y, void* stream);
 /* Host-pointer forms of the two calls above (blocks, U, x, y host arrays of entries / ndof

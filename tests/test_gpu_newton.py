@@ -190,8 +190,11 @@ def test_diag_blocks_equal_the_assembled_diagonal_blocks():
     D32 = torch.empty_like(D)
     op32.diag_blocks_dev(Un.float(), D32, alpha=2.0, shift=3.0)
     err32 = float((D32.view(g.J, Nt, Nt) - ref).abs().max() / ref.abs().max())
+    DT = torch.empty_like(D)
+    op64.diag_blocks_dev(Un, DT, alpha=2.0, shift=3.0, transposed=True)     # the same blocks, stored transposed
     say("  fp64 %.2e  fp32 %.2e" % (err, err32))
     assert err <= 1e-13 and err32 <= 1e-5
+    assert torch.equal(DT.view(g.J, Nt, Nt).transpose(1, 2), D.view(g.J, Nt, Nt))
 
 
 def test_newton_at_implicit_cfl_with_cell_block_preconditioners():
