@@ -190,6 +190,8 @@ def secondary_workload(args):
     advdiff  upwind + SIP (nu 0.01), 64^3 periodic, p=3: matrix-free FP64 and FP32 vs assembled SpMV
     euler32  Euler LF and HLLC, 32^3 periodic, p=2: fused residual + VJP
     xdg32    XDG Poisson, sphere in 32^3, p=2: cut-cell rules, assembly, apply, block-Jacobi PCG
+    blocklu  batched cell-block LU: the XDG 10/20-DOF blocks, the 100-DOF Euler p=3 blocks (with
+             inverses), and multi-cell blocks of 256 to 2048 DOF (Schwarz parts)
     """
     import numpy as np
     import torch
@@ -237,6 +239,48 @@ def secondary_workload(args):
         dof, desc = op.ndof, ("XDG Poisson, sphere r=0.3 in [0,1]^3, 32^3 cells, p=2, mu_A=-1, mu_B=-10, penalty 2, "
                               "agglomeration threshold 0: assembly, block-sparse apply, block-Jacobi PCG")
         op.close()
+    elif args.workload == "blocklu":
+        from bosss_amd.blocks import BlockBatch
+        cases = [("xdg_sphere_32cube_cells", [(10, 31008), (20, 1760)], True),
+                 ("euler_p3_64cube_cells", [(100, 262144)], True),
+                 ("schwarz_256", [(256, 256)], False), ("schwarz_540", [(540, 64)], False),
+                 ("schwarz_1024", [(1024, 64)], False), ("schwarz_2048", [(2048, 16)], False)]
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(0)
+        for name, groups, inv in cases:
+            sizes = [n for n, cnt in groups for _ in range(cnt)]
+            bb = BlockBatch(sizes, device=dev)
+            parts = []
+            for n, cnt in groups:   # uniform[-1, 1] entries plus a dominant diagonal, as the tests' blocks
+                blk = torch.rand((cnt, n, n), dtype=torch.float64, device=dev, generator=gen) * 2.0 - 1.0
+                blk += 0.2 * n * torch.eye(n, dtype=torch.float64, device=dev)
+                parts.append(blk.view(-1))
+            A0 = torch.cat(parts)
+            del parts
+            A = torch.empty_like(A0)
+            ipiv = torch.zeros(bb.ndof, dtype=torch.int32, device=dev)
+            info = torch.zeros(bb.nblk, dtype=torch.int32, device=dev)
+            per = []
+            for k in range(W0 + K):
+                A.copy_(A0)                       # every factorisation starts from the same blocks
+                ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                ev0.record()
+                bb.factor(A, None if inv else ipiv, info, inverse=inv)
+                ev1.record()
+                torch.cuda.synchronize()
+                if k >= W0:
+                    per.append(ev0.elapsed_time(ev1))
+            ms = float(np.median(per))
+            flops = sum(cnt * (2.0 if inv else 2.0 / 3.0) * n ** 3 for n, cnt in groups)
+            nbytes = 2 * 8 * sum(cnt * n * n for n, cnt in groups)
+            res[name] = {"blocks": {str(n): cnt for n, cnt in groups}, "inverse": inv, "ms": round(ms, 4),
+                         "gflop": round(flops / 1e9, 4), "tflops": round(flops / ms / 1e9, 3),
+                         "frac_fp64_peak": round(flops / ms / 1e9 / FP64_PEAK_TFLOPS, 4),
+                         "hbm_gb_s": round(nbytes / ms / 1e6, 1), "nonzero_info": int((info != 0).sum())}
+            del A, A0, ipiv, info
+            torch.cuda.empty_cache()
+        dof, desc = 0, ("batched cell-block LU (getf2 pivots; inverses by Gauss-Jordan up to 128 DOF, "
+                        "blocked LU with a matrix-unit trailing update above 128 DOF)")
     elif args.workload == "euler32":
         grid = CartesianGrid((32,) * 3, periodic=[True] * 3)
         for form in ("euler_lf", "euler_hllc"):
@@ -411,7 +455,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--split", action="store_true", help="time separate residual and VJP launches instead of the fused one")
     ap.add_argument("--no-newton", action="store_true", help="skip the implicit-step Newton comparison (FP64 vs mixed)")
-    ap.add_argument("--workload", default="euler64", choices=["euler64", "sip2d", "advdiff", "euler32", "xdg32"],
+    ap.add_argument("--workload", default="euler64", choices=["euler64", "sip2d", "advdiff", "euler32", "xdg32", "blocklu"],
                     help="euler64: the headline line; the others print the secondary configurations")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device copy first (known-size reference for HBM counter readings)")

@@ -361,97 +361,19 @@ __global__ void __launch_bounds__(256) k_block_lu_rows(int nblk, int nmax, const
   if (ok && r == 0 && info) info[b] = inf;
 }
 
-// Blocks above LU_MAXN (the multi-cell blocks of Schwarz smoothers, up to LU_BIG_MAXN): one
-// workgroup of 256 threads per block, DGETF2 in place in global memory with getf2's operation order
-// -- pivot = first row with the largest |a| in the column (workgroup
-// argmax, ties to the smaller row), row swap, column scaled by the reciprocal pivot, rank-1
-// update of the trailing rows (one row per wavefront, lanes along the row: coalesced).  A zero
-// pivot sets INFO and the factorisation continues.  O(n^3) global-memory traffic per block,
-// served mostly by L2; it makes such blocks factorable on the device, it is not a blocked LU.
+// Blocks above LU_MAXN (the multi-cell blocks of Schwarz smoothers, up to LU_BIG_MAXN) factor in
+// place in global memory, blocked (see the blocked forms below); the pivot search is getf2's:
+// the first row with the largest |a| in the column (workgroup argmax, ties to the smaller row).
 constexpr int LU_BIG_MAXN = 4096;
-// the two-launch-per-column form takes over for blocks of at least LU_SPLIT_MINN when the batch
+// the two-launch-per-panel form takes over for blocks of at least LU_SPLIT_MINN when the batch
 // has fewer than LU_SPLIT_MAXBLK blocks (fewer blocks than CUs leave the one-workgroup form idle)
 constexpr int LU_SPLIT_MINN = 512;
 constexpr int LU_SPLIT_MAXBLK = 256;
-template <typename Real>
-__global__ void __launch_bounds__(256) k_block_lu_big(int nblk, int nmax, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
-                                                      Real* __restrict__ A, int32_t* __restrict__ ipiv, const int64_t* __restrict__ pivoff,
-                                                      int32_t* __restrict__ info) {
-  __shared__ Real sv[4];
-  __shared__ int si[4];
-  __shared__ int s_piv, s_act;
-  const int b = blockIdx.x;
-  if (b >= nblk) return;
-  const int n = nsz[b];
-  if (n <= 0 || n > nmax) {
-    if (threadIdx.x == 0 && info) info[b] = -1;
-    return;
-  }
-  Real* a = A + off[b];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  int inf = 0;
-  for (int j = 0; j < n; j++) {
-    Real best = Real(-1);
-    int bi = n;
-    for (int i = j + t; i < n; i += 256) {
-      Real v = fabs(a[(size_t)i * n + j]);
-      if (i == j && v != v) v = Real(INFINITY);   // idamax: a NaN first element is never beaten
-      if (v > best || (v == best && i < bi)) { best = v; bi = i; }
-    }
-    wave_argmax(best, bi);
-    if (lane == 0) { sv[w] = best; si[w] = bi; }
-    __syncthreads();
-    if (t == 0) {
-      Real bv = sv[0];
-      int bx = si[0];
-      for (int k = 1; k < 4; k++)
-        if (sv[k] > bv || (sv[k] == bv && si[k] < bx)) { bv = sv[k]; bx = si[k]; }
-      s_piv = bx < n ? bx : j;   // no comparable candidate (all NaN): the diagonal, as idamax
-      // The pivot test is read here, once, before the barrier: after it the row swap overwrites
-      // a[p][j] (the thread owning column j may be in another wavefront).  NaN passes the test,
-      // as in DGETF2 (A(JP,J).NE.ZERO), and leaves INFO unset.
-      s_act = a[(size_t)s_piv * n + j] != Real(0);
-    }
-    __syncthreads();
-    const int p = s_piv;
-    if (t == 0 && ipiv) ipiv[pivoff[b] + j] = p + 1;
-    if (!s_act) {
-      if (!inf) inf = j + 1;
-      __syncthreads();
-      continue;
-    }
-    if (p != j)
-      for (int c = t; c < n; c += 256) {
-        const Real x = a[(size_t)j * n + c];
-        a[(size_t)j * n + c] = a[(size_t)p * n + c];
-        a[(size_t)p * n + c] = x;
-      }
-    __syncthreads();
-    const Real inv = Real(1) / a[(size_t)j * n + j];
-    for (int i = j + 1 + t; i < n; i += 256) a[(size_t)i * n + j] *= inv;
-    __syncthreads();
-    for (int i = j + 1 + w; i < n; i += 4) {
-      const Real l = a[(size_t)i * n + j];
-      for (int c = j + 1 + lane; c < n; c += 64) a[(size_t)i * n + c] -= l * a[(size_t)j * n + c];
-    }
-    __syncthreads();
-  }
-  if (t == 0 && info) info[b] = inf;
-}
-
-// The same factorisation for few large blocks (one workgroup per block would leave most CUs
-// idle), two columns per pass over the trailing matrix: k_lu_step pivots column j, applies its
-// rank-1 update to column j+1 and to row j+1 (getf2 does these at step j), pivots column j+1
-// and scales both columns (one workgroup per block); k_lu_update then applies both columns'
-// updates to every trailing element, a -= l u_j, then a -= m u_j+1, getf2's order (four rows
-// per workgroup, grid over all blocks' rows).  st[b] holds the block's INFO so far,
-// st[nblk + b] / st[2 nblk + b] whether column j / j+1 eliminates (0: done, invalid, or a zero
-// pivot).
 // Returns the pivot row and, in act, whether the pivot is non-zero.  Thread 0 reads the pivot
 // entry before the final barrier, so every thread decides on the same value even though the
 // row swap that follows overwrites it (NaN counts as non-zero, as in DGETF2).
 template <typename Real>
-__device__ int lu_pivot_wg(const Real* a, int n, int j, Real* sv, int* si, int* s_piv, int* s_act, bool& act) {
+__device__ __forceinline__ int lu_pivot_wg(const Real* a, int n, int j, Real* sv, int* si, int* s_piv, int* s_act, bool& act) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   Real best = Real(-1);
   int bi = n;
@@ -477,7 +399,7 @@ __device__ int lu_pivot_wg(const Real* a, int n, int j, Real* sv, int* si, int* 
 }
 
 template <typename Real>
-__device__ void lu_swap_rows(Real* a, int n, int r1, int r2) {
+__device__ __forceinline__ void lu_swap_rows(Real* a, int n, int r1, int r2) {
   for (int c = threadIdx.x; c < n; c += 256) {
     const Real x = a[(size_t)r1 * n + c];
     a[(size_t)r1 * n + c] = a[(size_t)r2 * n + c];
@@ -485,78 +407,187 @@ __device__ void lu_swap_rows(Real* a, int n, int r1, int r2) {
   }
 }
 
+// Blocked forms for blocks above 128 DOF (right-looking, panel width LU_NB): getf2 on the
+// LU_NB columns of a panel with whole-row exchanges -- getf2's pivots, exchanges and scaling, the
+// rank-1 updates applied to the panel columns only --, then U12 = L11^-1 A12 by forward
+// substitution (each element gets getf2's updates in getf2's order), then the trailing update
+// A22 -= L21 U12 as one rank-LU_NB product on v_mfma_f64_16x16x4f64.  Pivots, INFO and the
+// factors of the panel rows are getf2's; the trailing elements receive the same products summed
+// in the matrix units' order (rounding).  A column whose pivot is zero is skipped by getf2
+// (no scaling, no update): its L entries count as zero in the solve and the product (zmask).
+constexpr int LU_NB = 32;
+typedef double lu_f64x4 __attribute__((ext_vector_type(4)));
+
+// Panel [k0, k0 + nb) of block a (n x n, row-major), one workgroup.  Returns the panel's zero-pivot
+// mask (bit jj: column k0 + jj) and sets inf to the first zero pivot (1-based) if it is still 0.
 template <typename Real>
-__global__ void __launch_bounds__(256) k_lu_step(int j, int nblk, int nmax, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
-                                                 Real* __restrict__ A, int32_t* __restrict__ ipiv, const int64_t* __restrict__ pivoff,
-                                                 int32_t* __restrict__ st) {
-  __shared__ Real sv[4];
-  __shared__ int si[4];
-  __shared__ int s_piv, s_act;
-  const int b = blockIdx.x;
-  if (b >= nblk) return;
-  const int n = nsz[b];
+__device__ __forceinline__ unsigned lu_panel(Real* a, int n, int k0, int nb, int32_t* ipiv_b, int& inf, Real* sv, int* si, int* s_piv, int* s_act) {
   const int t = threadIdx.x;
-  if (j == 0 && t == 0) st[b] = (n <= 0 || n > nmax) ? -1 : 0;
-  if (n <= 0 || n > nmax || j >= n) {
-    if (t == 0) { st[nblk + b] = 0; st[2 * nblk + b] = 0; }
-    return;
-  }
-  Real* a = A + off[b];
-  // column j
-  bool act1;
-  const int p1 = lu_pivot_wg(a, n, j, sv, si, &s_piv, &s_act, act1);
-  if (t == 0 && ipiv) ipiv[pivoff[b] + j] = p1 + 1;
-  if (act1) {
-    if (p1 != j) lu_swap_rows(a, n, j, p1);
+  const int kend = k0 + nb;
+  unsigned zmask = 0;
+  for (int j = k0; j < kend; j++) {
+    bool act;
+    const int p = lu_pivot_wg(a, n, j, sv, si, s_piv, s_act, act);
+    if (t == 0 && ipiv_b) ipiv_b[j] = p + 1;
+    if (!act) {
+      if (!inf) inf = j + 1;
+      zmask |= 1u << (j - k0);
+      __syncthreads();
+      continue;
+    }
+    if (p != j) lu_swap_rows(a, n, j, p);
     __syncthreads();
     const Real inv = Real(1) / a[(size_t)j * n + j];
     for (int i = j + 1 + t; i < n; i += 256) a[(size_t)i * n + j] *= inv;
-  } else if (t == 0 && st[b] == 0) {
-    st[b] = j + 1;
+    __syncthreads();
+    const int w = kend - j - 1;   // panel columns right of j
+    if (w > 0)
+      for (int e = t; e < (n - j - 1) * w; e += 256) {
+        const int i = j + 1 + e / w, c = j + 1 + e % w;
+        a[(size_t)i * n + c] -= a[(size_t)i * n + j] * a[(size_t)j * n + c];
+      }
+    __syncthreads();
+  }
+  return zmask;
+}
+
+// U12 = L11^-1 A12 for the columns c = k0 + LU_NB + c_begin, + c_step, ... (one column per thread;
+// only full panels have columns to their right).  sL: LU_NB x LU_NB staging of L11 (zero-pivot
+// columns as zero), filled by the calling workgroup.
+template <typename Real>
+__device__ __forceinline__ void lu_trsm(Real* a, int n, int k0, unsigned zmask, int c_begin, int c_step, Real* sL) {
+  for (int e = threadIdx.x; e < LU_NB * LU_NB; e += blockDim.x) {
+    const int i = e / LU_NB, q = e % LU_NB;
+    sL[e] = (q < i && !((zmask >> q) & 1u)) ? a[(size_t)(k0 + i) * n + k0 + q] : Real(0);
   }
   __syncthreads();
-  bool act2 = false;
-  if (j + 1 < n) {
-    const int c1 = j + 1;
-    if (act1)   // step j's update of column j+1
-      for (int i = j + 1 + t; i < n; i += 256) a[(size_t)i * n + c1] -= a[(size_t)i * n + j] * a[(size_t)j * n + c1];
-    __syncthreads();
-    const int p2 = lu_pivot_wg(a, n, c1, sv, si, &s_piv, &s_act, act2);
-    if (t == 0 && ipiv) ipiv[pivoff[b] + c1] = p2 + 1;
-    if (act2 && p2 != c1) lu_swap_rows(a, n, c1, p2);
-    __syncthreads();
-    if (act1)   // step j's update of row j+1 beyond column j+1
-      for (int c = c1 + 1 + t; c < n; c += 256) a[(size_t)c1 * n + c] -= a[(size_t)c1 * n + j] * a[(size_t)j * n + c];
-    __syncthreads();
-    if (act2) {
-      const Real inv = Real(1) / a[(size_t)c1 * n + c1];
-      for (int i = c1 + 1 + t; i < n; i += 256) a[(size_t)i * n + c1] *= inv;
-    } else if (t == 0 && st[b] == 0) {
-      st[b] = c1 + 1;
+#pragma unroll 1
+  for (int c = k0 + LU_NB + c_begin; c < n; c += c_step) {
+    Real* pc = a + (size_t)k0 * n + c;
+    Real u[LU_NB];
+#pragma unroll
+    for (int i = 0; i < LU_NB; i++) u[i] = pc[i * n];
+    // column-oriented: u[i] -= L[i][q] u[q] for q = 0, 1, ... -- per element the order of getf2's steps
+#pragma unroll
+    for (int q = 0; q < LU_NB - 1; q++) {
+      asm volatile("" ::: "memory");   // L11 is read column by column in each pass, not hoisted out of the loop over c
+      const Real uq = u[q];
+#pragma unroll
+      for (int i = q + 1; i < LU_NB; i++) u[i] -= sL[i * LU_NB + q] * uq;
+    }
+#pragma unroll
+    for (int i = 1; i < LU_NB; i++) pc[i * n] = u[i];
+  }
+}
+
+// A22 -= L21 U12 on 16 x 16 tiles of the trailing matrix (rows and columns >= k0 + LU_NB, full
+// panels only): each wavefront takes the tile rows ti = r_begin, + r_step, ... and keeps that
+// row's L21 operands in registers while it walks the tile columns.
+template <typename Real>
+__device__ __forceinline__ void lu_trail(Real* a, int n, int k0, unsigned zmask, int r_begin, int r_step) {
+  const int k1 = k0 + LU_NB, m = n - k1;
+  if (m <= 0) return;
+  const int T = (m + 15) / 16;
+  const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  const Real* U = a + (size_t)k0 * n;            // U12 rows k0 .. k0 + LU_NB - 1
+#pragma unroll 1
+  for (int ti = r_begin; ti < T; ti += r_step) {
+    const int r0 = k1 + 16 * ti;
+    Real av[LU_NB / 4];
+#pragma unroll
+    for (int kk = 0; kk < LU_NB / 4; kk++) {
+      const int q = 4 * kk + grp;                           // panel column this lane supplies
+      const int row = r0 + col;                             // A row this lane supplies
+      av[kk] = (row < n && !((zmask >> q) & 1u)) ? -a[(size_t)row * n + k0 + q] : Real(0);
+    }
+#pragma unroll 1
+    for (int tj = 0; tj < T; tj++) {
+      const int c = k1 + 16 * tj + col;
+      lu_f64x4 acc;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = r0 + grp + 4 * r;
+        acc[r] = (row < n && c < n) ? a[(size_t)row * n + c] : Real(0);
+      }
+#pragma unroll
+      for (int kk = 0; kk < LU_NB / 4; kk++) {
+        const Real bv = c < n ? U[(4 * kk + grp) * n + c] : Real(0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk], bv, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = r0 + grp + 4 * r;
+        if (row < n && c < n) a[(size_t)row * n + c] = acc[r];
+      }
     }
   }
-  if (t == 0) { st[nblk + b] = act1 ? 1 : 0; st[2 * nblk + b] = act2 ? 1 : 0; }
+}
+
+// One workgroup per block, every panel in turn.
+template <typename Real>
+__global__ void __launch_bounds__(256) k_block_lu_blocked(int nblk, int nmax, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
+                                                          Real* __restrict__ A, int32_t* __restrict__ ipiv, const int64_t* __restrict__ pivoff,
+                                                          int32_t* __restrict__ info) {
+  __shared__ Real sv[4];
+  __shared__ int si[4];
+  __shared__ int s_piv, s_act;
+  __shared__ Real sL[LU_NB * LU_NB];
+  const int b = blockIdx.x;
+  if (b >= nblk) return;
+  const int n = nsz[b];
+  if (n <= 0 || n > nmax) {
+    if (threadIdx.x == 0 && info) info[b] = -1;
+    return;
+  }
+  Real* a = A + off[b];
+  int32_t* ipiv_b = ipiv ? ipiv + pivoff[b] : nullptr;
+  int inf = 0;
+  for (int k0 = 0; k0 < n; k0 += LU_NB) {
+    const int nb = n - k0 < LU_NB ? n - k0 : LU_NB;
+    const unsigned zmask = lu_panel(a, n, k0, nb, ipiv_b, inf, sv, si, &s_piv, &s_act);
+    if (k0 + nb < n) {   // a full panel with columns to its right
+      lu_trsm(a, n, k0, zmask, (int)threadIdx.x, 256, sL);
+      __syncthreads();
+      lu_trail(a, n, k0, zmask, (int)(threadIdx.x >> 6), 4);
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0 && info) info[b] = inf;
+}
+
+// The same for few large blocks, two launches per panel.  k_lu_panel (one workgroup per block)
+// factors the panel and solves U12; st[b] holds the block's INFO so far (-1: invalid size),
+// st[nblk + b] the panel's zero-pivot mask.  k_lu_trail spreads the trailing product of every
+// block over gridDim.x workgroups per block.
+template <typename Real>
+__global__ void __launch_bounds__(256) k_lu_panel(int k0, int nblk, int nmax, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
+                                                  Real* __restrict__ A, int32_t* __restrict__ ipiv, const int64_t* __restrict__ pivoff,
+                                                  int32_t* __restrict__ st) {
+  __shared__ Real sv[4];
+  __shared__ int si[4];
+  __shared__ int s_piv, s_act;
+  __shared__ Real sL[LU_NB * LU_NB];
+  const int b = blockIdx.x;
+  if (b >= nblk) return;
+  const int n = nsz[b];
+  if (k0 == 0 && threadIdx.x == 0) st[b] = (n <= 0 || n > nmax) ? -1 : 0;
+  if (n <= 0 || n > nmax || k0 >= n) return;
+  Real* a = A + off[b];
+  int inf = k0 == 0 ? 0 : st[b];
+  const int nb = n - k0 < LU_NB ? n - k0 : LU_NB;
+  const unsigned zmask = lu_panel(a, n, k0, nb, ipiv ? ipiv + pivoff[b] : nullptr, inf, sv, si, &s_piv, &s_act);
+  if (k0 + nb < n) lu_trsm(a, n, k0, zmask, (int)threadIdx.x, 256, sL);
+  if (threadIdx.x == 0) { st[b] = inf; st[nblk + b] = (int32_t)zmask; }
 }
 
 template <typename Real>
-__global__ void __launch_bounds__(256) k_lu_update(int j, int nblk, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
-                                                   Real* __restrict__ A, const int32_t* __restrict__ st) {
+__global__ void __launch_bounds__(256) k_lu_trail(int k0, int nblk, int nmax, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
+                                                  Real* __restrict__ A, const int32_t* __restrict__ st) {
   const int b = blockIdx.y;
   if (b >= nblk) return;
-  const bool act1 = st[nblk + b] != 0, act2 = st[2 * nblk + b] != 0;
-  if (!act1 && !act2) return;
   const int n = nsz[b];
-  const int i = j + 2 + blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (i >= n) return;
-  Real* a = A + off[b];
-  const Real l = act1 ? a[(size_t)i * n + j] : Real(0);
-  const Real m = act2 ? a[(size_t)i * n + j + 1] : Real(0);
-  for (int c = j + 2 + lane; c < n; c += 64) {
-    Real x = a[(size_t)i * n + c];
-    if (act1) x -= l * a[(size_t)j * n + c];
-    if (act2) x -= m * a[(size_t)(j + 1) * n + c];
-    a[(size_t)i * n + c] = x;
-  }
+  if (n <= 0 || n > nmax || k0 + LU_NB >= n) return;
+  lu_trail(A + off[b], n, k0, (unsigned)st[nblk + b], (int)(blockIdx.x * 4 + (threadIdx.x >> 6)), (int)(gridDim.x * 4));
 }
 
 __global__ void k_lu_info(int nblk, const int32_t* __restrict__ st, int32_t* __restrict__ info) {
@@ -649,14 +680,14 @@ int select_device(bosss_ctx* ctx) {
 }
 
 // ---- scratch of the allocation-free factorisation (bosss_hip_block_lu_noalloc_dev) ----
-// The split form keeps three status words per block between its launches.  The no-alloc entry
+// The split form keeps two status words per block between its launches.  The no-alloc entry
 // point takes them from this file-scope array instead of a per-call allocation: every device the
-// library runs on holds its own copy (12 KiB) until the process ends, and every call on one device
+// library runs on holds its own copy (8 KiB) until the process ends, and every call on one device
 // shares it, so calls that overlap on one device race.  Every word a call reads it has written
-// itself earlier in the same call (k_lu_step writes a block's words before k_lu_update reads
-// them), so nothing needs clearing between calls.
+// itself earlier in the same call (k_lu_panel writes a block's words before k_lu_trail and the
+// next k_lu_panel read them), so nothing needs clearing between calls.
 constexpr int LU_NOALLOC_MAXBLK = 1024;
-static __device__ __align__(256) int32_t scratch[3 * LU_NOALLOC_MAXBLK];
+static __device__ __align__(256) int32_t scratch[2 * LU_NOALLOC_MAXBLK];
 
 namespace {
 constexpr int LU_NOALLOC_MAXDEV = 64;
@@ -732,14 +763,14 @@ static int lu_launch_reg(int nblk, int nmax, const int32_t* n, const int64_t* of
   return BOSSS_OK;
 }
 
-// The split form (two launches per pair of columns) with its per-block status words st[3 nblk]
-// in device memory the caller provides; queued on s, not synchronised.
+// The split form (two launches per panel) with its per-block status words st[2 nblk] in device
+// memory the caller provides; queued on s, not synchronised.
 static hipError_t lu_launch_split(int nblk, int nmax, const int32_t* n, const int64_t* off, double* A, int32_t* ipiv,
                                   const int64_t* pivoff, int32_t* info, int32_t* st, hipStream_t s) {
-  for (int j = 0; j < nmax; j += 2) {
-    hipLaunchKernelGGL(k_lu_step<double>, dim3(nblk), dim3(256), 0, s, j, nblk, nmax, n, off, A, ipiv, pivoff, st);
-    const int left = nmax - j - 2;
-    if (left > 0) hipLaunchKernelGGL(k_lu_update<double>, dim3((unsigned)((left + 3) / 4), nblk), dim3(256), 0, s, j, nblk, n, off, A, st);
+  for (int k0 = 0; k0 < nmax; k0 += LU_NB) {
+    hipLaunchKernelGGL(k_lu_panel<double>, dim3(nblk), dim3(256), 0, s, k0, nblk, nmax, n, off, A, ipiv, pivoff, st);
+    const int m = nmax - k0 - LU_NB;   // trailing size of the largest block
+    if (m > 0) hipLaunchKernelGGL(k_lu_trail<double>, dim3((unsigned)((m + 63) / 64), nblk), dim3(256), 0, s, k0, nblk, nmax, n, off, A, st);
   }
   if (info) hipLaunchKernelGGL(k_lu_info, dim3((nblk + 255) / 256), dim3(256), 0, s, nblk, st, info);
   return hipGetLastError();
@@ -758,9 +789,9 @@ int bosss_hip_block_lu_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t
   if (want_inverse) { g_lu_err = "inverses are limited to blocks of 128 DOF"; return BOSSS_ERR_UNSUPPORTED; }
   hipStream_t s = (hipStream_t)stream;
   if (nmax >= LU_SPLIT_MINN && nblk < LU_SPLIT_MAXBLK) {
-    // few large blocks: two launches per pair of columns, the trailing update over many workgroups
+    // few large blocks: two launches per panel, the trailing update over many workgroups
     int32_t* st = nullptr;
-    hipError_t e = hipMalloc((void**)&st, 3 * (size_t)nblk * sizeof(int32_t));
+    hipError_t e = hipMalloc((void**)&st, 2 * (size_t)nblk * sizeof(int32_t));
     if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
     e = lu_launch_split(nblk, nmax, n, off, A, ipiv, pivoff, info, st, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);   // st is freed next
@@ -768,7 +799,7 @@ int bosss_hip_block_lu_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t
     if (e != hipSuccess) { g_lu_err = hipGetErrorString(e); return BOSSS_ERR_HIP; }
     return BOSSS_OK;
   }
-  hipLaunchKernelGGL(k_block_lu_big<double>, dim3(nblk), dim3(256), 0, s, nblk, nmax, n, off, A, ipiv, pivoff, info);
+  hipLaunchKernelGGL(k_block_lu_blocked<double>, dim3(nblk), dim3(256), 0, s, nblk, nmax, n, off, A, ipiv, pivoff, info);
   hipError_t eb = hipGetLastError();
   if (eb != hipSuccess) { g_lu_err = hipGetErrorString(eb); return BOSSS_ERR_HIP; }
   return BOSSS_OK;

@@ -187,10 +187,11 @@ int bosss_hip_op_spmv_tiles(const bosss_op* op, int32_t* chosen, int32_t* ncand,
  *      Blocks are dense row-major at element offsets off[b], sizes n[b] <= nmax <= 4096:
  *      nmax <= 128 holds each block in registers (up to 20 DOF several blocks per
  *      wavefront, above that one 256-thread workgroup per block); larger nmax
- *      factors in place in global memory with the same pivoting and operation order
- *      (one 256-thread workgroup per block, or, for fewer than 256 blocks with nmax >= 512,
- *      two launches per pair of columns with the trailing update over many workgroups;
- *      that form synchronises the stream before it returns).
+ *      factors in place in global memory, blocked: getf2 on 32-column panels (the same
+ *      pivots), the trailing update as a rank-32 product on the matrix units (one
+ *      256-thread workgroup per block, or, for fewer than 256 blocks with nmax >= 512,
+ *      two launches per panel with the trailing update over many workgroups; that form
+ *      synchronises the stream before it returns).
  *      ipiv (1-based, LAPACK convention) at pivoff[b]; info[b] as DGETRF's INFO
  *      (0, or j+1 for the first zero pivot; -1 if n[b] exceeds nmax).  The inverse
  *      (want_inverse) needs nmax <= 128.  In
@@ -204,11 +205,11 @@ int bosss_hip_block_lu_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t
                            int32_t* ipiv, const int64_t* pivoff, int32_t* info, int32_t want_inverse, void* stream);
 /* The same factorisation without allocating: no counterpart in the reference; for callers that
  * cannot allocate device memory (code under HIP graph capture, tight solver loops, hosts that
- * forbid hipMalloc).  Blocks above 128 DOF always use the split form (two launches per pair of
- * columns, same pivots and factors as above) with its per-block status words in a fixed device
- * array of the library (12 KiB per device, held until the process ends), so at most 1024 such
- * blocks per call (BOSSS_ERR_ARG beyond); up to 128 DOF it is the LDS kernel above, which needs
- * no scratch.  The call only queues work on `stream` and never synchronises.  Every call on one
+ * forbid hipMalloc).  Blocks above 128 DOF always use the split form (two launches per panel,
+ * same pivots and factors as above) with its per-block status words in a fixed device
+ * array of the library (8 KiB per device, held until the process ends), so at most 1024 such
+ * blocks per call (BOSSS_ERR_ARG beyond); up to 128 DOF it is the register kernels above,
+ * which need no scratch.  The call only queues work on `stream` and never synchronises.  Every call on one
  * device shares that array: calls that overlap on one device race. */
 int bosss_hip_block_lu_noalloc_dev(bosss_ctx* ctx, int nblk, int32_t nmax, const int32_t* n, const int64_t* off, double* A,
                                    int32_t* ipiv, const int64_t* pivoff, int32_t* info, int32_t want_inverse, void* stream);
