@@ -729,8 +729,8 @@ static int lu_check_args(bosss_ctx* ctx, int32_t nmax, const int32_t* n, const i
 static int lu_launch_reg(int nblk, int nmax, const int32_t* n, const int64_t* off, double* A, int32_t* ipiv, const int64_t* pivoff,
                          int32_t* info, int want_inverse, hipStream_t s) {
   if (nmax <= 20) {
-    // rows in registers, 64 / G blocks per wavefront (larger blocks would no longer unroll into
-    // registers: those take the LDS kernel)
+    // rows in registers, 64 / G blocks per wavefront (larger rows would not fit one lane's
+    // registers: those blocks take the workgroup-tile kernel)
     if (nmax <= 16)
       hipLaunchKernelGGL((k_block_lu_rows<double, 16, 16>), dim3((unsigned)((nblk + 15) / 16)), dim3(256), 0, s, nblk, nmax, n, off, A,
                          ipiv, pivoff, info, want_inverse);

@@ -2,8 +2,9 @@
 and checked against bosss_hip_block_lu_dev on the same blocks: two calls in a row (the second with
 fewer and smaller blocks, so stale scratch from the first would show), large blocks (split form,
 its per-block status words in the library's fixed device array), mid-size blocks (the existing
-entry point factors them one workgroup per block, the no-alloc one in the split form: same
-pivoting and operation order), small blocks with inverse (LDS kernel), zero pivots, and the
+entry point factors them one workgroup per block, the no-alloc one in the split form: the same
+panels, substitutions and tile products), small blocks with inverse (register kernels), zero
+pivots, and the
 refusal of a batch beyond the scratch size before anything is launched."""
 import ctypes
 
