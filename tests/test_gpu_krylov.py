@@ -56,6 +56,28 @@ def test_device_gmres_matches_host_gmres(restart, n):
         assert float((xd - xh).abs().max()) <= 1e-12 * float(xh.abs().max())
 
 
+
+@pytest.mark.parametrize("n", [300, 301])          # 4-float packs; one element per thread
+def test_device_gmres_fp32_matches_host_gmres(n):
+    """FP32 Krylov vectors (FP64 dots and Hessenberg matrix): the device solver against the host
+    restatement on the same FP32 operator, to 1e-5, and the solution against numpy's FP64 solve."""
+    import torch
+    from bosss_amd.newton import DeviceGmres, gmres
+    say("case device gmres fp32 n", n)
+    A, b, At, bt, _ = dense_case(n)
+    A32, b32 = At.float(), bt.float()
+
+    def apply32(v, out):
+        torch.mv(A32, v, out=out)
+    xh, ith, rh = gmres(apply32, b32, 1e-6, 20, 200)
+    dg = DeviceGmres(n, torch.float32, b32.device, 20)
+    xd, itd, rd = dg.solve(apply32, b32, 1e-6, 200, check_every=4)
+    say("  host %d it rel %.2e | device %d it rel %.2e" % (ith, rh, itd, rd))
+    assert abs(itd - ith) <= 1 and rd <= 1e-6
+    xs = np.linalg.solve(A, b)
+    assert np.abs(xd.double().cpu().numpy() - xs).max() <= 1e-5 * np.abs(xs).max()
+    assert float((xd - xh).abs().max()) <= 1e-5 * float(xh.abs().max())
+
 def test_device_gmres_graph_replays_match_eager():
     import torch
     from bosss_amd.newton import DeviceGmres
