@@ -372,12 +372,12 @@ constexpr int LU_SPLIT_MAXBLK = 256;
 // Returns the pivot row and, in act, whether the pivot is non-zero.  Thread 0 reads the pivot
 // entry before the final barrier, so every thread decides on the same value even though the
 // row swap that follows overwrites it (NaN counts as non-zero, as in DGETF2).
-template <typename Real>
+template <typename Real, int NT = 256>
 __device__ __forceinline__ int lu_pivot_wg(const Real* a, int n, int j, Real* sv, int* si, int* s_piv, int* s_act, bool& act) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   Real best = Real(-1);
   int bi = n;
-  for (int i = j + t; i < n; i += 256) {
+  for (int i = j + t; i < n; i += NT) {
     Real v = fabs(a[(size_t)i * n + j]);
     if (i == j && v != v) v = Real(INFINITY);   // idamax: a NaN first element is never beaten
     if (v > best || (v == best && i < bi)) { best = v; bi = i; }
@@ -388,7 +388,7 @@ __device__ __forceinline__ int lu_pivot_wg(const Real* a, int n, int j, Real* sv
   if (t == 0) {
     Real bv = sv[0];
     int bx = si[0];
-    for (int k = 1; k < 4; k++)
+    for (int k = 1; k < NT / 64; k++)
       if (sv[k] > bv || (sv[k] == bv && si[k] < bx)) { bv = sv[k]; bx = si[k]; }
     *s_piv = bx < n ? bx : j;   // no comparable candidate (all NaN): the diagonal, as idamax
     *s_act = a[(size_t)*s_piv * n + j] != Real(0);
@@ -398,9 +398,9 @@ __device__ __forceinline__ int lu_pivot_wg(const Real* a, int n, int j, Real* sv
   return *s_piv;
 }
 
-template <typename Real>
+template <typename Real, int NT = 256>
 __device__ __forceinline__ void lu_swap_rows(Real* a, int n, int r1, int r2) {
-  for (int c = threadIdx.x; c < n; c += 256) {
+  for (int c = threadIdx.x; c < n; c += NT) {
     const Real x = a[(size_t)r1 * n + c];
     a[(size_t)r1 * n + c] = a[(size_t)r2 * n + c];
     a[(size_t)r2 * n + c] = x;
@@ -420,14 +420,14 @@ typedef double lu_f64x4 __attribute__((ext_vector_type(4)));
 
 // Panel [k0, k0 + nb) of block a (n x n, row-major), one workgroup.  Returns the panel's zero-pivot
 // mask (bit jj: column k0 + jj) and sets inf to the first zero pivot (1-based) if it is still 0.
-template <typename Real>
+template <typename Real, int NT = 256>
 __device__ __forceinline__ unsigned lu_panel(Real* a, int n, int k0, int nb, int32_t* ipiv_b, int& inf, Real* sv, int* si, int* s_piv, int* s_act) {
   const int t = threadIdx.x;
   const int kend = k0 + nb;
   unsigned zmask = 0;
   for (int j = k0; j < kend; j++) {
     bool act;
-    const int p = lu_pivot_wg(a, n, j, sv, si, s_piv, s_act, act);
+    const int p = lu_pivot_wg<Real, NT>(a, n, j, sv, si, s_piv, s_act, act);
     if (t == 0 && ipiv_b) ipiv_b[j] = p + 1;
     if (!act) {
       if (!inf) inf = j + 1;
@@ -435,14 +435,14 @@ __device__ __forceinline__ unsigned lu_panel(Real* a, int n, int k0, int nb, int
       __syncthreads();
       continue;
     }
-    if (p != j) lu_swap_rows(a, n, j, p);
+    if (p != j) lu_swap_rows<Real, NT>(a, n, j, p);
     __syncthreads();
     const Real inv = Real(1) / a[(size_t)j * n + j];
-    for (int i = j + 1 + t; i < n; i += 256) a[(size_t)i * n + j] *= inv;
+    for (int i = j + 1 + t; i < n; i += NT) a[(size_t)i * n + j] *= inv;
     __syncthreads();
     const int w = kend - j - 1;   // panel columns right of j
     if (w > 0)
-      for (int e = t; e < (n - j - 1) * w; e += 256) {
+      for (int e = t; e < (n - j - 1) * w; e += NT) {
         const int i = j + 1 + e / w, c = j + 1 + e % w;
         a[(size_t)i * n + c] -= a[(size_t)i * n + j] * a[(size_t)j * n + c];
       }
@@ -559,12 +559,14 @@ __global__ void __launch_bounds__(256) k_block_lu_blocked(int nblk, int nmax, co
 // factors the panel and solves U12; st[b] holds the block's INFO so far (-1: invalid size),
 // st[nblk + b] the panel's zero-pivot mask.  k_lu_trail spreads the trailing product of every
 // block over gridDim.x workgroups per block.
+constexpr int LU_PANEL_THREADS = 512;   // few blocks: the panel's column steps get 8 wavefronts
 template <typename Real>
-__global__ void __launch_bounds__(256) k_lu_panel(int k0, int nblk, int nmax, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
-                                                  Real* __restrict__ A, int32_t* __restrict__ ipiv, const int64_t* __restrict__ pivoff,
-                                                  int32_t* __restrict__ st) {
-  __shared__ Real sv[4];
-  __shared__ int si[4];
+__global__ void __launch_bounds__(LU_PANEL_THREADS) k_lu_panel(int k0, int nblk, int nmax, const int32_t* __restrict__ nsz,
+                                                               const int64_t* __restrict__ off, Real* __restrict__ A,
+                                                               int32_t* __restrict__ ipiv, const int64_t* __restrict__ pivoff,
+                                                               int32_t* __restrict__ st) {
+  __shared__ Real sv[LU_PANEL_THREADS / 64];
+  __shared__ int si[LU_PANEL_THREADS / 64];
   __shared__ int s_piv, s_act;
   __shared__ Real sL[LU_NB * LU_NB];
   const int b = blockIdx.x;
@@ -575,8 +577,8 @@ __global__ void __launch_bounds__(256) k_lu_panel(int k0, int nblk, int nmax, co
   Real* a = A + off[b];
   int inf = k0 == 0 ? 0 : st[b];
   const int nb = n - k0 < LU_NB ? n - k0 : LU_NB;
-  const unsigned zmask = lu_panel(a, n, k0, nb, ipiv ? ipiv + pivoff[b] : nullptr, inf, sv, si, &s_piv, &s_act);
-  if (k0 + nb < n) lu_trsm(a, n, k0, zmask, (int)threadIdx.x, 256, sL);
+  const unsigned zmask = lu_panel<Real, LU_PANEL_THREADS>(a, n, k0, nb, ipiv ? ipiv + pivoff[b] : nullptr, inf, sv, si, &s_piv, &s_act);
+  if (k0 + nb < n) lu_trsm(a, n, k0, zmask, (int)threadIdx.x, LU_PANEL_THREADS, sL);
   if (threadIdx.x == 0) { st[b] = inf; st[nblk + b] = (int32_t)zmask; }
 }
 
@@ -768,7 +770,7 @@ static int lu_launch_reg(int nblk, int nmax, const int32_t* n, const int64_t* of
 static hipError_t lu_launch_split(int nblk, int nmax, const int32_t* n, const int64_t* off, double* A, int32_t* ipiv,
                                   const int64_t* pivoff, int32_t* info, int32_t* st, hipStream_t s) {
   for (int k0 = 0; k0 < nmax; k0 += LU_NB) {
-    hipLaunchKernelGGL(k_lu_panel<double>, dim3(nblk), dim3(256), 0, s, k0, nblk, nmax, n, off, A, ipiv, pivoff, st);
+    hipLaunchKernelGGL(k_lu_panel<double>, dim3(nblk), dim3(LU_PANEL_THREADS), 0, s, k0, nblk, nmax, n, off, A, ipiv, pivoff, st);
     const int m = nmax - k0 - LU_NB;   // trailing size of the largest block
     if (m > 0) hipLaunchKernelGGL(k_lu_trail<double>, dim3((unsigned)((m + 63) / 64), nblk), dim3(256), 0, s, k0, nblk, nmax, n, off, A, st);
   }
