@@ -146,21 +146,27 @@ def copy_calibration(dev, nbytes=1 << 30):
 
 def measured_ceilings(op, dev, nbytes=1 << 30):
     """The device's ceilings measured in this process: dense FP64 / FP32 MFMA and FP64 FMA
-    issue rates (bosss_hip_measure_peaks), and the HBM rate of a device-to-device copy of
-    nbytes (read + write bytes over the best of three copies)."""
+    issue rates (bosss_hip_measure_peaks), the FP64 rate of MFMA and FMA interleaved at equal
+    flops (bosss_hip_measure_fp64_mix: about either single rate when the two share one FP64
+    rate, about their sum if they ran side by side), and the HBM rate of a device-to-device copy
+    of nbytes (read + write bytes over the best of three copies)."""
     import ctypes
     import torch
     from bosss_amd._lib import lib
-    f64, f32, fma = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    f64, f32, fma, mix = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
     rc = lib().bosss_hip_measure_peaks(op.ctx.h, ctypes.byref(f64), ctypes.byref(f32), ctypes.byref(fma))
     if rc != 0:
         raise RuntimeError("bosss_hip_measure_peaks: %s" % lib().bosss_hip_peaks_last_error().decode())
+    rc = lib().bosss_hip_measure_fp64_mix(op.ctx.h, ctypes.byref(mix))
+    if rc != 0:
+        raise RuntimeError("bosss_hip_measure_fp64_mix: %s" % lib().bosss_hip_peaks_last_error().decode())
     a = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
     b = torch.empty_like(a)
     ms = min(_time_ms(lambda: b.copy_(a), 1, 1) for _ in range(3))
     del a, b
     return {"fp64_mfma_tflops": round(f64.value, 2), "fp32_mfma_tflops": round(f32.value, 2),
-            "fp64_fma_tflops": round(fma.value, 2), "hbm_copy_GBps": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
+            "fp64_fma_tflops": round(fma.value, 2), "fp64_mfma_fma_interleaved_tflops": round(mix.value, 2),
+            "hbm_copy_GBps": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
 
 
 def _time_ms(fn, steps, warmup):

@@ -96,6 +96,7 @@ SIGNATURES = {
     "bosss_hip_xdg_last_error": (ctypes.c_char_p, []),
     "bosss_hip_measure_peaks": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                                ctypes.POINTER(ctypes.c_double)]),
+    "bosss_hip_measure_fp64_mix": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
     "bosss_hip_peaks_last_error": (ctypes.c_char_p, []),
     "bosss_hip_gmres_workspace": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
     "bosss_hip_gmres_arnoldi_dev": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp,

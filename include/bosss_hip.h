@@ -350,6 +350,11 @@ const char* bosss_hip_krylov_last_error(void);
  * issue rates in TFLOP/s, best of three launches of independent accumulation chains over
  * the whole device (a few milliseconds each). */
 int bosss_hip_measure_peaks(bosss_ctx* ctx, double* tflops_fp64_mfma, double* tflops_fp32_mfma, double* tflops_fp64_fma);
+/* FP64 rate (TFLOP/s, MFMA and FMA flops together) of a stream that follows every
+ * v_mfma_f64_16x16x4f64 with 16 FP64 FMAs (equal flops): near the sum of the two single rates
+ * if the matrix core ran beside the FP64 VALU, near either single rate if they share one FP64
+ * rate (the ceiling of a kernel that issues both, as the fused Euler residual + VJP does). */
+int bosss_hip_measure_fp64_mix(bosss_ctx* ctx, double* tflops_fp64_mix);
 const char* bosss_hip_peaks_last_error(void);
 
 #ifdef __cplusplus

@@ -36,7 +36,7 @@
   X(bell_spmv_dev) X(op_assemble) X(bell_spmv) X(op_set_spmv_tile) X(op_spmv_tiles) X(block_lu_dev) X(block_lu_noalloc_dev) X(block_solve_dev)       \
   X(block_gemv_dev) X(block_lu) X(block_solve) X(block_gemv) X(block_last_error) X(xdg_create) X(xdg_destroy) X(xdg_info)     \
   X(xdg_assemble_dev) X(xdg_spmv_dev) X(xdg_solve_dev) X(xdg_assemble) X(xdg_spmv) X(xdg_solve) X(xdg_set_tile) X(xdg_tiles) X(xdg_last_error)            \
-  X(gmres_workspace) X(gmres_arnoldi_dev) X(gmres_update_dev) X(block_gemv_uniform_dev) X(krylov_last_error) X(measure_peaks) X(peaks_last_error)
+  X(gmres_workspace) X(gmres_arnoldi_dev) X(gmres_update_dev) X(block_gemv_uniform_dev) X(krylov_last_error) X(measure_peaks) X(measure_fp64_mix) X(peaks_last_error)
 
 struct Api {
 #define FIELD(n) decltype(&bosss_hip_##n) n = nullptr;

@@ -40,6 +40,22 @@ def test_measured_peaks_are_near_the_published_ceilings():
         assert 0.5 * pub < got < 1.1 * pub
 
 
+def test_measured_fp64_mix_rate():
+    """bosss_hip_measure_fp64_mix: FP64 MFMA and FMA interleaved at equal flops run at a rate
+    between half the published FP64 figure and the sum of the two single rates (whether the two
+    share one FP64 rate, as measured on MI355X, is reported by the benchmark, not asserted)."""
+    import ctypes
+    from bosss_amd._lib import lib
+    from bosss_amd.operator import Context
+    print("case measure_fp64_mix", flush=True)
+    ctx = Context.get(0)
+    mix = ctypes.c_double()
+    rc = lib().bosss_hip_measure_fp64_mix(ctx.h, ctypes.byref(mix))
+    assert rc == 0, lib().bosss_hip_peaks_last_error().decode()
+    print("  FP64 MFMA + FMA interleaved: %.1f TFLOP/s" % mix.value, flush=True)
+    assert 0.5 * 78.6 < mix.value < 1.1 * 2 * 78.6
+
+
 def test_device_wrappers_refuse_mismatched_tensors():
     """The device wrappers check dtype, size, contiguity and device of every tensor before any
     launch (the C entry points take bare pointers): short, FP32-for-FP64, strided and host
