@@ -50,6 +50,13 @@ struct TShape {
   static constexpr int NK = (N + 3) / 4, MT = (N + 15) / 16;
   static constexpr int O_L = 0, O_DL = (P + 1) * K2, O_W = 2 * (P + 1) * K2, O_CODE = O_W + K1;
   static constexpr int TAB = O_CODE + N;     // elements (Real) of the 1D table block
+  // Face trace table FT[f][node][m] = phi_m at node `node` of face f, stored after the 1D block
+  // in global memory: FROWS rows per face (whole 16-node tiles; the padding rows node >= KE hold
+  // node 0's values, so a trace evaluated at them is a valid state), FS Real per row (modes
+  // m >= N and the columns FZERO, FZERO + 1 are zero).  FS = 4 NK + 2 keeps the LDS reads of
+  // both contractions (lanes along nodes, or along modes) free of bank conflicts or 2-way.
+  static constexpr int FROWS = TE * 16, FS = NK * 4 + 2, FZERO = NK * 4;
+  static constexpr int FTAB = NF * FROWS * FS;
 };
 
 // Per-axis 1D indices of a node: volume node kv = (i*K1 + j)*K1 + k has x = k, y = j, z = i
@@ -186,6 +193,60 @@ __device__ __forceinline__ void t_test(const Real* tab, const int (&q)[3], int d
   }
 }
 
+// Copy the face trace table (global memory, after the 1D block) into LDS at ft.
+template <class TS, typename Real>
+__device__ __forceinline__ void ft_load(Real* ft, const Real* gtab) {
+  for (int i = threadIdx.x; i < TS::FTAB; i += blockDim.x) ft[i] = gtab[TS::TAB + i];
+}
+
+// Trace contraction from the face trace table: acc[v] (nodes x cells) = FT(face, node, mode) . B.
+// `row` points at FT(face, t * 16 + col, grp) of the lane; its A operands are row[4 kk], LDS reads
+// at immediate offsets (the padding modes are zero in the table).
+template <class TS, typename Real, int V, class BF>
+__device__ __forceinline__ void t_eval_ft(const Real* row, BF Bsrc, typename MM<Real>::acc (&acc)[V]) {
+#pragma unroll
+  for (int v = 0; v < V; v++) acc[v] = acc_zero<Real>();
+#pragma unroll
+  for (int kk = 0; kk < TS::NK; kk++) {
+    const Real a = row[4 * kk];
+#pragma unroll
+    for (int v = 0; v < V; v++) acc[v] = MM<Real>::mma(a, Bsrc(v, kk), acc[v]);
+  }
+}
+
+// The lane's mode columns of the face trace table in the test contraction: mode mt * 16 + col of
+// each 16-mode tile (padding modes >= N clamped to the zero column), and of the 4-mode remainder.
+template <class TS, typename Real, int V>
+struct FtCols {
+  typedef TAcc<TS, Real, V> A_;
+  int m[A_::MF];
+  int rem;
+  __device__ __forceinline__ explicit FtCols(int col) {
+#pragma unroll
+    for (int mt = 0; mt < A_::MF; mt++) m[mt] = mt * 16 + col < TS::N ? mt * 16 + col : TS::FZERO;
+    rem = A_::MF * 16 + (col & 3);
+  }
+};
+
+// Test contraction from the face trace table: `nrow` points at FT(face, node, 0) of the lane's
+// node; same accumulators and MFMA shapes as t_test.
+template <class TS, typename Real, int V>
+__device__ __forceinline__ void t_test_ft(const Real* nrow, const FtCols<TS, Real, V>& mc, const Real (&g)[V],
+                                          TAcc<TS, Real, V>& acc) {
+  typedef TAcc<TS, Real, V> A_;
+#pragma unroll
+  for (int mt = 0; mt < A_::MF; mt++) {
+    const Real a = nrow[mc.m[mt]];
+#pragma unroll
+    for (int v = 0; v < V; v++) acc.t[v][mt] = MM<Real>::mma(a, g[v], acc.t[v][mt]);
+  }
+  if constexpr (A_::REM) {
+    const Real a = nrow[mc.rem];
+#pragma unroll
+    for (int v = 0; v < V; v++) acc.rem[v] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, g[v], acc.rem[v], 0, 0, 0);
+  }
+}
+
 template <class TS, typename Real>
 __device__ __forceinline__ Real vol_weight(const Real* tab, const int (&q)[3]) {
   const Real* w = tab + TS::O_W;
@@ -245,6 +306,11 @@ template <int D> struct IsHLLC<FormEuler<D, true>> : std::true_type {};
 
 constexpr int TWAVES = 4;
 
+// kt_forward reads its value traces from the face trace table when the form's faces need no
+// gradients (the Euler fluxes).
+template <class Form>
+constexpr bool forward_face_table() { return !Form::FACE_GRADU && !Form::FACE_TESTG; }
+
 // Waves per SIMD the residual/JVP (kt_forward) and linear-form adjoint (kt_adjoint) kernels
 // are compiled for.  Their LDS use is small, so registers alone set the occupancy: FP32, and
 // FP64 up to p = 2, fit two waves with little or no spilling and gain 13-28 % from the
@@ -276,9 +342,15 @@ __global__ void __launch_bounds__(64 * TWAVES, (tensor_min_waves<Real, P>())) kt
   const int cell0 = (xcd_block(a.nxcd) * TWAVES + wave) * 16;
   Real* stU = tab + ((TS::TAB + 1) & ~1) + wave * (NARR * 16 * STRIDE);
   Real* stX = stU + 16 * STRIDE;
+  // face trace table: only for forms whose faces use values alone (the gradient traces of the
+  // other forms outweigh the value traces it saves, and its load costs short workgroups more)
+  constexpr bool FT = forward_face_table<Form>();
+  Real* ft = tab + ((TS::TAB + 1) & ~1) + TWAVES * NARR * 16 * STRIDE;
+  if constexpr (FT) ft_load<TS>(ft, a.tab);
   stage16<Real, NT>(stU, STRIDE, a.U, cell0, a.J, lane);
   if (TAN) stage16<Real, NT>(stX, STRIDE, a.X, cell0, a.J, lane);
   __syncthreads();
+  const FtCols<TS, Real, V> ftc(col);
 
   const int cell = cell0 + col;
   const bool valid = cell < a.J;
@@ -387,12 +459,24 @@ __global__ void __launch_bounds__(64 * TWAVES, (tensor_min_waves<Real, P>())) kt
       int qo[3], qn[3];
       face_axes<D, K1>(f, nodeE < TS::KE ? nodeE : 0, qo);
       face_axes<D, K1>(fo, nodeE < TS::KE ? nodeE : 0, qn);
+      // value traces from the face trace table (FT) or the 1D tables; gradient traces from the 1D tables
+      const Real* fto = ft + (f * TS::FROWS + t * 16 + col) * TS::FS + grp;
+      const Real* ftn = ft + (fo * TS::FROWS + t * 16 + col) * TS::FS + grp;
       acc_t uo[V], un[V], duo[V], dun[V], guo[D][V], gun[D][V], dguo[D][V], dgun[D][V];
-      t_eval<TS, Real, V>(tab, qo, -1, grp, Bu, uo);
-      t_eval<TS, Real, V>(tab, qn, -1, grp, Bnu, un);
-      if (TAN) {
-        t_eval<TS, Real, V>(tab, qo, -1, grp, Bx, duo);
-        t_eval<TS, Real, V>(tab, qn, -1, grp, Bnx, dun);
+      if constexpr (FT) {
+        t_eval_ft<TS, Real, V>(fto, Bu, uo);
+        t_eval_ft<TS, Real, V>(ftn, Bnu, un);
+        if (TAN) {
+          t_eval_ft<TS, Real, V>(fto, Bx, duo);
+          t_eval_ft<TS, Real, V>(ftn, Bnx, dun);
+        }
+      } else {
+        t_eval<TS, Real, V>(tab, qo, -1, grp, Bu, uo);
+        t_eval<TS, Real, V>(tab, qn, -1, grp, Bnu, un);
+        if (TAN) {
+          t_eval<TS, Real, V>(tab, qo, -1, grp, Bx, duo);
+          t_eval<TS, Real, V>(tab, qn, -1, grp, Bnx, dun);
+        }
       }
       if (Form::FACE_GRADU) {
 #pragma unroll
@@ -412,6 +496,7 @@ __global__ void __launch_bounds__(64 * TWAVES, (tensor_min_waves<Real, P>())) kt
         const bool pad = node >= TS::KE;
         int q[3];
         face_axes<D, K1>(f, pad ? 0 : node, q);
+        const Real* nrow = ft + (f * TS::FROWS + node) * TS::FS;
         const Real wq = pad ? Real(0) : face_weight<TS>(tab, node) * sg * sc;
         T Uo[V], Un[V], Go[V][D], Gn[V][D];
 #pragma unroll
@@ -454,7 +539,8 @@ __global__ void __launch_bounds__(64 * TWAVES, (tensor_min_waves<Real, P>())) kt
           if constexpr (TAN) s = fo_[v].d[0]; else s = fo_[v];
           g[v] = pad ? Real(0) : s * wq;
         }
-        t_test<TS, Real, V>(tab, q, -1, col, g, racc);
+        if constexpr (FT) t_test_ft<TS, Real, V>(nrow, ftc, g, racc);
+        else t_test<TS, Real, V>(tab, q, -1, col, g, racc);
         if (Form::FACE_TESTG) {
 #pragma unroll
           for (int e = 0; e < D; e++) {
@@ -820,6 +906,12 @@ constexpr size_t kt_lds_bytes(int narrays) {
   typedef TShape<D, P, K1> TS;
   return (size_t)(((TS::TAB + 1) & ~1) + TWAVES * narrays * 16 * ((V * TS::N) | 1)) * sizeof(Real);
 }
+// kt_forward: the same, followed by the face trace table for the forms that use it
+template <class Form, int D, int P, int K1, typename Real>
+constexpr size_t kt_forward_lds_bytes(int narrays) {
+  return kt_lds_bytes<D, P, K1, Form::V, Real>(narrays) +
+         (forward_face_table<Form>() ? (size_t)TShape<D, P, K1>::FTAB * sizeof(Real) : 0);
+}
 
 }  // namespace bosss
 
@@ -854,11 +946,14 @@ __global__ void __launch_bounds__(64 * TWAVES, (vjp_euler_min_waves<Real, P, FUS
 
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Real* tab = reinterpret_cast<Real*>(smem_raw);
+  Real* ft = tab + ((TS::TAB + 1) & ~1) + TWAVES * NSLOT * 64;
   for (int i = threadIdx.x; i < TS::TAB; i += blockDim.x) tab[i] = a.tab[i];
+  ft_load<TS>(ft, a.tab);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int col = lane & 15, grp = lane >> 4;
   Real* buf = tab + ((TS::TAB + 1) & ~1) + wave * (NSLOT * 64);
+  const FtCols<TS, Real, V> ftc(col);
   auto slot = [&](int arr, int v, int r) -> Real& { return buf[((arr * V + v) * 4 + r) * 64 + lane]; };
 
   const int cell = (xcd_block(a.nxcd) * TWAVES + wave) * 16 + col;
@@ -1014,18 +1109,17 @@ __global__ void __launch_bounds__(64 * TWAVES, (vjp_euler_min_waves<Real, P, FUS
     auto Bnu = [&](int v, int kk) { int m = 4 * kk + grp; return nbU[v * N + (m < N ? m : N - 1)] * ((has_nb && m < N) ? scn : Real(0)); };
 
     for (int t = 0; t < TS::TE; t++) {
-      const int nodeE = t * 16 + col;
-      int qo[3], qn[3];
-      face_axes<D, K1>(f, nodeE < TS::KE ? nodeE : 0, qo);
-      face_axes<D, K1>(fo, nodeE < TS::KE ? nodeE : 0, qn);
+      // trace operands FT(f, node, mode) and FT(fo, node, mode) of the tile's nodes
+      const Real* fto = ft + (f * TS::FROWS + t * 16 + col) * TS::FS + grp;
+      const Real* ftn = ft + (fo * TS::FROWS + t * 16 + col) * TS::FS + grp;
       {
         acc_t acc[V];
-        t_eval<TS, Real, V>(tab, qo, -1, grp, Bu, acc);
+        t_eval_ft<TS, Real, V>(fto, Bu, acc);
         park(0, acc);
       }
       {
         acc_t acc[V];
-        t_eval<TS, Real, V>(tab, qn, -1, grp, Bnu, acc);
+        t_eval_ft<TS, Real, V>(ftn, Bnu, acc);
         park(1, acc);
       }
       {
@@ -1036,10 +1130,8 @@ __global__ void __launch_bounds__(64 * TWAVES, (vjp_euler_min_waves<Real, P, FUS
 #pragma unroll
         for (int kk = 0; kk < TS::NK; kk++) {
           int m = 4 * kk + grp;
-          int code = tcode<TS>(tab, m);
-          Real mk = mode_mask<TS, Real>(m);
-          Real ao = tval<TS>(tab, qo, code, -1) * mk;
-          Real an = tval<TS>(tab, qn, code, -1) * mk;
+          Real ao = fto[4 * kk];
+          Real an = ftn[4 * kk];
 #pragma unroll
           for (int v = 0; v < V; v++) {
             const int mc = m < N ? m : N - 1;
@@ -1055,8 +1147,7 @@ __global__ void __launch_bounds__(64 * TWAVES, (vjp_euler_min_waves<Real, P, FUS
         if (t * 16 + MM<Real>::row(0, r) >= TS::KE) continue;   // all 4 nodes of this step are padding
         const int node = t * 16 + MM<Real>::row(grp, r);
         const bool pad = node >= TS::KE;
-        int q[3];
-        face_axes<D, K1>(f, pad ? 0 : node, q);
+        const Real* nrow = ft + (f * TS::FROWS + node) * TS::FS;
         const Real wsg = pad ? Real(0) : face_weight<TS>(tab, node) * sg;
         Real Uo[V], Un[V], gbar[V], fo_[V], ub[V];
 #pragma unroll
@@ -1071,11 +1162,11 @@ __global__ void __launch_bounds__(64 * TWAVES, (vjp_euler_min_waves<Real, P, FUS
         if (FUSED) {
 #pragma unroll
           for (int v = 0; v < V; v++) g[v] = pad ? Real(0) : fo_[v] * wsg * sc;
-          t_test<TS, Real, V>(tab, q, -1, col, g, racc);
+          t_test_ft<TS, Real, V>(nrow, ftc, g, racc);
         }
 #pragma unroll
         for (int v = 0; v < V; v++) g[v] = pad ? Real(0) : ub[v] * sc;
-        t_test<TS, Real, V>(tab, q, -1, col, g, aacc);
+        t_test_ft<TS, Real, V>(nrow, ftc, g, aacc);
       }
     }
   }
@@ -1108,11 +1199,14 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
 
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Real* tab = reinterpret_cast<Real*>(smem_raw);
+  Real* ft = tab + ((TS::TAB + 1) & ~1) + TWAVES * NSLOT * 64;
   for (int i = threadIdx.x; i < TS::TAB; i += blockDim.x) tab[i] = a.tab[i];
+  ft_load<TS>(ft, a.tab);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int col = lane & 15, grp = lane >> 4;
   Real* buf = tab + ((TS::TAB + 1) & ~1) + wave * (NSLOT * 64);
+  const FtCols<TS, Real, V> ftc(col);
   auto slot = [&](int arr, int v, int r) -> Real& { return buf[((arr * V + v) * 4 + r) * 64 + lane]; };
 
   const int cell = (xcd_block(a.nxcd) * TWAVES + wave) * 16 + col;
@@ -1230,28 +1324,26 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
     auto Bnx = [&](int v, int kk) { int m = 4 * kk + grp; return nbX[v * N + (m < N ? m : N - 1)] * ((has_nb && !a.own_only && m < N) ? scn : Real(0)); };
 
     for (int t = 0; t < TS::TE; t++) {
-      const int nodeE = t * 16 + col;
-      int qo[3], qn[3];
-      face_axes<D, K1>(f, nodeE < TS::KE ? nodeE : 0, qo);
-      face_axes<D, K1>(fo, nodeE < TS::KE ? nodeE : 0, qn);
+      // trace operands FT(f, node, mode) and FT(fo, node, mode) of the tile's nodes
+      const Real* fto = ft + (f * TS::FROWS + t * 16 + col) * TS::FS + grp;
+      const Real* ftn = ft + (fo * TS::FROWS + t * 16 + col) * TS::FS + grp;
       acc_t dun[V];
       {
         acc_t acc[V];
-        t_eval<TS, Real, V>(tab, qo, -1, grp, Bu, acc);
+        t_eval_ft<TS, Real, V>(fto, Bu, acc);
         park(0, acc);
-        t_eval<TS, Real, V>(tab, qn, -1, grp, Bnu, acc);
+        t_eval_ft<TS, Real, V>(ftn, Bnu, acc);
         park(1, acc);
-        t_eval<TS, Real, V>(tab, qo, -1, grp, Bx, acc);
+        t_eval_ft<TS, Real, V>(fto, Bx, acc);
         park(2, acc);
       }
-      t_eval<TS, Real, V>(tab, qn, -1, grp, Bnx, dun);
+      t_eval_ft<TS, Real, V>(ftn, Bnx, dun);
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         if (t * 16 + MM<Real>::row(0, r) >= TS::KE) continue;   // all 4 nodes of this step are padding
         const int node = t * 16 + MM<Real>::row(grp, r);
         const bool pad = node >= TS::KE;
-        int q[3];
-        face_axes<D, K1>(f, pad ? 0 : node, q);
+        const Real* nrow = ft + (f * TS::FROWS + node) * TS::FS;
         const Real wq = pad ? Real(0) : face_weight<TS>(tab, node) * sg * sc;
         T Uo[V], Un[V], Go[V][D], Gn[V][D];
 #pragma unroll
@@ -1273,7 +1365,7 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
         Real g[V];
 #pragma unroll
         for (int v = 0; v < V; v++) g[v] = pad ? Real(0) : fo_[v].d[0] * wq;
-        t_test<TS, Real, V>(tab, q, -1, col, g, racc);
+        t_test_ft<TS, Real, V>(nrow, ftc, g, racc);
       }
     }
   }
@@ -1284,10 +1376,11 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
   }
 }
 
+// LDS bytes of the Euler VJP / JVP kernels: 1D tables, per-wave node buffers, face trace table
 template <int D, int P, int K1, int V, typename Real>
 constexpr size_t kt_vjp_euler_lds_bytes() {
   typedef TShape<D, P, K1> TS;
-  return (size_t)(((TS::TAB + 1) & ~1) + TWAVES * 3 * V * 4 * 64) * sizeof(Real);
+  return (size_t)(((TS::TAB + 1) & ~1) + TWAVES * 3 * V * 4 * 64 + TS::FTAB) * sizeof(Real);
 }
 
 }  // namespace bosss
