@@ -930,7 +930,8 @@ namespace bosss {
 // inside the register file.
 // ----------------------------------------------------------------------------
 // Waves per SIMD the FP32 Euler VJP kernels are compiled for.  In FP32 the per-lane state fits
-// 256 registers and the LDS staging is 61 KB per workgroup, so two workgroups share a CU and
+// 256 registers and the LDS use is 79 KB per workgroup at p = 3 (node buffers 61 KB, face trace
+// table 17 KB), so two workgroups share a CU and
 // one wave's memory waits overlap the other's arithmetic (the fused p = 3 residual + VJP runs
 // 1.4x faster than at one wave).  The non-fused p = 3 variants reach two waves without the
 // hint and spill with it, so they are left to the compiler.
