@@ -37,10 +37,14 @@ __device__ __forceinline__ void wave_argmax(Real& v, int& idx) {
   }
 }
 
+// Shared state of the register-tile kernel.  The row placement (pos, lrow) is kept once per
+// wavefront: each wavefront applies every exchange to its own copy, so a column step needs one
+// workgroup barrier (publishing the column and the pivot) and no wavefront reads a copy that
+// another one is still writing.
 template <int RA>
 struct TileLDS {
   double col[2][16 * RA], row[2][16 * RA], pv[2];
-  int pos[16 * RA], lrow[16 * RA], piv[16 * RA], p[2];
+  int pos[4][16 * RA], lrow[4][16 * RA], piv[16 * RA], p[2];
 };
 
 // Loads block g (n x n, row-major) into the register tile and resets the row bookkeeping.
@@ -55,16 +59,30 @@ __device__ __forceinline__ void tile_load(double (&a)[RA][RA], const double* __r
       a[x][y] = 0.0;
       if (r < n && c < n) a[x][y] = g[(int64_t)r * n + c];
     }
-  for (int i = tid; i < 16 * RA; i += 256) { L.pos[i] = i; L.lrow[i] = i; }
+  for (int i = tid; i < 4 * 16 * RA; i += 256) {
+    const int w = i / (16 * RA), r = i - w * 16 * RA;
+    L.pos[w][r] = r;
+    L.lrow[w][r] = r;
+  }
   __syncthreads();
 }
 
 // Column steps 0..n-1 of getf2 (GJ = false) or of in-place Gauss-Jordan (GJ = true) on the register
 // tile.  Returns INFO (first zero pivot, 1-based; Gauss-Jordan stops there).  L.piv[j] receives the
-// position of step j's pivot row, L.pos / L.lrow the final row placement.
+// position of step j's pivot row, L.pos / L.lrow the final row placement (every wavefront's copy).
+//
+// One barrier per step: the 16 threads holding column j publish it and the pivot (row, value);
+// after the barrier the thread of each 16-lane group that holds the pivot row's entries of the
+// group's columns c = tc + 16 y publishes them in row[][c], which only that group reads back.
+// Column group tc lives in wavefront tc / 4 and the LDS accesses of one wavefront complete in
+// order, so this exchange needs no workgroup barrier; lane 0 of each wavefront applies the row
+// exchange to that wavefront's copy of the placement.
 template <int RA, bool GJ>
 __device__ __forceinline__ int tile_eliminate(double (&a)[RA][RA], int n, TileLDS<RA>& L) {
-  const int tid = threadIdx.x, tc = tid >> 4, tr = tid & 15;
+  const int tid = threadIdx.x, tc = tid >> 4, tr = tid & 15, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  int* pos = L.pos[w];
+  int* lrow = L.lrow[w];
   unsigned used = 0;   // bit x: row tr + 16 x is in a position <= j (LU form only)
   int inf = 0;
   for (int j = 0; j < n; j++) {
@@ -82,7 +100,7 @@ __device__ __forceinline__ int tile_eliminate(double (&a)[RA][RA], int n, TileLD
             const double v0 = a[x][yy];
             if (r < n) {
               L.col[k][r] = v0;
-              const int ps = L.pos[r];
+              const int ps = pos[r];
               if (ps >= j) {
                 double v = fabs(v0);
                 if (ps == j && v != v) v = (double)INFINITY;   // idamax: a NaN first element is never beaten
@@ -99,7 +117,7 @@ __device__ __forceinline__ int tile_eliminate(double (&a)[RA][RA], int n, TileLD
         if (ov > best || (ov == best && opos < bpos)) { best = ov; bpos = opos; bp = op; bval = oval; }
       }
       if (tr == 0) {
-        if (bp < 0) { bp = L.lrow[j]; bval = L.col[k][bp]; }   // no comparable candidate (all NaN): position j
+        if (bp < 0) { bp = lrow[j]; bval = L.col[k][bp]; }   // no comparable candidate (all NaN): position j
         L.p[k] = bp;
         L.pv[k] = bval;
       }
@@ -107,21 +125,21 @@ __device__ __forceinline__ int tile_eliminate(double (&a)[RA][RA], int n, TileLD
     __syncthreads();
     const int p = __builtin_amdgcn_readfirstlane(L.p[k]);
     const double pv = L.pv[k];
-    if (tid == 0) L.piv[j] = L.pos[p];
+    if (tid == 0) L.piv[j] = pos[p];
     if (!(pv != 0.0)) {   // zero pivot (NaN passes, as in DGETF2)
       if (!inf) inf = j + 1;
       if (GJ) return inf;
-      const int r0 = L.lrow[j];   // no exchange: the row in position j is done all the same
+      const int r0 = lrow[j];   // no exchange: the row in position j is done all the same
       if (tr == (r0 & 15)) used |= 1u << (r0 >> 4);
       continue;
     }
-    if (tid == 0) {
-      const int q = L.pos[p], rj = L.lrow[j];
-      L.pos[p] = j; L.pos[rj] = q; L.lrow[j] = p; L.lrow[q] = rj;
+    if (lane == 0) {   // read by this wavefront's lanes from the next step on
+      const int q = pos[p], rj = lrow[j];
+      pos[p] = j; pos[rj] = q; lrow[j] = p; lrow[q] = rj;
     }
     const int pc = p & 15, px = __builtin_amdgcn_readfirstlane(p >> 4);
     const double pinv = 1.0 / pv;
-    if (tr == pc) {
+    if (tr == pc) {   // the pivot row's entries of the group's columns tc + 16 y
 #pragma unroll
       for (int xx = 0; xx < RA; xx++)
         if (xx == px) {
@@ -132,7 +150,7 @@ __device__ __forceinline__ int tile_eliminate(double (&a)[RA][RA], int n, TileLD
           }
         }
     }
-    __syncthreads();
+    __builtin_amdgcn_wave_barrier();
     double u[RA];
 #pragma unroll
     for (int y = 0; y < RA; y++) u[y] = L.row[k][tc + 16 * y];
@@ -184,15 +202,22 @@ __device__ __forceinline__ void tile_store(const double (&a)[RA][RA], double* __
   for (int x = 0; x < RA; x++) {
     const int r = tr + 16 * x;
     if (r < n) {
-      const int64_t row = (int64_t)L.pos[r] * n;
+      const int64_t row = (int64_t)L.pos[0][r] * n;
 #pragma unroll
       for (int y = 0; y < RA; y++) {
         const int c = tc + 16 * y;
-        if (c < n) g[row + (inverse ? L.lrow[c] : c)] = a[x][y];
+        if (c < n) g[row + (inverse ? L.lrow[0][c] : c)] = a[x][y];
       }
     }
   }
 }
+
+// Waves per SIMD the register-tile kernel is compiled for.  The 100-DOF blocks (RA = 7) with the
+// inverse need 186 VGPRs unconstrained (two waves); at three waves (168 VGPRs, an 88 B per-lane
+// spill) the 262 144 blocks of p = 3 Euler factor and invert in 95 ms instead of 122 ms, the
+// column steps being latency bound (four waves spill 240 B and take 329 ms).
+template <int RA>
+constexpr int lu_tile_waves() { return RA == 7 ? 3 : 1; }
 
 // Blocks of 21..16 RA DOF (the 100-DOF cell blocks of p = 3 Euler, up to 128): one workgroup of
 // 256 threads per block, the whole block in registers.  Thread (tc, tr) = (tid >> 4, tid & 15)
@@ -201,17 +226,18 @@ __device__ __forceinline__ void tile_store(const double (&a)[RA][RA], double* __
 // row r has been given and lrow[] its inverse.  Each column step: the 16 threads holding column j
 // find the pivot among the rows not yet used (largest |a|, ties to the smaller position, a NaN in
 // position j never beaten: getf2's IDAMAX in getf2's row order) with shuffles inside their 16-lane
-// group and publish the column; the threads holding the pivot row publish it; every thread updates
-// its entries.  Factors, pivots and INFO are getf2's, operation for operation (multiplier a * (1 / pivot),
+// group and publish the column; the threads holding the pivot row publish it to their own
+// 16-lane groups; every thread updates its entries.  Factors, pivots and INFO are getf2's, operation for operation (multiplier a * (1 / pivot),
 // then the rank-one update of the columns after j).
 //
 // With INV the elimination is Gauss-Jordan in place (every row other than the pivot row updated at
 // every step: uniform work for all threads) and the inverse is written out with the row and column
 // permutation applied.  A zero pivot stops it; the block is then factored from its original values
 // instead (LU factors and INFO, no inverse, as DGETRF leaves them when DGETRI cannot proceed).  Register columns and rows are
-// picked with wavefront-uniform branches, so the block never leaves the register file.
+// picked with wavefront-uniform branches, so no register array is indexed dynamically (the block
+// stays in registers; RA = 7 with the inverse spills 88 B per lane at the three waves above).
 template <int RA, bool INV>
-__global__ void __launch_bounds__(256) k_block_lu_tile(int nblk, int nmax, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lu_tile_waves<RA>()))) k_block_lu_tile(int nblk, int nmax, const int32_t* __restrict__ nsz, const int64_t* __restrict__ off,
                                                        double* __restrict__ A, int32_t* __restrict__ ipiv,
                                                        const int64_t* __restrict__ pivoff, int32_t* __restrict__ info) {
   __shared__ TileLDS<RA> L;
