@@ -1338,7 +1338,12 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
         t_eval_ft<TS, Real, V>(fto, Bx, acc);
         park(2, acc);
       }
-      t_eval_ft<TS, Real, V>(ftn, Bnx, dun);
+      if (a.own_only) {   // neighbour tangents taken as zero (diagonal-block probes): no contraction
+#pragma unroll
+        for (int v = 0; v < V; v++) dun[v] = acc_zero<Real>();
+      } else {
+        t_eval_ft<TS, Real, V>(ftn, Bnx, dun);
+      }
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         if (t * 16 + MM<Real>::row(0, r) >= TS::KE) continue;   // all 4 nodes of this step are padding
