@@ -283,7 +283,8 @@ static void legendre_normalised(int P, double x, double* L, double* DL) {
 static bool build_tensor_tables(const bosss_tables_desc* t, const KernelSet* ks, int K1, std::vector<double>& tab, double* maxdiff) {
   const int D = t->D, P = ks->P, K2 = ks->t_k2, N = t->N;
   tab.assign(ks->ttab_size, 0.0);
-  if (t->Ke > ks->t_frows || ks->t_o_ft + 2 * D * ks->t_frows * ks->t_fs != ks->ttab_size) return false;
+  if (t->Ke > ks->t_frows || ks->t_ftab != 2 * D * ks->t_frows * ks->t_fs || ks->t_o_ft + (1 + D) * ks->t_ftab != ks->ttab_size)
+    return false;
   std::vector<double> x, w;
   gauss_legendre(K1, x, w);
   std::vector<double> L(P + 1), DL(P + 1);
@@ -341,14 +342,16 @@ static bool build_tensor_tables(const bosss_tables_desc* t, const KernelSet* ks,
         if (f == 0) cmp(w[k], t->face_weights[k]);
       }
       for (int n = 0; n < N; n++) {
-        const double phi = val(q, ex[n], -1);
-        cmp(phi, t->trace[((size_t)f * t->Ke + k) * N + n]);
-        for (int d = 0; d < D; d++) cmp(val(q, ex[n], d), t->dtrace[(((size_t)f * t->Ke + k) * N + n) * D + d]);
-        // face trace table row (f, k); the padding rows Ke .. t_frows-1 repeat node 0
-        double* ftf = &tab[ks->t_o_ft + (size_t)f * ks->t_frows * ks->t_fs];
-        ftf[(size_t)k * ks->t_fs + n] = phi;
-        if (k == 0)
-          for (int kp = t->Ke; kp < ks->t_frows; kp++) ftf[(size_t)kp * ks->t_fs + n] = phi;
+        // face table e (-1: phi, d: d phi / d xi_d), row (f, k); the padding rows Ke .. t_frows-1 repeat node 0
+        for (int e = -1; e < D; e++) {
+          const double v = val(q, ex[n], e);
+          if (e < 0) cmp(v, t->trace[((size_t)f * t->Ke + k) * N + n]);
+          else cmp(v, t->dtrace[(((size_t)f * t->Ke + k) * N + n) * D + e]);
+          double* ftf = &tab[ks->t_o_ft + (size_t)(1 + e) * ks->t_ftab + (size_t)f * ks->t_frows * ks->t_fs];
+          ftf[(size_t)k * ks->t_fs + n] = v;
+          if (k == 0)
+            for (int kp = t->Ke; kp < ks->t_frows; kp++) ftf[(size_t)kp * ks->t_fs + n] = v;
+        }
       }
     }
   *maxdiff = md / scale;

@@ -57,6 +57,8 @@ struct TShape {
   // both contractions (lanes along nodes, or along modes) free of bank conflicts or 2-way.
   static constexpr int FROWS = TE * 16, FS = NK * 4 + 2, FZERO = NK * 4;
   static constexpr int FTAB = NF * FROWS * FS;
+  // 1 + D face tables in this layout follow each other: phi_m, then d phi_m / d xi_e, e < D.
+  static constexpr int NFTAB = 1 + D;
 };
 
 // Per-axis 1D indices of a node: volume node kv = (i*K1 + j)*K1 + k has x = k, y = j, z = i
@@ -193,10 +195,10 @@ __device__ __forceinline__ void t_test(const Real* tab, const int (&q)[3], int d
   }
 }
 
-// Copy the face trace table (global memory, after the 1D block) into LDS at ft.
+// Copy the first ntab face tables (global memory, after the 1D block) into LDS at ft.
 template <class TS, typename Real>
-__device__ __forceinline__ void ft_load(Real* ft, const Real* gtab) {
-  for (int i = threadIdx.x; i < TS::FTAB; i += blockDim.x) ft[i] = gtab[TS::TAB + i];
+__device__ __forceinline__ void ft_load(Real* ft, const Real* gtab, int ntab = 1) {
+  for (int i = threadIdx.x; i < ntab * TS::FTAB; i += blockDim.x) ft[i] = gtab[TS::TAB + i];
 }
 
 // Trace contraction from the face trace table: acc[v] (nodes x cells) = FT(face, node, mode) . B.
@@ -306,10 +308,10 @@ template <int D> struct IsHLLC<FormEuler<D, true>> : std::true_type {};
 
 constexpr int TWAVES = 4;
 
-// kt_forward reads its value traces from the face trace table when the form's faces need no
-// gradients (the Euler fluxes).
-template <class Form>
-constexpr bool forward_face_table() { return !Form::FACE_GRADU && !Form::FACE_TESTG; }
+// Face tables kt_forward loads: the trace table, and for forms whose faces use gradients (SIP,
+// advection-diffusion) the D reference-derivative tables as well.
+template <class Form, int D>
+constexpr int forward_face_tables() { return (Form::FACE_GRADU || Form::FACE_TESTG) ? 1 + D : 1; }
 
 // Waves per SIMD the residual/JVP (kt_forward) and linear-form adjoint (kt_adjoint) kernels
 // are compiled for.  Their LDS use is small, so registers alone set the occupancy: FP32, and
@@ -342,11 +344,9 @@ __global__ void __launch_bounds__(64 * TWAVES, (tensor_min_waves<Real, P>())) kt
   const int cell0 = (xcd_block(a.nxcd) * TWAVES + wave) * 16;
   Real* stU = tab + ((TS::TAB + 1) & ~1) + wave * (NARR * 16 * STRIDE);
   Real* stX = stU + 16 * STRIDE;
-  // face trace table: only for forms whose faces use values alone (the gradient traces of the
-  // other forms outweigh the value traces it saves, and its load costs short workgroups more)
-  constexpr bool FT = forward_face_table<Form>();
+  // face tables: traces, and the reference-derivative traces where the faces use gradients
   Real* ft = tab + ((TS::TAB + 1) & ~1) + TWAVES * NARR * 16 * STRIDE;
-  if constexpr (FT) ft_load<TS>(ft, a.tab);
+  ft_load<TS>(ft, a.tab, forward_face_tables<Form, D>());
   stage16<Real, NT>(stU, STRIDE, a.U, cell0, a.J, lane);
   if (TAN) stage16<Real, NT>(stX, STRIDE, a.X, cell0, a.J, lane);
   __syncthreads();
@@ -459,33 +459,25 @@ __global__ void __launch_bounds__(64 * TWAVES, (tensor_min_waves<Real, P>())) kt
       int qo[3], qn[3];
       face_axes<D, K1>(f, nodeE < TS::KE ? nodeE : 0, qo);
       face_axes<D, K1>(fo, nodeE < TS::KE ? nodeE : 0, qn);
-      // value traces from the face trace table (FT) or the 1D tables; gradient traces from the 1D tables
+      // trace operands FT(f, node, mode) and FT(fo, node, mode) of the tile's nodes; the
+      // derivative table of direction e follows at (1 + e) FTAB
       const Real* fto = ft + (f * TS::FROWS + t * 16 + col) * TS::FS + grp;
       const Real* ftn = ft + (fo * TS::FROWS + t * 16 + col) * TS::FS + grp;
       acc_t uo[V], un[V], duo[V], dun[V], guo[D][V], gun[D][V], dguo[D][V], dgun[D][V];
-      if constexpr (FT) {
-        t_eval_ft<TS, Real, V>(fto, Bu, uo);
-        t_eval_ft<TS, Real, V>(ftn, Bnu, un);
-        if (TAN) {
-          t_eval_ft<TS, Real, V>(fto, Bx, duo);
-          t_eval_ft<TS, Real, V>(ftn, Bnx, dun);
-        }
-      } else {
-        t_eval<TS, Real, V>(tab, qo, -1, grp, Bu, uo);
-        t_eval<TS, Real, V>(tab, qn, -1, grp, Bnu, un);
-        if (TAN) {
-          t_eval<TS, Real, V>(tab, qo, -1, grp, Bx, duo);
-          t_eval<TS, Real, V>(tab, qn, -1, grp, Bnx, dun);
-        }
+      t_eval_ft<TS, Real, V>(fto, Bu, uo);
+      t_eval_ft<TS, Real, V>(ftn, Bnu, un);
+      if (TAN) {
+        t_eval_ft<TS, Real, V>(fto, Bx, duo);
+        t_eval_ft<TS, Real, V>(ftn, Bnx, dun);
       }
       if (Form::FACE_GRADU) {
 #pragma unroll
         for (int e = 0; e < D; e++) {
-          t_eval<TS, Real, V>(tab, qo, e, grp, Bu, guo[e]);
-          t_eval<TS, Real, V>(tab, qn, e, grp, Bnu, gun[e]);
+          t_eval_ft<TS, Real, V>(fto + (1 + e) * TS::FTAB, Bu, guo[e]);
+          t_eval_ft<TS, Real, V>(ftn + (1 + e) * TS::FTAB, Bnu, gun[e]);
           if (TAN) {
-            t_eval<TS, Real, V>(tab, qo, e, grp, Bx, dguo[e]);
-            t_eval<TS, Real, V>(tab, qn, e, grp, Bnx, dgun[e]);
+            t_eval_ft<TS, Real, V>(fto + (1 + e) * TS::FTAB, Bx, dguo[e]);
+            t_eval_ft<TS, Real, V>(ftn + (1 + e) * TS::FTAB, Bnx, dgun[e]);
           }
         }
       }
@@ -494,8 +486,6 @@ __global__ void __launch_bounds__(64 * TWAVES, (tensor_min_waves<Real, P>())) kt
         if (t * 16 + MM<Real>::row(0, r) >= TS::KE) continue;   // all 4 nodes of this step are padding
         const int node = t * 16 + MM<Real>::row(grp, r);
         const bool pad = node >= TS::KE;
-        int q[3];
-        face_axes<D, K1>(f, pad ? 0 : node, q);
         const Real* nrow = ft + (f * TS::FROWS + node) * TS::FS;
         const Real wq = pad ? Real(0) : face_weight<TS>(tab, node) * sg * sc;
         T Uo[V], Un[V], Go[V][D], Gn[V][D];
@@ -539,8 +529,7 @@ __global__ void __launch_bounds__(64 * TWAVES, (tensor_min_waves<Real, P>())) kt
           if constexpr (TAN) s = fo_[v].d[0]; else s = fo_[v];
           g[v] = pad ? Real(0) : s * wq;
         }
-        if constexpr (FT) t_test_ft<TS, Real, V>(nrow, ftc, g, racc);
-        else t_test<TS, Real, V>(tab, q, -1, col, g, racc);
+        t_test_ft<TS, Real, V>(nrow, ftc, g, racc);
         if (Form::FACE_TESTG) {
 #pragma unroll
           for (int e = 0; e < D; e++) {
@@ -553,7 +542,7 @@ __global__ void __launch_bounds__(64 * TWAVES, (tensor_min_waves<Real, P>())) kt
               }
               g[v] = pad ? Real(0) : s * wq;
             }
-            t_test<TS, Real, V>(tab, q, e, col, g, racc);
+            t_test_ft<TS, Real, V>(nrow + (1 + e) * TS::FTAB, ftc, g, racc);
           }
         }
       }
@@ -906,11 +895,11 @@ constexpr size_t kt_lds_bytes(int narrays) {
   typedef TShape<D, P, K1> TS;
   return (size_t)(((TS::TAB + 1) & ~1) + TWAVES * narrays * 16 * ((V * TS::N) | 1)) * sizeof(Real);
 }
-// kt_forward: the same, followed by the face trace table for the forms that use it
+// kt_forward: the same, followed by its face tables
 template <class Form, int D, int P, int K1, typename Real>
 constexpr size_t kt_forward_lds_bytes(int narrays) {
   return kt_lds_bytes<D, P, K1, Form::V, Real>(narrays) +
-         (forward_face_table<Form>() ? (size_t)TShape<D, P, K1>::FTAB * sizeof(Real) : 0);
+         (size_t)forward_face_tables<Form, D>() * TShape<D, P, K1>::FTAB * sizeof(Real);
 }
 
 }  // namespace bosss

@@ -19,7 +19,7 @@ struct KernelSet {
   void (*k32[NVARIANT])(KArgs<float>);
   // tensor-table kernels (dg_tensor.hpp): 1D tables in LDS, own coefficients staged in LDS
   int P, ttab_size, t_o_l, t_o_dl, t_o_w, t_o_code, t_k2;
-  int t_o_ft, t_frows, t_fs;   // face trace table: offset (after the 1D block), rows per face, row stride
+  int t_o_ft, t_frows, t_fs, t_ftab;   // face tables: offset (after the 1D block), rows per face, row stride, size of one
   void (*kt64[NVARIANT])(TArgs<double>);
   void (*kt32[NVARIANT])(TArgs<float>);
   size_t ktlds64[NVARIANT], ktlds32[NVARIANT];
@@ -70,9 +70,9 @@ KernelSet make_kernel_set(int kind) {
   constexpr int P = degree_from_modes(D, N);
   static_assert(tdeg_count(D, P) == N, "N must be a full total-degree basis");
   typedef TShape<D, P, K1> TSh;
-  k.P = P; k.ttab_size = TSh::TAB + TSh::FTAB; k.t_o_l = TSh::O_L; k.t_o_dl = TSh::O_DL; k.t_o_w = TSh::O_W; k.t_o_code = TSh::O_CODE;
+  k.P = P; k.ttab_size = TSh::TAB + TSh::NFTAB * TSh::FTAB; k.t_o_l = TSh::O_L; k.t_o_dl = TSh::O_DL; k.t_o_w = TSh::O_W; k.t_o_code = TSh::O_CODE;
   k.t_k2 = TSh::K2;
-  k.t_o_ft = TSh::TAB; k.t_frows = TSh::FROWS; k.t_fs = TSh::FS;
+  k.t_o_ft = TSh::TAB; k.t_frows = TSh::FROWS; k.t_fs = TSh::FS; k.t_ftab = TSh::FTAB;
   k.kt64[KernelSet::RES] = kt_forward<Form, D, P, K1, double, MODE_RES>;
   k.ktlds64[KernelSet::RES] = kt_forward_lds_bytes<Form, D, P, K1, double>(1);
   set_tensor_linearisations<Form, D, P, K1, double>(k.kt64, k.ktlds64);
