@@ -308,10 +308,10 @@ template <int D> struct IsHLLC<FormEuler<D, true>> : std::true_type {};
 
 constexpr int TWAVES = 4;
 
-// Face tables kt_forward loads: the trace table, and for forms whose faces use gradients (SIP,
-// advection-diffusion) the D reference-derivative tables as well.
+// Face tables kt_forward and kt_adjoint load: the trace table, and for forms whose faces use
+// gradients (SIP, advection-diffusion) the D reference-derivative tables as well.
 template <class Form, int D>
-constexpr int forward_face_tables() { return (Form::FACE_GRADU || Form::FACE_TESTG) ? 1 + D : 1; }
+constexpr int form_face_tables() { return (Form::FACE_GRADU || Form::FACE_TESTG) ? 1 + D : 1; }
 
 // Waves per SIMD the residual/JVP (kt_forward) and linear-form adjoint (kt_adjoint) kernels
 // are compiled for.  Their LDS use is small, so registers alone set the occupancy: FP32, and
@@ -346,7 +346,7 @@ __global__ void __launch_bounds__(64 * TWAVES, (tensor_min_waves<Real, P>())) kt
   Real* stX = stU + 16 * STRIDE;
   // face tables: traces, and the reference-derivative traces where the faces use gradients
   Real* ft = tab + ((TS::TAB + 1) & ~1) + TWAVES * NARR * 16 * STRIDE;
-  ft_load<TS>(ft, a.tab, forward_face_tables<Form, D>());
+  ft_load<TS>(ft, a.tab, form_face_tables<Form, D>());
   stage16<Real, NT>(stU, STRIDE, a.U, cell0, a.J, lane);
   if (TAN) stage16<Real, NT>(stX, STRIDE, a.X, cell0, a.J, lane);
   __syncthreads();
@@ -585,9 +585,12 @@ __global__ void __launch_bounds__(64 * TWAVES, (adjoint_min_waves<Real, P, FUSED
   const int cell0 = (xcd_block(a.nxcd) * TWAVES + wave) * 16;
   Real* stU = tab + ((TS::TAB + 1) & ~1) + wave * (2 * 16 * STRIDE);
   Real* stW = stU + 16 * STRIDE;
+  Real* ft = tab + ((TS::TAB + 1) & ~1) + TWAVES * 2 * 16 * STRIDE;   // face tables
+  ft_load<TS>(ft, a.tab, form_face_tables<Form, D>());
   stage16<Real, NT>(stU, STRIDE, a.U, cell0, a.J, lane);
   stage16<Real, NT>(stW, STRIDE, a.X, cell0, a.J, lane);
   __syncthreads();
+  const FtCols<TS, Real, V> ftc(col);
 
   const int cell = cell0 + col;
   const bool valid = cell < a.J;
@@ -740,25 +743,24 @@ __global__ void __launch_bounds__(64 * TWAVES, (adjoint_min_waves<Real, P, FUSED
     auto Bnw = [&](int v, int kk) { int m = 4 * kk + grp; return nbW[v * N + (m < N ? m : N - 1)] * ((has_nb && m < N) ? Real(1) : Real(0)); };
 
     for (int t = 0; t < TS::TE; t++) {
-      const int nodeE = t * 16 + col;
-      int qo[3], qn[3];
-      face_axes<D, K1>(f, nodeE < TS::KE ? nodeE : 0, qo);
-      face_axes<D, K1>(fo, nodeE < TS::KE ? nodeE : 0, qn);
+      // trace operands from the face tables (derivative table of direction e at (1 + e) FTAB)
+      const Real* fto = ft + (f * TS::FROWS + t * 16 + col) * TS::FS + grp;
+      const Real* ftn = ft + (fo * TS::FROWS + t * 16 + col) * TS::FS + grp;
       acc_t uo[V], un[V];
-      t_eval<TS, Real, V>(tab, qo, -1, grp, Bu, uo);
-      t_eval<TS, Real, V>(tab, qn, -1, grp, Bnu, un);
+      t_eval_ft<TS, Real, V>(fto, Bu, uo);
+      t_eval_ft<TS, Real, V>(ftn, Bnu, un);
       acc_t guo[D][V], gun[D][V], wo[V], wn[V], gwo[D][V], gwn[D][V];
-      t_eval<TS, Real, V>(tab, qo, -1, grp, Bw, wo);
-      t_eval<TS, Real, V>(tab, qn, -1, grp, Bnw, wn);
+      t_eval_ft<TS, Real, V>(fto, Bw, wo);
+      t_eval_ft<TS, Real, V>(ftn, Bnw, wn);
 #pragma unroll
       for (int e = 0; e < D; e++) {
         if (Form::FACE_GRADU) {
-          t_eval<TS, Real, V>(tab, qo, e, grp, Bu, guo[e]);
-          t_eval<TS, Real, V>(tab, qn, e, grp, Bnu, gun[e]);
+          t_eval_ft<TS, Real, V>(fto + (1 + e) * TS::FTAB, Bu, guo[e]);
+          t_eval_ft<TS, Real, V>(ftn + (1 + e) * TS::FTAB, Bnu, gun[e]);
         }
         if (Form::FACE_TESTG) {
-          t_eval<TS, Real, V>(tab, qo, e, grp, Bw, gwo[e]);
-          t_eval<TS, Real, V>(tab, qn, e, grp, Bnw, gwn[e]);
+          t_eval_ft<TS, Real, V>(fto + (1 + e) * TS::FTAB, Bw, gwo[e]);
+          t_eval_ft<TS, Real, V>(ftn + (1 + e) * TS::FTAB, Bnw, gwn[e]);
         }
       }
 #pragma unroll
@@ -766,8 +768,7 @@ __global__ void __launch_bounds__(64 * TWAVES, (adjoint_min_waves<Real, P, FUSED
         if (t * 16 + MM<Real>::row(0, r) >= TS::KE) continue;   // all 4 nodes of this step are padding
         const int node = t * 16 + MM<Real>::row(grp, r);
         const bool pad = node >= TS::KE;
-        int q[3];
-        face_axes<D, K1>(f, pad ? 0 : node, q);
+        const Real* nrow = ft + (f * TS::FROWS + node) * TS::FS;
         const Real wsg = pad ? Real(0) : face_weight<TS>(tab, node) * sg;
         TD Uo[V], Un[V], Go[V][D], Gn[V][D];
 #pragma unroll
@@ -841,7 +842,7 @@ __global__ void __launch_bounds__(64 * TWAVES, (adjoint_min_waves<Real, P, FUSED
           const Real wq = wsg * sc;
 #pragma unroll
           for (int v = 0; v < V; v++) g[v] = pad ? Real(0) : fo_[v].v * wq;
-          t_test<TS, Real, V>(tab, q, -1, col, g, racc);
+          t_test_ft<TS, Real, V>(nrow, ftc, g, racc);
           if (Form::FACE_TESTG) {
 #pragma unroll
             for (int e = 0; e < D; e++) {
@@ -852,13 +853,13 @@ __global__ void __launch_bounds__(64 * TWAVES, (adjoint_min_waves<Real, P, FUSED
                 for (int d = 0; d < D; d++) s += go_[v][d].v * iJ[e][d];
                 g[v] = pad ? Real(0) : s * wq;
               }
-              t_test<TS, Real, V>(tab, q, e, col, g, racc);
+              t_test_ft<TS, Real, V>(nrow + (1 + e) * TS::FTAB, ftc, g, racc);
             }
           }
         }
 #pragma unroll
         for (int v = 0; v < V; v++) g[v] = pad ? Real(0) : ub[v] * sc;
-        t_test<TS, Real, V>(tab, q, -1, col, g, aacc);
+        t_test_ft<TS, Real, V>(nrow, ftc, g, aacc);
         if (Form::FACE_GRADU) {
 #pragma unroll
           for (int e = 0; e < D; e++) {
@@ -869,7 +870,7 @@ __global__ void __launch_bounds__(64 * TWAVES, (adjoint_min_waves<Real, P, FUSED
               for (int d = 0; d < D; d++) s += gb[v][d] * iJ[e][d];
               g[v] = pad ? Real(0) : s * sc;
             }
-            t_test<TS, Real, V>(tab, q, e, col, g, aacc);
+            t_test_ft<TS, Real, V>(nrow + (1 + e) * TS::FTAB, ftc, g, aacc);
           }
         }
       }
@@ -895,11 +896,11 @@ constexpr size_t kt_lds_bytes(int narrays) {
   typedef TShape<D, P, K1> TS;
   return (size_t)(((TS::TAB + 1) & ~1) + TWAVES * narrays * 16 * ((V * TS::N) | 1)) * sizeof(Real);
 }
-// kt_forward: the same, followed by its face tables
+// kt_forward and kt_adjoint: the same, followed by their face tables
 template <class Form, int D, int P, int K1, typename Real>
 constexpr size_t kt_forward_lds_bytes(int narrays) {
   return kt_lds_bytes<D, P, K1, Form::V, Real>(narrays) +
-         (size_t)forward_face_tables<Form, D>() * TShape<D, P, K1>::FTAB * sizeof(Real);
+         (size_t)form_face_tables<Form, D>() * TShape<D, P, K1>::FTAB * sizeof(Real);
 }
 
 }  // namespace bosss

@@ -48,7 +48,7 @@ void set_tensor_linearisations(void (**kt)(TArgs<Real>), size_t* lds) {
     kt[KernelSet::VJP] = kt_adjoint<Form, D, P, K1, Real, false>;
     kt[KernelSet::RES_VJP] = kt_adjoint<Form, D, P, K1, Real, true>;
     lds[KernelSet::JVP] = kt_forward_lds_bytes<Form, D, P, K1, Real>(2);
-    lds[KernelSet::VJP] = lds[KernelSet::RES_VJP] = kt_lds_bytes<D, P, K1, Form::V, Real>(2);
+    lds[KernelSet::VJP] = lds[KernelSet::RES_VJP] = kt_forward_lds_bytes<Form, D, P, K1, Real>(2);
   }
 }
 
