@@ -86,6 +86,10 @@ CASES = [
     ("euler_hllc", 2, 5, 2, True),
     ("euler_hllc", 3, 9, 2, True),  # 729 cells = 12 workgroups: XCD block renumbering with a remainder
     ("sip", 2, 30, 2, False),       # 900 cells = 15 workgroups
+    ("sip", 2, 5, 1, True),         # the remaining compiled shapes: SIP 2D p = 1 and 3, 3D p = 1,
+    ("sip", 2, 4, 3, True),
+    ("sip", 3, 3, 1, False),
+    ("advdiff", 2, 5, 2, True),     # upwind + SIP 2D p = 2
 ]
 
 
