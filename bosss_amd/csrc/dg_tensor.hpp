@@ -455,10 +455,6 @@ __global__ void __launch_bounds__(64 * TWAVES, (tensor_min_waves<Real, P>())) kt
     auto Bnx = [&](int v, int kk) { int m = 4 * kk + grp; return nbX[v * N + (m < N ? m : N - 1)] * ((has_nb && !a.own_only && m < N) ? scn : Real(0)); };
 
     for (int t = 0; t < TS::TE; t++) {
-      const int nodeE = t * 16 + col;
-      int qo[3], qn[3];
-      face_axes<D, K1>(f, nodeE < TS::KE ? nodeE : 0, qo);
-      face_axes<D, K1>(fo, nodeE < TS::KE ? nodeE : 0, qn);
       // trace operands FT(f, node, mode) and FT(fo, node, mode) of the tile's nodes; the
       // derivative table of direction e follows at (1 + e) FTAB
       const Real* fto = ft + (f * TS::FROWS + t * 16 + col) * TS::FS + grp;
