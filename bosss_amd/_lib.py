@@ -63,6 +63,7 @@ SIGNATURES = {
     "bosss_hip_op_destroy": (ctypes.c_int, [_vp]),
     "bosss_hip_op_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32),
                                          ctypes.POINTER(ctypes.c_int32)]),
+    "bosss_hip_op_kernel_family": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "bosss_hip_evaluate": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]),
     "bosss_hip_jvp": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
     "bosss_hip_vjp": (ctypes.c_int, [_vp, _dp, _dp, _dp]),

@@ -448,6 +448,12 @@ int bosss_hip_op_info(const bosss_op* op, int64_t* ndof, int32_t* nvars, int32_t
   return BOSSS_OK;
 }
 
+int bosss_hip_op_kernel_family(const bosss_op* op, int32_t* tensor_tables) {
+  if (!op || !tensor_tables) return fail(BOSSS_ERR_ARG, "NULL argument");
+  *tensor_tables = op->tensor ? 1 : 0;
+  return BOSSS_OK;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------- launching

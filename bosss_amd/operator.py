@@ -103,6 +103,13 @@ class DifferentialOperator:
         check(L.bosss_hip_op_info(oh, ctypes.byref(ndof), ctypes.byref(nv), ctypes.byref(nm)))
         self.ndof, self.nvars, self.N = ndof.value, nv.value, nm.value
 
+    def uses_tensor_tables(self):
+        """True if the operator runs the tensor-table kernels (the fast path), False if the
+        generic full-table kernels (bosss_hip_op_kernel_family)."""
+        t = ctypes.c_int32()
+        check(lib().bosss_hip_op_kernel_family(self.h, ctypes.byref(t)))
+        return bool(t.value)
+
     def close(self):
         if getattr(self, "h", None):
             lib().bosss_hip_op_destroy(self.h)

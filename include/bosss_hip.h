@@ -109,6 +109,11 @@ int bosss_hip_op_create(bosss_ctx* ctx, const bosss_grid* grid, const bosss_tabl
 int bosss_hip_op_destroy(bosss_op* op);
 /* ndof = J*V*N; nvars = V; nmodes = N. */
 int bosss_hip_op_info(const bosss_op* op, int64_t* ndof, int32_t* nvars, int32_t* nmodes);
+/* *tensor_tables = 1 if the operator runs the tensor-table kernels (1D factors, face tables;
+ * built and checked against the uploaded tables at creation), 0 if it runs the generic
+ * full-table kernels (BOSSS_HIP_GENERIC_TABLES=1, or tables that are not a tensor product).
+ * No counterpart in the reference: lets a host or a test confirm which kernel family runs. */
+int bosss_hip_op_kernel_family(const bosss_op* op, int32_t* tensor_tables);
 
 /* ---- residual (replaces EvaluatorNonLin.Evaluate, DifferentialOperator.cs:1584-1661;
  *      IEvaluator.cs:81-97):  R <- beta*R + alpha*Op(U).  Host pointers, synchronous. ---- */

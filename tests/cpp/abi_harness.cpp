@@ -31,7 +31,7 @@
 // ---------------------------------------------------------------- by-name binding
 #define BOSSS_API(X)                                                                                                        \
   X(last_error) X(buffer_create) X(buffer_destroy) X(buffer_upload) X(buffer_download) X(ctx_create) X(ctx_destroy) X(sync)   \
-  X(ctx_stream) X(grid_upload) X(grid_destroy) X(op_create) X(op_destroy) X(op_info) X(evaluate) X(jvp) X(vjp)                \
+  X(ctx_stream) X(grid_upload) X(grid_destroy) X(op_create) X(op_destroy) X(op_info) X(op_kernel_family) X(evaluate) X(jvp) X(vjp)                \
   X(op_set_check_finite) X(evaluate_dev) X(jvp_dev) X(vjp_dev) X(evaluate_vjp_dev) X(op_matrix_layout) X(op_assemble_dev) X(op_diag_blocks_dev)    \
   X(bell_spmv_dev) X(op_assemble) X(bell_spmv) X(op_set_spmv_tile) X(op_spmv_tiles) X(block_lu_dev) X(block_lu_noalloc_dev) X(block_solve_dev)       \
   X(block_gemv_dev) X(block_lu) X(block_solve) X(block_gemv) X(block_last_error) X(xdg_create) X(xdg_destroy) X(xdg_info)     \

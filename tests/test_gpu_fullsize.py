@@ -41,6 +41,7 @@ def case():
     n, p = N_CELLS, 3
     g = CartesianGrid((n,) * 3, periodic=[True] * 3)
     op = DifferentialOperator(g, "euler_hllc", p, gamma=1.4)
+    assert op.uses_tensor_tables()   # the headline kernels, not the generic-table fallback
     ref = RefModel(3, (n,) * 3, p, default_quad_order("euler_hllc", p), periodic=[1, 1, 1])
     prm = params_vector(1.3, 1.0, (1.0, 0.5, 0.25), 1.4)
     U = euler_state(g, op.N, seed=0)

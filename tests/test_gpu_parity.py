@@ -94,9 +94,12 @@ CASES = [
 
 
 @pytest.mark.parametrize("form,D,n,p,per", CASES)
-def test_residual_matches_oracle(form, D, n, p, per):
-    say("case residual", form, D, n, p, per)
+def test_residual_matches_oracle(form, D, n, p, per, table_path):
+    say("case residual", form, D, n, p, per, table_path)
     op, ref, prm, U = make_case(form, D, n, p, per)
+    # the family the case asked for is the one that runs (a table check that failed at creation
+    # would otherwise send the tensor cases through the generic kernels unnoticed)
+    assert op.uses_tensor_tables() == (table_path == "tensor")
     R = op.evaluate(U)
     Rref = ref.evaluate(KIND[form], prm, U)
     pf, nw = rel_err(R, Rref, op.nvars, op.N)
