@@ -314,8 +314,9 @@ template <class Form, int D>
 constexpr int form_face_tables() { return (Form::FACE_GRADU || Form::FACE_TESTG) ? 1 + D : 1; }
 
 // Waves per SIMD the residual/JVP (kt_forward) and linear-form adjoint (kt_adjoint) kernels
-// are compiled for.  Their LDS use is small, so registers alone set the occupancy: FP32, and
-// FP64 up to p = 2, fit two waves with little or no spilling and gain 13-28 % from the
+// are compiled for.  Their LDS (1D tables, staged coefficients, face tables; at most about 90 KB
+// per workgroup, for the FP64 p = 3 scalar forms at one wave) fits every occupancy below, so
+// registers set it: FP32, and FP64 up to p = 2, fit two waves with little or no spilling and gain 13-28 % from the
 // overlap.  The FP64 p = 3 kernels, and the FP64 fused residual + adjoint at p = 2, would
 // spill 260-300 B per lane and run slower (measured), so they stay at one wave.
 template <typename Real, int P>
