@@ -98,3 +98,24 @@ def test_device_wrappers_refuse_mismatched_tensors():
     with pytest.raises(ValueError):
         bb.solve(M, None, v)
     torch.cuda.synchronize()
+
+
+def test_op_create_refuses_what_no_kernel_covers():
+    """bosss_hip_op_create refuses with BOSSS_ERR_UNSUPPORTED, and a message naming the reason, an
+    Euler operator on a grid with boundary edges (the Euler kernels carry no boundary flux; every
+    Euler workload here is periodic) and a degree that no kernel is compiled for; the same grid,
+    periodic in every direction, is accepted."""
+    from bosss_amd._lib import BosssError
+    from bosss_amd.grid import CartesianGrid
+    from bosss_amd.operator import DifferentialOperator
+    print("case op_create refusals", flush=True)
+    for form in ("euler_lf", "euler_hllc"):
+        with pytest.raises(BosssError, match="periodic grids only") as e:
+            DifferentialOperator(CartesianGrid((3, 3, 3), periodic=[True, True, False]), form, 2)
+        assert e.value.code == -3
+    with pytest.raises(BosssError, match="no kernel") as e:
+        DifferentialOperator(CartesianGrid((2, 2, 2), periodic=[True] * 3), "euler_hllc", 4)
+    assert e.value.code == -3
+    op = DifferentialOperator(CartesianGrid((3, 3, 3), periodic=[True] * 3), "euler_hllc", 2)
+    assert op.uses_tensor_tables()
+    op.close()
