@@ -25,7 +25,7 @@
 //    sc_own*W_own.trace - sc_nb*W_nb.trace is needed) and uses the hand-written reverse
 //    sweeps of euler_adjoint.hpp;
 //  * workgroups are renumbered so that each XCD (own L2) walks one contiguous range
-//    of cells (xcd_block).
+//    of cells, for the Euler JVP and VJP kernels on large planes plane-fastest (xcd_block).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -287,6 +287,7 @@ struct TArgs {
   Real* R;
   Real* out;
   int nxcd;             // XCDs the workgroups are dealt over (1: no remapping)
+  int plane_blocks;     // workgroups per plane of the slowest grid direction (0: range order)
   int own_only;         // JVP: neighbour tangents taken as zero (J(U)_jj X_j, the diagonal blocks)
 };
 
@@ -294,10 +295,21 @@ struct TArgs {
 // round-robin over the XCDs, each with its own L2; renumbering the blocks so that
 // XCD x walks one contiguous range of cells keeps a cell's face neighbours (the
 // adjacent cell blocks) in the same L2.  Bijective for any grid size.
-__device__ __forceinline__ int xcd_block(int nxcd) {
+//
+// With plane_blocks = W > 0 and a range of whole planes (per = g W blocks), XCD x walks its
+// g planes plane-fastest: its i-th workgroup takes block (i mod g) W + i / g of the range, so
+// the blocks across the slowest direction (one plane apart, the x-neighbours of a Cartesian
+// grid numbered x-slowest) run next to each other instead of W workgroups apart, and a cell's
+// coefficients are still in L2 when its neighbour in the next plane reads them.  A bijection
+// of the range onto itself: mixed-radix digits (i mod g, i / g) of i in [0, g W).
+__device__ __forceinline__ int xcd_block(int nxcd, int plane_blocks) {
   const int b = blockIdx.x, nb = gridDim.x;
   if (nxcd <= 1) return b;
   const int per = nb / nxcd, rem = nb % nxcd, x = b % nxcd, i = b / nxcd;
+  if (plane_blocks > 0 && rem == 0 && per >= plane_blocks && per % plane_blocks == 0) {
+    const int g = per / plane_blocks;
+    return x * per + (i % g) * plane_blocks + i / g;
+  }
   return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
 }
 
@@ -342,7 +354,7 @@ __global__ void __launch_bounds__(64 * TWAVES, (tensor_min_waves<Real, P>())) kt
   for (int i = threadIdx.x; i < TS::TAB; i += blockDim.x) tab[i] = a.tab[i];
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int col = lane & 15, grp = lane >> 4;
-  const int cell0 = (xcd_block(a.nxcd) * TWAVES + wave) * 16;
+  const int cell0 = (xcd_block(a.nxcd, a.plane_blocks) * TWAVES + wave) * 16;
   Real* stU = tab + ((TS::TAB + 1) & ~1) + wave * (NARR * 16 * STRIDE);
   Real* stX = stU + 16 * STRIDE;
   // face tables: traces, and the reference-derivative traces where the faces use gradients
@@ -579,7 +591,7 @@ __global__ void __launch_bounds__(64 * TWAVES, (adjoint_min_waves<Real, P, FUSED
   for (int i = threadIdx.x; i < TS::TAB; i += blockDim.x) tab[i] = a.tab[i];
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int col = lane & 15, grp = lane >> 4;
-  const int cell0 = (xcd_block(a.nxcd) * TWAVES + wave) * 16;
+  const int cell0 = (xcd_block(a.nxcd, a.plane_blocks) * TWAVES + wave) * 16;
   Real* stU = tab + ((TS::TAB + 1) & ~1) + wave * (2 * 16 * STRIDE);
   Real* stW = stU + 16 * STRIDE;
   Real* ft = tab + ((TS::TAB + 1) & ~1) + TWAVES * 2 * 16 * STRIDE;   // face tables
@@ -944,7 +956,7 @@ __global__ void __launch_bounds__(64 * TWAVES, (vjp_euler_min_waves<Real, P, FUS
   const FtCols<TS, Real, V> ftc(col);
   auto slot = [&](int arr, int v, int r) -> Real& { return buf[((arr * V + v) * 4 + r) * 64 + lane]; };
 
-  const int cell = (xcd_block(a.nxcd) * TWAVES + wave) * 16 + col;
+  const int cell = (xcd_block(a.nxcd, a.plane_blocks) * TWAVES + wave) * 16 + col;
   const bool valid = cell < a.J;
   const int cj = valid ? cell : 0;
   const Real sc = Real(a.scaling[cj]);
@@ -1197,7 +1209,7 @@ __global__ void __launch_bounds__(64 * TWAVES) __attribute__((amdgpu_waves_per_e
   const FtCols<TS, Real, V> ftc(col);
   auto slot = [&](int arr, int v, int r) -> Real& { return buf[((arr * V + v) * 4 + r) * 64 + lane]; };
 
-  const int cell = (xcd_block(a.nxcd) * TWAVES + wave) * 16 + col;
+  const int cell = (xcd_block(a.nxcd, a.plane_blocks) * TWAVES + wave) * 16 + col;
   const bool valid = cell < a.J;
   const int cj = valid ? cell : 0;
   const Real sc = Real(a.scaling[cj]);
